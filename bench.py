@@ -1,0 +1,201 @@
+#!/usr/bin/env python3
+"""Throughput of the Helmholtz solve path on one MI355X.
+
+Workload (the metric's configuration): E = 64^3 = 262,144 hexahedra, N = 7
+(lx1 = 8), every GLL node mapped by x_i <- x_i + 0.1 h sin(2 pi x) sin(2 pi y)
+sin(2 pi z), h1 = 1, h2 = 10, f ~ U(-1,1) per unique node (seeded), Dirichlet
+on all faces, solved by hmholtz (dssum, mask, chktcg1, Jacobi-PCG cggo) in
+FP64 to 1e-8 relative residual.  One step = one full solve with all inputs
+resident in HBM (setup excluded).
+
+value = (unique free DOF x PCG iterations) / wall time, in GDOF*iter/s.
+roofline: the fused axhelm kernel of the CG iteration (p update + operator +
+interior dot), average duration from the library's HIP-event timers.
+cpu_baseline: the oracle's cggo (C, OpenMP over all host threads) on a
+bounded sample of the same workload.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "PCG GDOF·iter/s at E=262144,N=7: FP64 vs mixed @1e-8 rel; %HBM roofline"
+HBM_PEAK_GBS = 8000.0
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def make_rhs(box, glo, ngv, bm1_host=None, seed=2024):
+    from nek5000_amd.mesh import hash_uniform
+
+    uid = box.unique_ids(glo, ngv).reshape(-1)
+    f = 2.0 * hash_uniform(seed, uid) - 1.0
+    return f.reshape(box.nel, box.lx1, box.lx1, box.lx1)
+
+
+def fused_axhelm_bytes_per_node(lx1, h1_field=False, h2_kind="uniform"):
+    """Algorithmic HBM bytes per local node of the fused CG axhelm kernel:
+    reads r, d, p (3 words), g1..g6 (6), bm1 when h2 != 0 (1), h1 when it is
+    a field (1), mult at element-interior nodes (fraction ((N-1)/(N+1))^3);
+    writes p and w (2)."""
+    words = 3 + 6 + 2
+    if h2_kind == "uniform":
+        words += 1
+    elif h2_kind == "field":
+        words += 2
+    if h1_field:
+        words += 1
+    fint = ((lx1 - 2) / lx1) ** 3
+    return 8.0 * (words + fint)
+
+
+def cpu_baseline(nel1d, lx1, deform, target_seconds):
+    """The oracle's hmholtz on E=nel1d^3 with the same operator; bounded by
+    maxit so the run takes about target_seconds."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    from oracle import oracle as O
+    from problems import Problem, box as mkbox
+
+    P = Problem(mkbox(nel1d, lx1, deform=deform))
+    h1 = np.ones(P.shape)
+    h2 = np.full(P.shape, 10.0)
+    rhs = P.bm1 * P.uniform_field(2024)
+    # calibrate on a few iterations
+    t = time.perf_counter()
+    _, _, n_cal, _, _ = P.field.hmholtz(rhs, h1, h2, -1e-30, 5)
+    dt = (time.perf_counter() - t) / max(n_cal, 1)
+    maxit = int(max(10, min(900, target_seconds / max(dt, 1e-6))))
+    t = time.perf_counter()
+    _, _, niter, _, _ = P.field.hmholtz(rhs, h1, h2, -1e-8, maxit)
+    el = time.perf_counter() - t
+    free = P.free_dof()
+    return dict(value=free * niter / el / 1e9, unit="GDOF*iter/s", cores=O.num_threads(),
+                kind="port",
+                sample=(f"oracle hmholtz/cggo (C, OpenMP) on E={nel1d}^3 N={lx1-1} "
+                        f"deformed, h1=1 h2=10, {niter} Jacobi-PCG iterations "
+                        f"({free:.0f} free DOF) in {el:.1f} s"))
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--nel", type=int, default=64, help="elements per direction")
+    ap.add_argument("--lx1", type=int, default=8)
+    ap.add_argument("--tol", type=float, default=1e-8)
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--cpu-nel", type=int, default=16)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py measures one GPU: this build has no multi-GPU path "
+              "(elements would be partitioned across GPUs with a halo exchange; "
+              "see DESIGN.md).", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    from nek5000_amd import _lib, solver
+    from nek5000_amd.mesh import BoxMesh
+
+    solver.init(0)
+    lib = _lib.load()
+    t0 = time.time()
+    box = BoxMesh(args.nel, args.nel, args.nel, args.lx1, deform=0.1)
+    log(f"[bench] mesh E={box.nel} lx1={box.lx1} n={box.n}")
+    F = solver.box_field(box)
+    log(f"[bench] field set up in {time.time() - t0:.1f} s")
+    f = make_rhs(box, F.glo_num, F.ngv)
+    bm1, _ = F.mass()
+    rhs0 = bm1 * torch.as_tensor(f, device="cuda")
+    del f
+    h1 = torch.ones_like(rhs0)
+    h2 = torch.full_like(rhs0, 10.0)
+    free_dof = float(torch.sum(F.mask * F.mult))
+    log(f"[bench] free unique DOF = {free_dof:.0f}")
+
+    def solve(r):
+        u = torch.zeros_like(rhs0)
+        return F.hmholtz("TEMP", u, r, h1, h2, F.mask, F.mult, -args.tol, 900)
+
+    for w in range(args.warmup):
+        out = solve(rhs0.clone())
+        log(f"[bench] warmup {w}: niter={out['niter']}")
+    rhss = [rhs0.clone() for _ in range(args.steps)]
+    lib.nkh_timers_enable(1)
+    lib.nkh_timers_reset()
+    torch.cuda.synchronize()
+    t_start = time.perf_counter()
+    iters = []
+    for k in range(args.steps):
+        out = solve(rhss[k])
+        iters.append(out["niter"])
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t_start
+    lib.nkh_timers_enable(0)
+    log(f"[bench] timed {args.steps} solves: iters={iters} in {elapsed:.3f} s")
+
+    import ctypes
+
+    sec = ctypes.c_double()
+    calls = ctypes.c_int64()
+    lib.nkh_timers_read(b"axhm", ctypes.byref(sec), ctypes.byref(calls))
+    t_ax = sec.value / max(calls.value, 1)
+    lib.nkh_timers_read(b"dsum", ctypes.byref(sec), ctypes.byref(calls))
+    t_gs = sec.value / max(calls.value, 1)
+    bpn = fused_axhelm_bytes_per_node(args.lx1)
+    ax_bytes = bpn * box.n
+    achieved = ax_bytes / t_ax / 1e9
+    total_iters = sum(iters)
+    value = free_dof * total_iters / elapsed / 1e9
+
+    cpu = None
+    if not args.no_cpu:
+        log("[bench] cpu baseline ...")
+        cpu = cpu_baseline(args.cpu_nel, args.lx1, 0.1, args.cpu_seconds)
+        log(f"[bench] cpu baseline {cpu}")
+
+    line = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "GDOF*iter/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": elapsed / args.steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {
+            "workload": (f"hmholtz E={args.nel}^3={box.nel} N={args.lx1 - 1} deformed box "
+                         f"(0.1h sin map), h1=1 h2=10, f~U(-1,1), Dirichlet, "
+                         f"Jacobi-PCG FP64 to {args.tol:g} relative"),
+            "nel": box.nel, "lx1": args.lx1, "preconditioner": "jacobi", "precision": "fp64",
+            "free_dof": free_dof, "iterations_per_solve": iters,
+            "axhelm_ms": t_ax * 1e3, "gs_ms": t_gs * 1e3,
+            "ms_per_iteration": elapsed / total_iters * 1e3,
+        },
+        "roofline": {
+            "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+            "kernel": "k_axhelm<8,1,false,1> (CG-fused axhelm)",
+            "bytes_per_launch": ax_bytes,
+        },
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

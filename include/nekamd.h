@@ -1,0 +1,151 @@
+/*
+ * nekamd.h -- C ABI of the MI355X implementation of Nek5000's
+ * Helmholtz/Poisson solve path (axhelm, dssum/gs_op, cggo, hmholtz).
+ *
+ * Conventions
+ *   - Every entry point returns NKH_OK (0) or a positive error code; the text
+ *     of the last error is available from nkh_last_error().  Nek's Fortran
+ *     shims turn a nonzero code into exitti(), keeping Nek's abort convention
+ *     (comm_mpi.f:550-637).
+ *   - Fields are Nek's (lx1,ly1,lz1,nel) column-major REAL*8 arrays: node
+ *     (i,j,k) of element e at i + lx1*(j + lx1*(k + lx1*e)).
+ *   - nkh_* entry points take HOST pointers (what Nek's Fortran passes by
+ *     reference); nkd_* entry points take DEVICE pointers to arrays already
+ *     resident in HBM.  Both run on the library's current HIP stream and
+ *     return after the result is complete (nkh_*) or enqueued (nkd_*).
+ *   - A "mesh" is the integer handle of one element set with its geometry and
+ *     gather-scatter map: Nek's imsh=1 (nelv, velocity) or imsh=2 (nelt,
+ *     temperature) meshes map to two handles.
+ */
+#ifndef NEKAMD_H
+#define NEKAMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define NKH_OK 0
+#define NKH_ERR_ARG 1         /* invalid argument or size */
+#define NKH_ERR_HIP 2         /* HIP runtime error */
+#define NKH_ERR_STATE 3       /* call out of order (no init, no geometry, ...) */
+#define NKH_ERR_NOMEM 4       /* device allocation failed */
+#define NKH_ERR_UNSUPPORTED 5 /* valid in Nek, not implemented here (2-D, axisymmetric) */
+
+/* Solver knobs Nek reads from its commons inside hmholtz/cggo. */
+typedef struct {
+  int istep;       /* /TSTEP/ istep: cggo may stop at iteration 1 when istep<=5 (hmholtz.f:778) */
+  int param18;     /* param(18): 1 -> plain residual norm, else B^-1-weighted (hmholtz.f:755-759) */
+  double param22;  /* param(22): chktcg1 gate (hmholtz.f:54) and relative tol when <0 (:764) */
+  double restol;   /* restol(ifield): overrides tin when nonzero and tin>0 (hmholtz.f:677) */
+  int precision;   /* 0: FP64 (the drop-in); 1: FP32 operator inside FP64 refinement */
+  int reserved;
+} nkh_params;
+
+/* What the reference leaves behind after a solve. */
+typedef struct {
+  int niter;      /* /iterhm/ niterhm */
+  int converged;  /* rbn2 <= tol reached before maxit */
+  double rbn0;    /* residual norm at iteration 1 */
+  double rbn2;    /* residual norm at exit */
+  double tol;     /* absolute tolerance used */
+} nkh_result;
+
+/* ---- lifetime ---------------------------------------------------------- */
+
+/* Select the GPU and create the library stream.  Replaces nothing in the
+   reference (Nek has no device); called once from the Fortran shim's setup. */
+int nkh_init(int device);
+int nkh_finalize(void);
+const char *nkh_last_error(void);
+/* Run subsequent work on the caller's stream (hipStream_t passed as void*);
+   NULL restores the library stream. */
+int nkh_set_stream(void *hip_stream);
+int nkh_synchronize(void);
+
+/* ---- basis and numbering (host) --------------------------------------- */
+
+/* speclib.f:107-122 ZWGLL: np GLL points and weights. */
+int nkh_zwgll(double *z, double *w, int np);
+/* speclib.f:800-833 DGLL: D(i,j) = phi_j'(z_i) and its transpose, ld = nz. */
+int nkh_dgll(double *d, double *dt, const double *z, int nz);
+/* navier8.f:2004-2360 SETVERT3D with ifcenter=.false. on one rank:
+   vertex(2,2,2,nel) int64 -> glo_num(nx,nx,nx,nel) int64; *ngv = ids used. */
+int nkh_setvert3d(int64_t *glo_num, int64_t *ngv, int nx, int nel, const int64_t *vertex);
+
+/* ---- meshes ------------------------------------------------------------ */
+
+/* coef.f:1-359 GENWZ (3-D GLL part): creates a mesh of nel elements with
+   lx1 points per direction and its 1-D basis. */
+int nkh_mesh_create(int lx1, int nel, int *mesh);
+int nkh_mesh_destroy(int mesh);
+/* coef.f:879-927 XYZRST + :555-632 GLMAPM1 + :633-785 GEODAT1 on the GPU:
+   g1m1..g6m1 and bm1 from the GLL coordinates (host arrays). */
+int nkh_mesh_geom_from_xyz(int mesh, const double *xm1, const double *ym1, const double *zm1);
+/* same, coordinates already on the device */
+int nkd_mesh_geom_from_xyz(int mesh, const double *xm1, const double *ym1, const double *zm1);
+/* Upload Nek's own /GMFACT/ g1m1..g6m1 and /MASS/ bm1 (host arrays). */
+int nkh_mesh_set_geom(int mesh, const double *g1, const double *g2, const double *g3,
+                      const double *g4, const double *g5, const double *g6, const double *bm1);
+/* dssum.f:1-31 SETUPDS (fgslib_gs_setup) from Nek's glo_num (host), then
+   coef.f:1500-1543 SETINVM (binvm1 = 1/dssum(bm1)) and volvm1 = sum(bm1). */
+int nkh_mesh_gs_setup(int mesh, const int64_t *glo_num);
+/* n = lx1^3*nel, vol = volvm1, nseg/nsingle = shared groups / unshared ids */
+int nkh_mesh_info(int mesh, int64_t *n, double *vol, int64_t *nseg, int64_t *nsingle,
+                  int *max_mult);
+/* Device copies of bm1 / binvm1 (device pointers owned by the library). */
+int nkh_mesh_ptrs(int mesh, const double **bm1, const double **binvm1);
+
+/* ---- operators on device arrays ---------------------------------------- */
+
+/* hmholtz.f:72-259 AXHELM (3-D): au = h1*A u + h2*B u, local (unassembled).
+   isd is accepted for signature parity; only the non-axisymmetric path exists. */
+int nkd_axhelm(int mesh, double *au, const double *u, const double *h1, const double *h2,
+               int isd);
+/* dssum.f:33-98 DSSUM: u_i <- sum over nodes sharing glo_num(i). */
+int nkd_dssum(int mesh, double *u);
+/* dssum.f:100-161 DSOP: op '+', '*', 'm' (min), 'M' (max). */
+int nkd_dsop(int mesh, double *u, char op);
+/* hmholtz.f:380-524 SETPREC: inverse of the assembled Jacobi diagonal. */
+int nkd_setprec(int mesh, double *d, const double *h1, const double *h2);
+/* hmholtz.f:2-69 HMHOLTZ (non-PRES field): dssum(rhs); rhs*=mask; chktcg1;
+   cggo.  rhs is modified in place as in the reference; u receives the
+   solution.  history (optional, 5*min(maxit,900) doubles) receives per
+   iteration (rbn2, rtz1, rho, alpha, beta). */
+int nkd_hmholtz(int mesh, const char *name, double *u, double *rhs, const double *h1,
+                const double *h2, const double *mask, const double *mult, double tli,
+                int maxit, int isd, const nkh_params *prm, nkh_result *res,
+                double *history);
+/* hmholtz.f:611-846 CGGO (Jacobi branch): f is the assembled, masked rhs
+   (left unchanged), binv = binvm1/bintm1 of the mesh. */
+int nkd_cggo(int mesh, double *x, const double *f, const double *h1, const double *h2,
+             const double *mask, const double *mult, double tin, int maxit,
+             const nkh_params *prm, nkh_result *res, double *history);
+
+/* ---- timers ----------------------------------------------------------- */
+
+/* Mirror of Nek's CTIMER counters (hmholtz.f:112-113 naxhm/taxhm, :26-31
+   nhmhz/thmhz, dssum.f:52-95 tdsum).  When enabled, solves record HIP events
+   around the axhelm and gather-scatter launches of each iteration that does
+   work, on the stream those kernels run on.  name: "axhm", "dsum", "hmhz". */
+int nkh_timers_enable(int on);
+int nkh_timers_reset(void);
+int nkh_timers_read(const char *name, double *seconds, int64_t *calls);
+
+/* ---- the same operators on host arrays (Fortran drop-in) ---------------- */
+
+int nkh_axhelm(int mesh, double *au, const double *u, const double *h1, const double *h2,
+               int isd);
+int nkh_dssum(int mesh, double *u);
+int nkh_hmholtz(int mesh, const char *name, double *u, double *rhs, const double *h1,
+                const double *h2, const double *mask, const double *mult, double tli,
+                int maxit, int isd, const nkh_params *prm, nkh_result *res,
+                double *history);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* NEKAMD_H */

@@ -1,0 +1,107 @@
+"""ctypes binding of libnekamd.so (the C ABI declared in include/nekamd.h).
+
+The library is the product: there is no CPU fallback.  Loading fails loudly
+when the shared object is missing (build it with ``python -m nek5000_amd.build``).
+"""
+import ctypes
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libnekamd.so")
+
+c_int = ctypes.c_int
+c_int64 = ctypes.c_int64
+c_double = ctypes.c_double
+c_char = ctypes.c_char
+c_char_p = ctypes.c_char_p
+c_void_p = ctypes.c_void_p
+P = ctypes.POINTER
+
+
+class NkhParams(ctypes.Structure):
+    _fields_ = [
+        ("istep", c_int),
+        ("param18", c_int),
+        ("param22", c_double),
+        ("restol", c_double),
+        ("precision", c_int),
+        ("reserved", c_int),
+    ]
+
+
+class NkhResult(ctypes.Structure):
+    _fields_ = [
+        ("niter", c_int),
+        ("converged", c_int),
+        ("rbn0", c_double),
+        ("rbn2", c_double),
+        ("tol", c_double),
+    ]
+
+
+# name -> (restype, argtypes); every declaration of include/nekamd.h
+SIGNATURES = {
+    "nkh_init": (c_int, [c_int]),
+    "nkh_finalize": (c_int, []),
+    "nkh_last_error": (c_char_p, []),
+    "nkh_set_stream": (c_int, [c_void_p]),
+    "nkh_synchronize": (c_int, []),
+    "nkh_zwgll": (c_int, [c_void_p, c_void_p, c_int]),
+    "nkh_dgll": (c_int, [c_void_p, c_void_p, c_void_p, c_int]),
+    "nkh_setvert3d": (c_int, [c_void_p, P(c_int64), c_int, c_int, c_void_p]),
+    "nkh_mesh_create": (c_int, [c_int, c_int, P(c_int)]),
+    "nkh_mesh_destroy": (c_int, [c_int]),
+    "nkh_mesh_geom_from_xyz": (c_int, [c_int, c_void_p, c_void_p, c_void_p]),
+    "nkd_mesh_geom_from_xyz": (c_int, [c_int, c_void_p, c_void_p, c_void_p]),
+    "nkh_mesh_set_geom": (c_int, [c_int] + [c_void_p] * 7),
+    "nkh_mesh_gs_setup": (c_int, [c_int, c_void_p]),
+    "nkh_mesh_info": (c_int, [c_int, P(c_int64), P(c_double), P(c_int64), P(c_int64), P(c_int)]),
+    "nkh_mesh_ptrs": (c_int, [c_int, P(c_void_p), P(c_void_p)]),
+    "nkd_axhelm": (c_int, [c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_int]),
+    "nkd_dssum": (c_int, [c_int, c_void_p]),
+    "nkd_dsop": (c_int, [c_int, c_void_p, c_char]),
+    "nkd_setprec": (c_int, [c_int, c_void_p, c_void_p, c_void_p]),
+    "nkd_hmholtz": (c_int, [c_int, c_char_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
+                            c_void_p, c_double, c_int, c_int, P(NkhParams), P(NkhResult),
+                            c_void_p]),
+    "nkd_cggo": (c_int, [c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
+                         c_double, c_int, P(NkhParams), P(NkhResult), c_void_p]),
+    "nkh_timers_enable": (c_int, [c_int]),
+    "nkh_timers_reset": (c_int, []),
+    "nkh_timers_read": (c_int, [c_char_p, P(c_double), P(c_int64)]),
+    "nkh_axhelm": (c_int, [c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_int]),
+    "nkh_dssum": (c_int, [c_int, c_void_p]),
+    "nkh_hmholtz": (c_int, [c_int, c_char_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
+                            c_void_p, c_double, c_int, c_int, P(NkhParams), P(NkhResult),
+                            c_void_p]),
+}
+
+_lib = None
+
+
+class NekAmdError(RuntimeError):
+    pass
+
+
+def load(path=LIB_PATH):
+    """Load libnekamd.so and attach prototypes; raises if it is not built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(path):
+        raise NekAmdError(
+            f"{path} is missing: the HIP library is the only implementation; "
+            "build it with `python -m nek5000_amd.build`")
+    lib = ctypes.CDLL(path)
+    for name, (res, args) in SIGNATURES.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = lib
+    return lib
+
+
+def check(rc, what=""):
+    if rc != 0:
+        msg = _lib.nkh_last_error().decode(errors="replace") if _lib else ""
+        raise NekAmdError(f"{what} failed with code {rc}: {msg}")

@@ -1,0 +1,559 @@
+// Jacobi-preconditioned CG (hmholtz.f:611-846 CGGO) and its helpers
+// (hmholtz.f:527-609 CHKTCG1, :380-524 SETPREC), all data resident on the GPU.
+//
+// One CG iteration of the reference,
+//     z = r*d ; rtz1 = (z,r)_mult ; rbn2 = sqrt((r,r)_{binv*mult}/vol)
+//     exit test ; beta ; p = z + beta p ; w = A p ; dssum(w) ; w *= mask
+//     rho = (w,p)_mult ; alpha ; x += alpha p ; r -= alpha w
+// runs as five launches:
+//     k_cg_update_dots : x/r update of the previous iteration + both dots
+//     k_cg_scalars1    : one block: sums the dot partials, exit test, beta
+//     axhelm (MODE 1)  : p update fused in, w = A p, rho over interior nodes
+//     gs_sum_mask_dot  : dssum + mask + rho over element-boundary nodes
+//     k_cg_scalars2    : one block: rho, alpha
+// Scalars stay on the device (CgState); the host enqueues iterations ahead
+// and reads the state back every few iterations.  Kernels of iterations past
+// the exit return immediately.
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+#include "nekamd_internal.h"
+
+namespace nekamd {
+
+namespace {
+
+constexpr int kMaxCg = 900;  // hmholtz.f:636 maxcg
+
+__device__ __forceinline__ double wave_sum(double v) {
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
+  return v;
+}
+
+// Sum of `count` partials laid out with stride 1, by one 256-thread block in
+// a fixed order.  Result valid on thread 0.
+__device__ double block_sum_array(const double *__restrict__ part, int count) {
+  __shared__ double sb[4];
+  double v = 0.0;
+  for (int q = threadIdx.x; q < count; q += blockDim.x) v += part[q];
+  v = wave_sum(v);
+  if ((threadIdx.x & 63) == 0) sb[threadIdx.x >> 6] = v;
+  __syncthreads();
+  double t = 0.0;
+  if (threadIdx.x == 0) t = ((sb[0] + sb[1]) + sb[2]) + sb[3];
+  __syncthreads();
+  return t;
+}
+
+__device__ __forceinline__ void block_store2(double a, double b, double *pa, double *pb) {
+  __shared__ double sa[4], sbb[4];
+  a = wave_sum(a);
+  b = wave_sum(b);
+  if ((threadIdx.x & 63) == 0) {
+    sa[threadIdx.x >> 6] = a;
+    sbb[threadIdx.x >> 6] = b;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    pa[blockIdx.x] = ((sa[0] + sa[1]) + sa[2]) + sa[3];
+    pb[blockIdx.x] = ((sbb[0] + sbb[1]) + sbb[2]) + sbb[3];
+  }
+}
+
+// ---- generic reductions (setup) ----------------------------------------
+// kind 0: sum a*b*c (c may be null -> 1, b may be null -> 1)
+// kind 1: max |a|   kind 2: max a   kind 3: min a
+__global__ void __launch_bounds__(256)
+    k_reduce(const double *__restrict__ a, const double *__restrict__ b,
+             const double *__restrict__ c, int64_t n, int kind, double *__restrict__ part) {
+  __shared__ double sb[4];
+  double v = (kind == 0 || kind == 1) ? 0.0 : (kind == 2 ? -INFINITY : INFINITY);
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    const double x = a[i];
+    if (kind == 0) {
+      double t = x;
+      if (b) t = t * b[i];
+      if (c) t = t * c[i];
+      v += t;
+    } else if (kind == 1) {
+      v = fmax(v, fabs(x));
+    } else if (kind == 2) {
+      v = fmax(v, x);
+    } else {
+      v = fmin(v, x);
+    }
+  }
+  for (int o = 32; o > 0; o >>= 1) {
+    const double t = __shfl_down(v, o, 64);
+    if (kind == 0) v += t;
+    else if (kind == 3) v = fmin(v, t);
+    else v = fmax(v, t);
+  }
+  if ((threadIdx.x & 63) == 0) sb[threadIdx.x >> 6] = v;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double t = sb[0];
+    for (int q = 1; q < 4; ++q) {
+      if (kind == 0) t += sb[q];
+      else if (kind == 3) t = fmin(t, sb[q]);
+      else t = fmax(t, sb[q]);
+    }
+    part[blockIdx.x] = t;
+  }
+}
+
+__global__ void __launch_bounds__(256)
+    k_reduce_final(const double *__restrict__ part, int count, int kind,
+                   double *__restrict__ out) {
+  __shared__ double sb[4];
+  double v = (kind == 0 || kind == 1) ? 0.0 : (kind == 2 ? -INFINITY : INFINITY);
+  for (int q = threadIdx.x; q < count; q += blockDim.x) {
+    const double x = part[q];
+    if (kind == 0) v += x;
+    else if (kind == 3) v = fmin(v, x);
+    else v = fmax(v, x);
+  }
+  for (int o = 32; o > 0; o >>= 1) {
+    const double t = __shfl_down(v, o, 64);
+    if (kind == 0) v += t;
+    else if (kind == 3) v = fmin(v, t);
+    else v = fmax(v, t);
+  }
+  if ((threadIdx.x & 63) == 0) sb[threadIdx.x >> 6] = v;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double t = sb[0];
+    for (int q = 1; q < 4; ++q) {
+      if (kind == 0) t += sb[q];
+      else if (kind == 3) t = fmin(t, sb[q]);
+      else t = fmax(t, sb[q]);
+    }
+    *out = t;
+  }
+}
+
+// ---- elementwise ----------------------------------------------------------
+__global__ void k_fill(double *__restrict__ a, double v, int64_t n) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x)
+    a[i] = v;
+}
+__global__ void k_mul(double *__restrict__ a, const double *__restrict__ b, int64_t n) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x)
+    a[i] = a[i] * b[i];
+}
+__global__ void k_inv(double *__restrict__ a, int64_t n) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x)
+    a[i] = 1.0 / a[i];
+}
+// r = f, x = 0, p = 0; dm = d*mult; bm = binv*mult (or mult when param18=1)
+__global__ void k_cg_init(double *__restrict__ x, double *__restrict__ r,
+                          double *__restrict__ p, const double *__restrict__ f,
+                          const double *__restrict__ d, const double *__restrict__ mult,
+                          const double *__restrict__ binv, double *__restrict__ dm,
+                          double *__restrict__ bm, int plain_norm, int64_t n) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    r[i] = f[i];
+    x[i] = 0.0;
+    p[i] = 0.0;
+    dm[i] = d[i] * mult[i];
+    bm[i] = plain_norm ? mult[i] : binv[i] * mult[i];
+  }
+}
+
+// x += alpha p ; r -= alpha w (when an update is pending) ; partial sums of
+// (d*r)*r*mult and binv*mult*r*r (vlsc3 / vlsc32, hmholtz.f:754-759)
+__global__ void __launch_bounds__(256)
+    k_cg_update_dots(double *__restrict__ x, double *__restrict__ r,
+                     const double *__restrict__ p, const double *__restrict__ w,
+                     const double *__restrict__ dm, const double *__restrict__ bm, int64_t n,
+                     const CgState *__restrict__ st, double *__restrict__ part1,
+                     double *__restrict__ part2) {
+  if (st->done) return;
+  const bool upd = st->pending != 0;
+  const double alpha = st->alpha;
+  double s1 = 0.0, s2 = 0.0;
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    double ri = r[i];
+    if (upd) {
+      x[i] = x[i] + alpha * p[i];
+      ri = ri + (-alpha) * w[i];
+      r[i] = ri;
+    }
+    const double rr = ri * ri;
+    s1 += dm[i] * rr;
+    s2 += bm[i] * rr;
+  }
+  block_store2(s1, s2, part1, part2);
+}
+
+__global__ void __launch_bounds__(256)
+    k_cg_scalars1(CgState *__restrict__ st, const double *__restrict__ part1,
+                  const double *__restrict__ part2, double *__restrict__ hist) {
+  if (st->done) return;
+  const double s1 = block_sum_array(part1, kRedBlocks);
+  const double s2 = block_sum_array(part2, kRedBlocks);
+  if (threadIdx.x != 0) return;
+  st->pending = 0;
+  const int iter = st->iter + 1;
+  st->iter = iter;
+  if (iter > st->maxit) {  // loop exhausted (hmholtz.f:817: niter = iter-1)
+    st->done = 1;
+    st->niter = st->maxit;
+    return;
+  }
+  st->rtz2 = st->rtz1;
+  st->rtz1 = s1;
+  const double rbn2 = sqrt(s2 / st->vol);
+  st->rbn2 = rbn2;
+  if (iter == 1) st->rbn0 = rbn2;
+  if (st->param22 < 0) st->tol = fabs(st->param22) * st->rbn0;
+  if (st->tin < 0) st->tol = fabs(st->tin) * st->rbn0;
+  if (hist) {
+    hist[5 * (iter - 1) + 0] = rbn2;
+    hist[5 * (iter - 1) + 1] = s1;
+  }
+  if (rbn2 <= st->tol && (iter > 1 || st->istep <= 5)) {
+    st->done = 1;
+    st->converged = 1;
+    st->niter = iter - 1;
+    return;
+  }
+  st->beta = (iter == 1) ? 0.0 : st->rtz1 / st->rtz2;
+}
+
+__global__ void __launch_bounds__(256)
+    k_cg_scalars2(CgState *__restrict__ st, const double *__restrict__ rp1, int n1,
+                  const double *__restrict__ rp2, int n2, double *__restrict__ hist) {
+  if (st->done) return;
+  const double a = block_sum_array(rp1, n1);
+  const double b = (n2 > 0) ? block_sum_array(rp2, n2) : 0.0;
+  if (threadIdx.x != 0) return;
+  const double rho = a + b;
+  st->rho = rho;
+  st->alpha = st->rtz1 / rho;
+  st->pending = 1;
+  if (hist) {
+    const int it = st->iter;
+    hist[5 * (it - 1) + 2] = rho;
+    hist[5 * (it - 1) + 3] = st->alpha;
+    hist[5 * (it - 1) + 4] = st->beta;
+  }
+}
+
+// rho over all nodes (used when ids are not zero exactly at interiors):
+// part[b] = sum w*p*mult over the block's nodes
+__global__ void __launch_bounds__(256)
+    k_dot3_guarded(const double *__restrict__ a, const double *__restrict__ b,
+                   const double *__restrict__ c, int64_t n, const CgState *__restrict__ st,
+                   double *__restrict__ part) {
+  __shared__ double sb[4];
+  if (st->done) return;
+  double v = 0.0;
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x)
+    v += a[i] * b[i] * c[i];
+  v = wave_sum(v);
+  if ((threadIdx.x & 63) == 0) sb[threadIdx.x >> 6] = v;
+  __syncthreads();
+  if (threadIdx.x == 0) part[blockIdx.x] = ((sb[0] + sb[1]) + sb[2]) + sb[3];
+}
+
+__global__ void k_mask_guarded(double *__restrict__ w, const double *__restrict__ mask,
+                               int64_t n, const CgState *__restrict__ st) {
+  if (st->done) return;
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x)
+    w[i] = w[i] * mask[i];
+}
+
+// max |mask - 1| over element-interior nodes, one partial per block
+__global__ void __launch_bounds__(256)
+    k_interior_mask_dev(const double *__restrict__ mask, int64_t n, int np,
+                        double *__restrict__ part) {
+  __shared__ double sb[4];
+  const int64_t np3 = (int64_t)np * np * np;
+  double v = 0.0;
+  for (int64_t l = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; l < n;
+       l += (int64_t)gridDim.x * blockDim.x) {
+    const int q = (int)(l % np3);
+    const int i = q % np, j = (q / np) % np, k = q / (np * np);
+    if (i > 0 && i < np - 1 && j > 0 && j < np - 1 && k > 0 && k < np - 1)
+      v = fmax(v, fabs(mask[l] - 1.0));
+  }
+  for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_down(v, o, 64));
+  if ((threadIdx.x & 63) == 0) sb[threadIdx.x >> 6] = v;
+  __syncthreads();
+  if (threadIdx.x == 0) part[blockIdx.x] = fmax(fmax(sb[0], sb[1]), fmax(sb[2], sb[3]));
+}
+
+int grid_for(int64_t n) {
+  int64_t g = (n + 255) / 256;
+  if (g > 8192) g = 8192;
+  if (g < 1) g = 1;
+  return (int)g;
+}
+
+// Reduce on the device and read the scalar back (synchronous).
+int reduce_host(const Mesh &m, const double *a, const double *b, const double *c, int64_t n,
+                int kind, double *out, hipStream_t s) {
+  NK_CHECK(m.red != nullptr, NKH_ERR_STATE, "reduction buffer not allocated");
+  hipLaunchKernelGGL(k_reduce, dim3(kRedBlocks), dim3(256), 0, s, a, b, c, n, kind, m.red);
+  hipLaunchKernelGGL(k_reduce_final, dim3(1), dim3(256), 0, s, m.red, kRedBlocks, kind,
+                     m.red + 4 * kRedBlocks);
+  NK_HIP(hipGetLastError());
+  NK_HIP(hipMemcpyAsync(out, m.red + 4 * kRedBlocks, sizeof(double), hipMemcpyDeviceToHost, s));
+  NK_HIP(hipStreamSynchronize(s));
+  return NKH_OK;
+}
+
+}  // namespace
+
+int ensure_small(Mesh &m) {
+  if (!m.st) NK_HIP(hipMalloc(&m.st, sizeof(CgState)));
+  if (!m.red) NK_HIP(hipMalloc(&m.red, sizeof(double) * (4 * kRedBlocks + 64 + 5 * kMaxCg)));
+  return NKH_OK;
+}
+
+int ensure_workspace(Mesh &m) {
+  NK_TRY(ensure_small(m));
+  const size_t need = (size_t)6 * m.n;
+  if (m.ws && m.ws_doubles >= need) return NKH_OK;
+  if (m.ws) (void)hipFree(m.ws);
+  m.ws = nullptr;
+  if (hipMalloc(&m.ws, sizeof(double) * need) != hipSuccess) {
+    set_error("workspace allocation of %zu doubles failed", need);
+    return NKH_ERR_NOMEM;
+  }
+  m.ws_doubles = need;
+  return NKH_OK;
+}
+
+// Reduce a coefficient field to a uniform value when it is constant (the
+// multiply by the scalar is then bit-identical to the multiply by the field).
+static int classify(Mesh &m, const double *f, Coef *c, hipStream_t s) {
+  NK_TRY(ensure_small(m));
+  double mn, mx;
+  NK_TRY(reduce_host(m, f, nullptr, nullptr, m.n, 3, &mn, s));
+  NK_TRY(reduce_host(m, f, nullptr, nullptr, m.n, 2, &mx, s));
+  if (mn == mx) {
+    c->f = nullptr;
+    c->s = mn;
+  } else {
+    c->f = f;
+    c->s = 0.0;
+  }
+  return NKH_OK;
+}
+
+int classify_coefs(Mesh &m, const double *h1, const double *h2, Coef *c1, Coef *c2,
+                   hipStream_t s) {
+  NK_TRY(classify(m, h1, c1, s));
+  // setfast (hmholtz.f:303-305): ifh2 = vlamax(h2) > 0; a uniform 0 drops the term
+  NK_TRY(classify(m, h2, c2, s));
+  return NKH_OK;
+}
+
+int mul_inplace(double *a, const double *b, int64_t n, hipStream_t s) {
+  hipLaunchKernelGGL(k_mul, dim3(grid_for(n)), dim3(256), 0, s, a, b, n);
+  NK_HIP(hipGetLastError());
+  return NKH_OK;
+}
+
+int run_setprec(Mesh &m, double *d, Coef h1, Coef h2, hipStream_t s) {
+  NK_TRY(launch_setprec(m, d, h1, h2, s));
+  NK_TRY(gs_op(m.gs, d, 0, s));
+  hipLaunchKernelGGL(k_inv, dim3(grid_for(m.n)), dim3(256), 0, s, d, m.n);
+  NK_HIP(hipGetLastError());
+  return NKH_OK;
+}
+
+int run_chktcg1(Mesh &m, double *tol, const double *res, Coef h1, Coef h2,
+                const double *mask, const double *mult) {
+  hipStream_t s = cur_stream();
+  NK_TRY(ensure_workspace(m));
+  const int64_t n = m.n;
+  double *w1 = m.ws, *w2 = m.ws + n;
+  const double eps = 1.0e-13, acondno = 10.0;
+  double a;
+  // rinit = sqrt(glsc3(binv*res, res, mult)/vol)
+  hipLaunchKernelGGL(k_fill, dim3(grid_for(n)), dim3(256), 0, s, w1, 1.0, n);
+  NK_HIP(hipMemcpyAsync(w2, m.binv, sizeof(double) * n, hipMemcpyDeviceToDevice, s));
+  hipLaunchKernelGGL(k_mul, dim3(grid_for(n)), dim3(256), 0, s, w2, res, n);
+  NK_TRY(reduce_host(m, w2, res, mult, n, 0, &a, s));
+  const double rinit = std::sqrt(a / m.vol);
+  const double rmin = eps * rinit;
+  if (*tol < rmin) *tol = rmin;
+  double bcneu1, bcneu2;
+  NK_TRY(reduce_host(m, w1, mask, mult, n, 0, &bcneu1, s));
+  NK_TRY(reduce_host(m, w1, w1, mult, n, 0, &bcneu2, s));
+  const double bctest = std::fabs(bcneu1 - bcneu2);
+  NK_TRY(launch_axhelm(m, w2, w1, h1, h2, s));
+  double sq;
+  NK_TRY(reduce_host(m, w2, w2, m.bm1, n, 0, &sq, s));
+  const double bcrob = std::sqrt(sq / m.vol);
+  if (bctest < 0.1 && bcrob < eps * acondno) {
+    const double tolmin = rinit * eps * 10.0;
+    if (*tol < tolmin) *tol = tolmin;
+  }
+  return NKH_OK;
+}
+
+int run_cggo(Mesh &m, double *x, const double *f, Coef h1, Coef h2, const double *mask,
+             const double *mult, double tin, int maxit, const nkh_params &prm,
+             nkh_result *res, double *history_host) {
+  hipStream_t s = cur_stream();
+  NK_TRY(ensure_workspace(m));
+  const int64_t n = m.n;
+  double *r = m.ws, *w = m.ws + n, *p = m.ws + 2 * n, *d = m.ws + 3 * n, *dm = m.ws + 4 * n,
+         *bm = m.ws + 5 * n;
+  double *part1 = m.red, *part2 = m.red + kRedBlocks, *rho1 = m.red + 2 * kRedBlocks,
+         *rho2 = m.red + 3 * kRedBlocks;
+
+  double tol = std::fabs(tin);
+  if (prm.restol != 0.0) tol = prm.restol;
+  if (tin < 0) tol = std::fabs(tin);
+  const int niter = maxit < kMaxCg ? maxit : kMaxCg;
+
+  NK_TRY(run_setprec(m, d, h1, h2, s));
+  hipLaunchKernelGGL(k_cg_init, dim3(grid_for(n)), dim3(256), 0, s, x, r, p, f, d, mult,
+                     m.binv, dm, bm, prm.param18 == 1 ? 1 : 0, n);
+  NK_HIP(hipGetLastError());
+
+  double fmax;
+  NK_TRY(reduce_host(m, f, nullptr, nullptr, n, 1, &fmax, s));
+  nkh_result out{};
+  if (fmax == 0.0) {
+    if (res) *res = out;
+    return NKH_OK;
+  }
+  // null-space check (hmholtz.f:705-720): pure Neumann with h2 == 0
+  double h2max, skmin;
+  if (h2.uniform()) h2max = h2.s;
+  else NK_TRY(reduce_host(m, h2.f, nullptr, nullptr, n, 2, &h2max, s));
+  NK_TRY(reduce_host(m, mask, nullptr, nullptr, n, 3, &skmin, s));
+  NK_CHECK(!(skmin > 0 && h2max == 0), NKH_ERR_UNSUPPORTED,
+           "cggo: pure-Neumann mean correction (ifmcor) is not implemented");
+
+  // The fused kernels finish element-interior nodes inside axhelm (no dssum
+  // there, mult taken from the array, mask assumed 1).  Use them only when the
+  // ids are zero exactly at interiors and the mask is 1 at every interior node.
+  bool fused = m.gs.interior_zero;
+  if (fused) {
+    hipLaunchKernelGGL(k_interior_mask_dev, dim3(kRedBlocks), dim3(256), 0, s, mask, n, m.np,
+                       m.red);
+    hipLaunchKernelGGL(k_reduce_final, dim3(1), dim3(256), 0, s, m.red, kRedBlocks, 2,
+                       m.red + 4 * kRedBlocks);
+    NK_HIP(hipGetLastError());
+    double dev;
+    NK_HIP(hipMemcpyAsync(&dev, m.red + 4 * kRedBlocks, sizeof(double), hipMemcpyDeviceToHost, s));
+    NK_HIP(hipStreamSynchronize(s));
+    fused = (dev == 0.0);
+  }
+
+  CgState h{};
+  h.rtz1 = 1.0;
+  h.tol = tol;
+  h.tin = tin;
+  h.vol = m.vol;
+  h.param22 = prm.param22;
+  h.maxit = niter;
+  h.istep = prm.istep;
+  h.param18 = prm.param18;
+  NK_HIP(hipMemcpyAsync(m.st, &h, sizeof(CgState), hipMemcpyHostToDevice, s));
+
+  double *hist = nullptr;
+  if (history_host) {
+    hist = m.red + 4 * kRedBlocks + 64;
+    hipLaunchKernelGGL(k_fill, dim3(grid_for(5 * kMaxCg)), dim3(256), 0, s, hist, 0.0,
+                       (int64_t)5 * kMaxCg);
+  }
+
+  CgState hs{};
+  int launched = 0;
+  int check_every = 1;
+  // CTIMER mirror: event pairs around axhelm and gs of every launched iteration
+  const bool timed = timers().on;
+  std::vector<hipEvent_t> ev;
+  if (timed) {
+    ev.resize((size_t)4 * (niter + 1));
+    for (auto &e : ev) NK_HIP(hipEventCreate(&e));
+  }
+  hipEvent_t t_solve0 = nullptr, t_solve1 = nullptr;
+  if (timed) {
+    NK_HIP(hipEventCreate(&t_solve0));
+    NK_HIP(hipEventCreate(&t_solve1));
+    NK_HIP(hipEventRecord(t_solve0, s));
+  }
+  while (true) {
+    hipLaunchKernelGGL(k_cg_update_dots, dim3(kRedBlocks), dim3(256), 0, s, x, r, p, w, dm,
+                       bm, n, m.st, part1, part2);
+    hipLaunchKernelGGL(k_cg_scalars1, dim3(1), dim3(256), 0, s, m.st, part1, part2, hist);
+    if (fused) {
+      if (timed) NK_HIP(hipEventRecord(ev[4 * launched + 0], s));
+      NK_TRY(launch_axhelm_cg(m, w, p, r, d, h1, h2, mult, m.st, rho1, s));
+      if (timed) NK_HIP(hipEventRecord(ev[4 * launched + 1], s));
+      NK_TRY(gs_sum_mask_dot(m.gs, w, mask, p, mult, m.st, rho2, s));
+      if (timed) NK_HIP(hipEventRecord(ev[4 * launched + 2], s));
+      hipLaunchKernelGGL(k_cg_scalars2, dim3(1), dim3(256), 0, s, m.st, rho1, kRedBlocks,
+                         rho2, kRedBlocks, hist);
+    } else {
+      NK_TRY(launch_axhelm_cg(m, w, p, r, d, h1, h2, mult, m.st, nullptr, s));
+      NK_TRY(gs_op(m.gs, w, 0, s));
+      hipLaunchKernelGGL(k_mask_guarded, dim3(grid_for(n)), dim3(256), 0, s, w, mask, n, m.st);
+      hipLaunchKernelGGL(k_dot3_guarded, dim3(kRedBlocks), dim3(256), 0, s, w, p, mult, n,
+                         m.st, rho1);
+      hipLaunchKernelGGL(k_cg_scalars2, dim3(1), dim3(256), 0, s, m.st, rho1, kRedBlocks,
+                         rho2, 0, hist);
+    }
+    NK_HIP(hipGetLastError());
+    ++launched;
+    if (launched % check_every == 0 || launched > niter) {
+      NK_HIP(hipMemcpyAsync(&hs, m.st, sizeof(CgState), hipMemcpyDeviceToHost, s));
+      NK_HIP(hipStreamSynchronize(s));
+      if (hs.done) break;
+      NK_CHECK(launched <= niter + 1, NKH_ERR_STATE, "cggo: state machine did not terminate");
+      if (launched >= 8) check_every = 4;
+    }
+  }
+  if (timed) {
+    NK_HIP(hipEventRecord(t_solve1, s));
+    NK_HIP(hipEventSynchronize(t_solve1));
+    float ms = 0.f;
+    // axhelm and gs did work in launches 1..niter (later ones returned at once)
+    for (int it = 0; it < hs.niter && fused; ++it) {
+      NK_HIP(hipEventElapsedTime(&ms, ev[4 * it + 0], ev[4 * it + 1]));
+      timers().taxhm += ms * 1e-3;
+      timers().naxhm += 1;
+      NK_HIP(hipEventElapsedTime(&ms, ev[4 * it + 1], ev[4 * it + 2]));
+      timers().tdsum += ms * 1e-3;
+      timers().ndsum += 1;
+    }
+    NK_HIP(hipEventElapsedTime(&ms, t_solve0, t_solve1));
+    timers().thmhz += ms * 1e-3;
+    timers().nhmhz += 1;
+    for (auto &e : ev) (void)hipEventDestroy(e);
+    (void)hipEventDestroy(t_solve0);
+    (void)hipEventDestroy(t_solve1);
+  }
+  out.niter = hs.niter;
+  out.converged = hs.converged;
+  out.rbn0 = hs.rbn0;
+  out.rbn2 = hs.rbn2;
+  out.tol = hs.tol;
+  if (res) *res = out;
+  if (history_host) {
+    NK_HIP(hipMemcpyAsync(history_host, hist, sizeof(double) * 5 * kMaxCg,
+                          hipMemcpyDeviceToHost, s));
+    NK_HIP(hipStreamSynchronize(s));
+  }
+  return NKH_OK;
+}
+
+}  // namespace nekamd

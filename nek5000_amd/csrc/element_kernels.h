@@ -1,0 +1,225 @@
+// Element kernels of the Helmholtz path: axhelm (plain and fused into the
+// CG iteration), geometric factors, Jacobi diagonal -- templates shared by
+// the translation units that instantiate them.
+//
+// Work decomposition (all kernels): one NP x NP thread slab per element, the
+// slab walks the NP layers of the element in registers (the t-direction
+// column of u lives in the thread's registers, the current r/s layer in LDS).
+// At NP = 8 the slab is exactly one 64-lane wavefront.  Elements are visited
+// with a grid-stride loop over a fixed grid so partial sums land in a fixed
+// number of slots.
+#pragma once
+#include "nekamd_internal.h"
+
+namespace nekamd {
+
+template <int NP>
+struct DMat {
+  double d[NP * NP];  // D(i,j) at d[i + NP*j]  (dxm1, column-major)
+};
+
+// Elements per block: pack small elements so a block is at least 64 lanes.
+template <int NP>
+constexpr int epb() {
+  return (NP * NP >= 64) ? 1 : (64 / (NP * NP));
+}
+
+// Sum over the block's threads through LDS (blockDim need not fill whole
+// wavefronts), in a fixed tree order -> deterministic.  sbuf holds blockDim
+// doubles.  Result valid on thread 0.
+__device__ __forceinline__ double block_sum(double v, double *sbuf) {
+  const int t = threadIdx.x, nt = blockDim.x;
+  __syncthreads();
+  sbuf[t] = v;
+  __syncthreads();
+  for (int h = 1; h < nt; h <<= 1) {
+    if ((t % (2 * h)) == 0 && t + h < nt) sbuf[t] += sbuf[t + h];
+    __syncthreads();
+  }
+  return sbuf[0];
+}
+
+// ---------------------------------------------------------------------------
+// axhelm: hmholtz.f:188-218 (3-D deformed branch) and :225 (h2 term).
+//   tmp1 = h1*(g1*ur + g4*us + g5*ut)   (same order as col3/addcol3 there)
+//   tmp2 = h1*(g2*us + g4*ur + g6*ut)
+//   tmp3 = h1*(g3*ut + g5*ur + g6*us)
+//   au   = D^T_r tmp1 + D^T_s tmp2 + D^T_t tmp3  (+ (h2*bm1)*u)
+// MODE 0: au = A u.  MODE 1 (CG): p = d*r + beta*p is formed on the fly and
+// stored, w = A p, and sum(w*p*mult) over element-interior nodes is written
+// to one slot per block.
+// ---------------------------------------------------------------------------
+template <int NP, int MODE, bool H1F, int H2M>
+__global__ void __launch_bounds__(NP * NP * epb<NP>())
+    k_axhelm(DMat<NP> Dm, const double *__restrict__ Dg, double *__restrict__ au,
+             const double *__restrict__ uin, const double *__restrict__ g, int64_t gstride,
+             const double *__restrict__ h1f, double h1s, const double *__restrict__ h2f,
+             double h2s, const double *__restrict__ bm1, int64_t nel,
+             double *__restrict__ pout, const double *__restrict__ r,
+             const double *__restrict__ dinv, const double *__restrict__ mult,
+             const CgState *__restrict__ st, double *__restrict__ rho_part) {
+  constexpr int NP2 = NP * NP, NP3 = NP2 * NP, EPB = epb<NP>();
+  __shared__ double s_D[NP][NP + 1];
+  __shared__ double s_u[EPB][NP3];
+  __shared__ double s_wr[EPB][NP3];
+  __shared__ double s_ws[EPB][NP3];
+  __shared__ double s_red[NP2 * EPB];
+
+  double beta = 0.0;
+  if constexpr (MODE == 1) {
+    if (st->done) return;
+    beta = st->beta;
+  }
+  const int tid = threadIdx.x;
+  const int le = tid / NP2;
+  const int ij = tid - le * NP2;
+  const int i = ij % NP, j = ij / NP;
+  for (int t = tid; t < NP2; t += blockDim.x) s_D[t % NP][t / NP] = Dg[t];
+
+  double rho = 0.0;
+  for (int64_t eb = blockIdx.x; eb * EPB < nel; eb += gridDim.x) {
+    const int64_t e = eb * EPB + le;
+    const bool valid = (le < EPB) && (e < nel);
+    const int64_t base = e * NP3 + ij;
+    double r_u[NP], r_au[NP];
+    __syncthreads();  // LDS reuse across element groups (and s_D fill)
+#pragma unroll
+    for (int k = 0; k < NP; ++k) {
+      double v = 0.0;
+      if (valid) {
+        const int64_t l = base + k * NP2;
+        if constexpr (MODE == 1) {
+          v = beta * uin[l] + dinv[l] * r[l];  // add2s1: p = beta*p + z, z = r*d
+          pout[l] = v;
+        } else {
+          v = uin[l];
+        }
+      }
+      r_u[k] = v;
+      s_u[le][k * NP2 + ij] = v;
+      r_au[k] = 0.0;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < NP; ++k) {
+      double ur = 0.0, us = 0.0, ut = 0.0;
+#pragma unroll
+      for (int m = 0; m < NP; ++m) {
+        ur += s_D[i][m] * s_u[le][k * NP2 + j * NP + m];
+        us += s_D[j][m] * s_u[le][k * NP2 + m * NP + i];
+        ut += Dm.d[k + NP * m] * r_u[m];
+      }
+      double wr = 0.0, ws = 0.0, wt = 0.0;
+      if (valid) {
+        const int64_t l = base + k * NP2;
+        const double G1 = g[l], G2 = g[gstride + l], G3 = g[2 * gstride + l];
+        const double G4 = g[3 * gstride + l], G5 = g[4 * gstride + l],
+                     G6 = g[5 * gstride + l];
+        wr = ur * G1;
+        ws = us * G2;
+        wt = ut * G3;
+        wr = wr + us * G4;
+        wr = wr + ut * G5;
+        ws = ws + ur * G4;
+        ws = ws + ut * G6;
+        wt = wt + ur * G5;
+        wt = wt + us * G6;
+        const double h1 = H1F ? h1f[l] : h1s;
+        wr *= h1;
+        ws *= h1;
+        wt *= h1;
+      }
+      s_wr[le][k * NP2 + ij] = wr;
+      s_ws[le][k * NP2 + ij] = ws;
+#pragma unroll
+      for (int kk = 0; kk < NP; ++kk) r_au[kk] += Dm.d[k + NP * kk] * wt;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < NP; ++k) {
+      double ar = 0.0, as = 0.0;
+#pragma unroll
+      for (int m = 0; m < NP; ++m) {
+        ar += s_D[m][i] * s_wr[le][k * NP2 + j * NP + m];
+        as += s_D[m][j] * s_ws[le][k * NP2 + m * NP + i];
+      }
+      if (valid) {
+        const int64_t l = base + k * NP2;
+        double a = ar + as + r_au[k];
+        if constexpr (H2M == 1) a = a + (h2s * bm1[l]) * r_u[k];
+        if constexpr (H2M == 2) a = a + (h2f[l] * bm1[l]) * r_u[k];
+        au[l] = a;
+        if constexpr (MODE == 1) {
+          if (rho_part != nullptr && i > 0 && i < NP - 1 && j > 0 && j < NP - 1 && k > 0 &&
+              k < NP - 1)
+            rho += a * r_u[k] * mult[l];
+        }
+      }
+    }
+  }
+  if constexpr (MODE == 1) {
+    if (rho_part != nullptr) {
+      const double t = block_sum(rho, s_red);
+      if (threadIdx.x == 0) rho_part[blockIdx.x] = t;
+    }
+  }
+}
+
+template <int NP, int MODE>
+static int axhelm_dispatch(const Mesh &m, double *au, const double *u, Coef h1, Coef h2,
+                           double *pout, const double *r, const double *dinv,
+                           const double *mult, const CgState *st, double *rho_part,
+                           hipStream_t s) {
+  DMat<NP> Dm;
+  for (int t = 0; t < NP * NP; ++t) Dm.d[t] = m.D[t];
+  constexpr int EPB = epb<NP>();
+  const int64_t ngroups = (m.nel + EPB - 1) / EPB;
+  int grid;
+  if (MODE == 1) grid = kRedBlocks;
+  else grid = (int)((ngroups < (int64_t)1 << 20) ? ngroups : ((int64_t)1 << 20));
+  if (grid < 1) grid = 1;
+  const dim3 block(NP * NP * EPB);
+  // h2 mode: 0 none (reference ifh2 false), 1 uniform, 2 field
+  const int h2m = h2.uniform() ? (h2.s != 0.0 ? 1 : 0) : 2;
+  const bool h1f = !h1.uniform();
+#define NK_LAUNCH(H1F_, H2M_)                                                            \
+  hipLaunchKernelGGL((k_axhelm<NP, MODE, H1F_, H2M_>), dim3(grid), block, 0, s, Dm, m.Dd, \
+                     au, u, m.g, m.n, h1.f, h1.s, h2.f, h2.s, m.bm1, m.nel, pout, r, dinv, \
+                     mult, st, rho_part)
+  if (h1f) {
+    if (h2m == 0) NK_LAUNCH(true, 0);
+    else if (h2m == 1) NK_LAUNCH(true, 1);
+    else NK_LAUNCH(true, 2);
+  } else {
+    if (h2m == 0) NK_LAUNCH(false, 0);
+    else if (h2m == 1) NK_LAUNCH(false, 1);
+    else NK_LAUNCH(false, 2);
+  }
+#undef NK_LAUNCH
+  NK_HIP(hipGetLastError());
+  return NKH_OK;
+}
+
+#define NK_NP_SWITCH(np, CALL)                                      \
+  switch (np) {                                                     \
+    case 2: return CALL(2);                                         \
+    case 3: return CALL(3);                                         \
+    case 4: return CALL(4);                                         \
+    case 5: return CALL(5);                                         \
+    case 6: return CALL(6);                                         \
+    case 7: return CALL(7);                                         \
+    case 8: return CALL(8);                                         \
+    case 9: return CALL(9);                                         \
+    case 10: return CALL(10);                                       \
+    case 11: return CALL(11);                                       \
+    case 12: return CALL(12);                                       \
+    case 13: return CALL(13);                                       \
+    case 14: return CALL(14);                                       \
+    case 15: return CALL(15);                                       \
+    case 16: return CALL(16);                                       \
+    default:                                                        \
+      set_error("lx1=%d outside the supported range 2..16", np);    \
+      return NKH_ERR_UNSUPPORTED;                                   \
+  }
+
+}  // namespace nekamd

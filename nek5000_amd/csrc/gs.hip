@@ -1,0 +1,196 @@
+// Gather-scatter (dssum / dsop) on one GPU.
+//
+// Reference semantics: dssum.f:33-98 calls fgslib_gs_op(gsh,u,1,1,0): every
+// local node with nonzero glo_num receives the sum over all local nodes with
+// the same id; glo_num == 0 nodes are untouched (setvert3d gives element
+// interiors id 0, navier8.f:2337-2347).
+//
+// Layout: at setup the nonzero ids are sorted (id, local index) on the host;
+// groups of >= 2 members become segments (off/idx, int32, ordered by id so
+// that the segments of one face or edge are adjacent), unpartnered ids become
+// `single`.  One thread reduces one segment in index order, so results are
+// deterministic and independent of the launch shape; no atomics are needed
+// because every segment is owned by exactly one thread.
+#include <algorithm>
+#include <climits>
+#include <numeric>
+#include <vector>
+
+#include "nekamd_internal.h"
+
+namespace nekamd {
+
+int gs_build(GsMap &gs, const int64_t *glo, int64_t n, int np, int64_t nel) {
+  NK_CHECK(n < ((int64_t)1 << 31), NKH_ERR_ARG, "gs: %lld local nodes exceed int32 indexing",
+           (long long)n);
+  gs_free(gs);
+  // Group nonzero ids.  Ids from setvert3d are dense in [1, ngv], so a
+  // counting sort (stable in the local index) does it in O(n); arbitrary ids
+  // fall back to a comparison sort of (id, index) pairs.  Both give groups in
+  // increasing id order with members in increasing local index.
+  std::vector<int32_t> off, idx, single;
+  int maxm = 1;
+  int64_t gmax = 0, gmin = INT64_MAX, nnz = 0;
+  for (int64_t i = 0; i < n; ++i)
+    if (glo[i] != 0) {
+      gmax = std::max(gmax, glo[i]);
+      gmin = std::min(gmin, glo[i]);
+      ++nnz;
+    }
+  auto emit = [&](const int32_t *mem, int64_t cnt) {
+    if (cnt >= 2) {
+      off.push_back((int32_t)idx.size());
+      idx.insert(idx.end(), mem, mem + cnt);
+      maxm = std::max(maxm, (int)cnt);
+    } else if (cnt == 1) {
+      single.push_back(mem[0]);
+    }
+  };
+  if (nnz > 0 && gmin > 0 && gmax <= 4 * n + 16) {
+    std::vector<int32_t> cnt((size_t)gmax + 2, 0);
+    for (int64_t i = 0; i < n; ++i)
+      if (glo[i] != 0) ++cnt[(size_t)glo[i] + 1];
+    for (size_t k = 1; k < cnt.size(); ++k) cnt[k] += cnt[k - 1];
+    std::vector<int32_t> perm((size_t)nnz);
+    std::vector<int32_t> pos(cnt.begin(), cnt.end() - 1);
+    for (int64_t i = 0; i < n; ++i)
+      if (glo[i] != 0) perm[(size_t)pos[(size_t)glo[i]]++] = (int32_t)i;
+    idx.reserve((size_t)nnz);
+    for (int64_t g = 1; g <= gmax; ++g)
+      emit(perm.data() + cnt[(size_t)g], cnt[(size_t)g + 1] - cnt[(size_t)g]);
+  } else if (nnz > 0) {
+    std::vector<std::pair<int64_t, int32_t>> p;
+    p.reserve((size_t)nnz);
+    for (int64_t i = 0; i < n; ++i)
+      if (glo[i] != 0) p.emplace_back(glo[i], (int32_t)i);
+    std::sort(p.begin(), p.end());
+    std::vector<int32_t> mem;
+    for (size_t a = 0; a < p.size();) {
+      size_t b = a;
+      mem.clear();
+      while (b < p.size() && p[b].first == p[a].first) mem.push_back(p[b++].second);
+      emit(mem.data(), (int64_t)mem.size());
+      a = b;
+    }
+  }
+  off.push_back((int32_t)idx.size());
+  // structural check used by the fused CG kernels: ids are zero exactly at
+  // element-interior nodes (then those nodes can be finished inside axhelm)
+  bool iz = (n == (int64_t)np * np * np * nel);
+  if (iz) {
+    const int np3 = np * np * np;
+    for (int64_t l = 0; l < n && iz; ++l) {
+      const int q = (int)(l % np3);
+      const int i = q % np, j = (q / np) % np, k = q / (np * np);
+      const bool interior = i > 0 && i < np - 1 && j > 0 && j < np - 1 && k > 0 && k < np - 1;
+      if (interior != (glo[l] == 0)) iz = false;
+    }
+  }
+  gs.n = n;
+  gs.nseg = (int64_t)off.size() - 1;
+  gs.nmem = (int64_t)idx.size();
+  gs.nsingle = (int64_t)single.size();
+  gs.interior_zero = iz;
+  gs.max_mult = maxm;
+  NK_HIP(hipMalloc(&gs.off, sizeof(int32_t) * off.size()));
+  NK_HIP(hipMemcpy(gs.off, off.data(), sizeof(int32_t) * off.size(), hipMemcpyHostToDevice));
+  if (!idx.empty()) {
+    NK_HIP(hipMalloc(&gs.idx, sizeof(int32_t) * idx.size()));
+    NK_HIP(hipMemcpy(gs.idx, idx.data(), sizeof(int32_t) * idx.size(), hipMemcpyHostToDevice));
+  }
+  if (!single.empty()) {
+    NK_HIP(hipMalloc(&gs.single, sizeof(int32_t) * single.size()));
+    NK_HIP(hipMemcpy(gs.single, single.data(), sizeof(int32_t) * single.size(),
+                     hipMemcpyHostToDevice));
+  }
+  return NKH_OK;
+}
+
+void gs_free(GsMap &gs) {
+  if (gs.off) (void)hipFree(gs.off);
+  if (gs.idx) (void)hipFree(gs.idx);
+  if (gs.single) (void)hipFree(gs.single);
+  gs = GsMap();
+}
+
+template <int OP>
+__device__ __forceinline__ double gs_combine(double a, double b) {
+  if constexpr (OP == 0) return a + b;
+  else if constexpr (OP == 1) return a * b;
+  else if constexpr (OP == 2) return b < a ? b : a;
+  else return b > a ? b : a;
+}
+
+template <int OP>
+__global__ void k_gs_op(double *__restrict__ u, const int32_t *__restrict__ off,
+                        const int32_t *__restrict__ idx, int64_t nseg) {
+  for (int64_t s = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; s < nseg;
+       s += (int64_t)gridDim.x * blockDim.x) {
+    const int32_t a = off[s], b = off[s + 1];
+    double acc = u[idx[a]];
+    for (int32_t c = a + 1; c < b; ++c) acc = gs_combine<OP>(acc, u[idx[c]]);
+    for (int32_t c = a; c < b; ++c) u[idx[c]] = acc;
+  }
+}
+
+int gs_op(const GsMap &gs, double *u, int op, hipStream_t s) {
+  if (gs.nseg == 0) return NKH_OK;
+  int grid = (int)((gs.nseg + 255) / 256);
+  if (grid > 65536) grid = 65536;
+  switch (op) {
+    case 0: hipLaunchKernelGGL(k_gs_op<0>, dim3(grid), dim3(256), 0, s, u, gs.off, gs.idx, gs.nseg); break;
+    case 1: hipLaunchKernelGGL(k_gs_op<1>, dim3(grid), dim3(256), 0, s, u, gs.off, gs.idx, gs.nseg); break;
+    case 2: hipLaunchKernelGGL(k_gs_op<2>, dim3(grid), dim3(256), 0, s, u, gs.off, gs.idx, gs.nseg); break;
+    case 3: hipLaunchKernelGGL(k_gs_op<3>, dim3(grid), dim3(256), 0, s, u, gs.off, gs.idx, gs.nseg); break;
+    default: set_error("gs_op: unknown op %d", op); return NKH_ERR_ARG;
+  }
+  NK_HIP(hipGetLastError());
+  return NKH_OK;
+}
+
+// dssum + mask + rho partial (hmholtz.f:797-801): for every nonzero-id node
+// w <- mask * sum_{same id} w, and sum(w*p*mult) over those nodes.
+__global__ void __launch_bounds__(256)
+    k_gs_sum_mask_dot(double *__restrict__ w, const double *__restrict__ mask,
+                      const double *__restrict__ p, const double *__restrict__ mult,
+                      const int32_t *__restrict__ off, const int32_t *__restrict__ idx,
+                      int64_t nseg, const int32_t *__restrict__ single, int64_t nsingle,
+                      const CgState *__restrict__ st, double *__restrict__ rho_part) {
+  __shared__ double sbuf[4];
+  if (st->done) return;
+  double rho = 0.0;
+  const int64_t total = nseg + nsingle;
+  for (int64_t s = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; s < total;
+       s += (int64_t)gridDim.x * blockDim.x) {
+    if (s < nseg) {
+      const int32_t a = off[s], b = off[s + 1];
+      double acc = w[idx[a]];
+      for (int32_t c = a + 1; c < b; ++c) acc += w[idx[c]];
+      for (int32_t c = a; c < b; ++c) {
+        const int32_t l = idx[c];
+        const double v = acc * mask[l];
+        w[l] = v;
+        rho += v * p[l] * mult[l];
+      }
+    } else {
+      const int32_t l = single[s - nseg];
+      const double v = w[l] * mask[l];
+      w[l] = v;
+      rho += v * p[l] * mult[l];
+    }
+  }
+  for (int o = 32; o > 0; o >>= 1) rho += __shfl_down(rho, o, 64);
+  if ((threadIdx.x & 63) == 0) sbuf[threadIdx.x >> 6] = rho;
+  __syncthreads();
+  if (threadIdx.x == 0) rho_part[blockIdx.x] = sbuf[0] + sbuf[1] + sbuf[2] + sbuf[3];
+}
+
+int gs_sum_mask_dot(const GsMap &gs, double *w, const double *mask, const double *p,
+                    const double *mult, const CgState *st, double *rho_part, hipStream_t s) {
+  hipLaunchKernelGGL(k_gs_sum_mask_dot, dim3(kRedBlocks), dim3(256), 0, s, w, mask, p, mult,
+                     gs.off, gs.idx, gs.nseg, gs.single, gs.nsingle, st, rho_part);
+  NK_HIP(hipGetLastError());
+  return NKH_OK;
+}
+
+}  // namespace nekamd

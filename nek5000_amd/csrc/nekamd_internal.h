@@ -1,0 +1,151 @@
+// Internal declarations shared by the translation units of libnekamd.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstdio>
+
+#include "../../include/nekamd.h"
+
+namespace nekamd {
+
+// GLL points per direction the kernels are instantiated for: 2..kMaxNP.
+constexpr int kMaxNP = 16;
+// Fixed grid of the reduction kernels: partial sums have this many slots, so
+// every reduction sums the same partials in the same order (deterministic).
+constexpr int kRedBlocks = 2048;
+
+void set_error(const char *fmt, ...);
+
+#define NK_HIP(call)                                                              \
+  do {                                                                            \
+    hipError_t e_ = (call);                                                       \
+    if (e_ != hipSuccess) {                                                       \
+      ::nekamd::set_error("%s:%d %s: %s", __FILE__, __LINE__, #call,              \
+                          hipGetErrorString(e_));                                 \
+      return NKH_ERR_HIP;                                                         \
+    }                                                                             \
+  } while (0)
+
+#define NK_CHECK(cond, code, ...)                                                 \
+  do {                                                                            \
+    if (!(cond)) {                                                                \
+      ::nekamd::set_error(__VA_ARGS__);                                           \
+      return (code);                                                              \
+    }                                                                             \
+  } while (0)
+
+#define NK_TRY(call)                                                              \
+  do {                                                                            \
+    int rc_ = (call);                                                             \
+    if (rc_ != NKH_OK) return rc_;                                                \
+  } while (0)
+
+// Gather-scatter map over the local nodes of one mesh (gslib gs_op semantics
+// on one rank).  Nodes with a nonzero global id are grouped by id, groups in
+// increasing id order.  off/idx list the groups of two or more members;
+// single lists the nonzero-id nodes without partner.
+struct GsMap {
+  int64_t n = 0;
+  int64_t nseg = 0;
+  int64_t nmem = 0;
+  int64_t nsingle = 0;
+  int32_t *off = nullptr;     // device, nseg+1
+  int32_t *idx = nullptr;     // device, nmem
+  int32_t *single = nullptr;  // device, nsingle
+  bool interior_zero = false; // glo_num == 0 exactly at element-interior nodes
+  int max_mult = 0;
+};
+
+// State of one PCG solve, resident on the device.  The scalar kernels update
+// it; the vector kernels read alpha/beta from it and return at once when
+// `done` is set, so the host can enqueue iterations ahead without reading
+// the residual back every iteration.
+struct CgState {
+  double rtz1, rtz2, rho, alpha, beta;
+  double rbn0, rbn2, tol, tin, vol, param22;
+  int iter;       // current iteration number (1-based)
+  int maxit;      // min(maxit, 900)
+  int done;
+  int niter;
+  int converged;
+  int pending;    // x/r update of the previous iteration not yet applied
+  int istep;
+  int param18;
+};
+
+struct Mesh {
+  int np = 0;        // GLL points per direction (lx1)
+  int64_t nel = 0;
+  int64_t n = 0;     // np^3 * nel
+  double D[kMaxNP * kMaxNP];   // D(i,j) column-major, as dxm1
+  double w[kMaxNP];            // 1-D GLL weights
+  double z[kMaxNP];            // 1-D GLL points
+  double *Dd = nullptr;        // device copy of D
+  double *g = nullptr;         // device, 6*n, factor-major: g[f*n + l]
+  double *bm1 = nullptr;       // device
+  double *binv = nullptr;      // device
+  double vol = 0.0;            // sum of bm1
+  bool has_geom = false;
+  GsMap gs;
+  bool has_gs = false;
+  // solve workspace (allocated on first solve)
+  double *ws = nullptr;
+  size_t ws_doubles = 0;
+  CgState *st = nullptr;       // device
+  double *red = nullptr;       // device, partial sums
+};
+
+// CTIMER-style accumulators (seconds, calls), see nkh_timers_*.
+struct Timers {
+  bool on = false;
+  double taxhm = 0, tdsum = 0, thmhz = 0;
+  int64_t naxhm = 0, ndsum = 0, nhmhz = 0;
+};
+Timers &timers();
+
+Mesh *get_mesh(int id);
+hipStream_t cur_stream();
+int ensure_workspace(Mesh &m);  // solve vectors + state + partial sums
+int ensure_small(Mesh &m);      // state + partial sums only
+
+// ---- kernels.hip ----
+// Coefficient description: field pointer or uniform scalar.
+struct Coef {
+  const double *f = nullptr;  // field (device) or nullptr
+  double s = 0.0;             // value when uniform
+  bool uniform() const { return f == nullptr; }
+};
+
+int launch_axhelm(const Mesh &m, double *au, const double *u, Coef h1, Coef h2,
+                  hipStream_t s);
+int launch_geom(Mesh &m, const double *x, const double *y, const double *z, hipStream_t s);
+int launch_setprec(const Mesh &m, double *d, Coef h1, Coef h2, hipStream_t s);
+// axhelm fused into the CG iteration: p = d*r + beta*p; w = A p (h1/h2 as
+// above), rho partial over element-interior nodes when rho_part != nullptr.
+int launch_axhelm_cg(const Mesh &m, double *w, double *p, const double *r, const double *d,
+                     Coef h1, Coef h2, const double *mult, const CgState *st,
+                     double *rho_part, hipStream_t s);
+
+// ---- gs.hip ----
+int gs_build(GsMap &gs, const int64_t *glo_num_host, int64_t n, int np, int64_t nel);
+void gs_free(GsMap &gs);
+int gs_op(const GsMap &gs, double *u, int op, hipStream_t s);
+// gs '+' on w fused with w*=mask on all nonzero-id nodes and the rho partial
+// sum over them (w*p*mult); skipped when st->done.
+int gs_sum_mask_dot(const GsMap &gs, double *w, const double *mask, const double *p,
+                    const double *mult, const CgState *st, double *rho_part,
+                    hipStream_t s);
+
+// ---- cg.hip ----
+int run_cggo(Mesh &m, double *x, const double *f, Coef h1, Coef h2, const double *mask,
+             const double *mult, double tin, int maxit, const nkh_params &prm,
+             nkh_result *res, double *history_host);
+int run_chktcg1(Mesh &m, double *tol, const double *res, Coef h1, Coef h2,
+                const double *mask, const double *mult);
+int classify_coefs(Mesh &m, const double *h1, const double *h2, Coef *c1, Coef *c2,
+                   hipStream_t s);
+int run_setprec(Mesh &m, double *d, Coef h1, Coef h2, hipStream_t s);
+int mul_inplace(double *a, const double *b, int64_t n, hipStream_t s);
+
+}  // namespace nekamd

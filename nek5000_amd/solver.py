@@ -1,0 +1,230 @@
+"""Host-side mirror of Nek5000's Helmholtz solve interface over libnekamd.
+
+Names and argument meanings follow the reference routines:
+
+  * ``Field.axhelm(au, u, h1, h2, imesh=1, isd=1)``  -- hmholtz.f:72
+  * ``Field.dssum(u)`` / ``Field.dsop(u, op)``        -- dssum.f:33 / :100
+  * ``Field.setprec(d, h1, h2)``                      -- hmholtz.f:380
+  * ``Field.hmholtz(name, u, rhs, h1, h2, mask, mult, tli, maxit, isd)``
+                                                      -- hmholtz.f:2
+  * ``Field.cggo(x, f, h1, h2, mask, mult, tin, maxit)`` -- hmholtz.f:611
+
+Arrays are torch tensors on the GPU (float64, Nek layout (nel,lz1,ly1,lx1)
+C-order = (lx1,ly1,lz1,nel) column-major).  torch is only the allocator and
+the stream: every operation runs in libnekamd's HIP kernels.  Errors raise
+NekAmdError (Nek would call exitti).
+"""
+import ctypes
+
+import numpy as np
+
+from . import _lib
+from ._lib import NkhParams, NkhResult, check, load
+from .mesh import BoxMesh
+
+_initialised = {"device": None}
+
+
+def _torch():
+    import torch
+
+    return torch
+
+
+def init(device=0):
+    lib = load()
+    if _initialised["device"] is None:
+        torch = _torch()
+        if not torch.cuda.is_available():
+            raise _lib.NekAmdError("no GPU visible: libnekamd runs on the GPU only")
+        torch.cuda.set_device(device)
+        check(lib.nkh_init(device), "nkh_init")
+        _initialised["device"] = device
+    return lib
+
+
+def _sync_stream(lib):
+    torch = _torch()
+    check(lib.nkh_set_stream(ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)),
+          "nkh_set_stream")
+
+
+def _ptr(t):
+    torch = _torch()
+    if not isinstance(t, torch.Tensor) or not t.is_cuda:
+        raise TypeError("expected a CUDA tensor")
+    if t.dtype != torch.float64 or not t.is_contiguous():
+        raise TypeError("expected a contiguous float64 tensor")
+    return ctypes.c_void_p(t.data_ptr())
+
+
+def default_params(istep=1, param18=0, param22=1e-7, restol=1e-7, precision=0):
+    """Nek defaults (reader_par.f:26-27,76-79): param(22)=1e-7, restol=param(22)."""
+    return NkhParams(istep, param18, param22, restol, precision, 0)
+
+
+class Field:
+    """One element set with geometry, numbering, gs map, mask and multiplicity
+    resident on the GPU (Nek's imsh mesh plus its v?mask/vmult)."""
+
+    def __init__(self, lx1, nel, device=0):
+        self.lib = init(device)
+        self.lx1 = lx1
+        self.nel = nel
+        self.n = nel * lx1 ** 3
+        h = ctypes.c_int(-1)
+        check(self.lib.nkh_mesh_create(lx1, nel, ctypes.byref(h)), "nkh_mesh_create")
+        self.handle = h.value
+        self.mask = None
+        self.mult = None
+        self.glo_num = None
+        self.ngv = None
+
+    @property
+    def shape(self):
+        return (self.nel, self.lx1, self.lx1, self.lx1)
+
+    def zeros(self):
+        torch = _torch()
+        return torch.zeros(self.shape, dtype=torch.float64, device="cuda")
+
+    def to_dev(self, a):
+        torch = _torch()
+        return torch.as_tensor(np.ascontiguousarray(a, dtype=np.float64).reshape(self.shape),
+                               device="cuda")
+
+    # ---- setup ----------------------------------------------------------
+    def geom_from_xyz(self, x, y, z):
+        """coef.f glmapm1 + geodat1 on the GPU from GLL coordinates."""
+        xd, yd, zd = (self.to_dev(a) if not hasattr(a, "is_cuda") else a for a in (x, y, z))
+        _sync_stream(self.lib)
+        check(self.lib.nkd_mesh_geom_from_xyz(self.handle, _ptr(xd), _ptr(yd), _ptr(zd)),
+              "nkd_mesh_geom_from_xyz")
+
+    def set_geom(self, g, bm1):
+        """Upload precomputed g1m1..g6m1 (list of 6 host arrays) and bm1."""
+        arrs = [np.ascontiguousarray(a, dtype=np.float64).reshape(-1) for a in list(g) + [bm1]]
+        check(self.lib.nkh_mesh_set_geom(self.handle, *[a.ctypes.data for a in arrs]),
+              "nkh_mesh_set_geom")
+
+    def gs_setup(self, glo_num):
+        glo = np.ascontiguousarray(glo_num, dtype=np.int64).reshape(-1)
+        assert glo.size == self.n
+        check(self.lib.nkh_mesh_gs_setup(self.handle, glo.ctypes.data), "nkh_mesh_gs_setup")
+        self.glo_num = glo
+
+    def info(self):
+        n = ctypes.c_int64()
+        vol = ctypes.c_double()
+        nseg = ctypes.c_int64()
+        nsingle = ctypes.c_int64()
+        mm = ctypes.c_int()
+        check(self.lib.nkh_mesh_info(self.handle, ctypes.byref(n), ctypes.byref(vol),
+                                     ctypes.byref(nseg), ctypes.byref(nsingle),
+                                     ctypes.byref(mm)), "nkh_mesh_info")
+        return dict(n=n.value, vol=vol.value, nseg=nseg.value, nsingle=nsingle.value,
+                    max_mult=mm.value)
+
+    def mass(self):
+        """(bm1, binvm1) as tensors (copies of the library's device arrays)."""
+        pb = ctypes.c_void_p()
+        pi = ctypes.c_void_p()
+        check(self.lib.nkh_mesh_ptrs(self.handle, ctypes.byref(pb), ctypes.byref(pi)),
+              "nkh_mesh_ptrs")
+        bm1 = self.zeros()
+        binv = self.zeros()
+        _copy_raw_to_tensor(pb.value, bm1)
+        _copy_raw_to_tensor(pi.value, binv)
+        return bm1, binv
+
+    # ---- operators ------------------------------------------------------
+    def axhelm(self, au, u, h1, h2, imesh=1, isd=1):
+        _sync_stream(self.lib)
+        check(self.lib.nkd_axhelm(self.handle, _ptr(au), _ptr(u), _ptr(h1), _ptr(h2), isd),
+              "nkd_axhelm")
+
+    def dssum(self, u):
+        _sync_stream(self.lib)
+        check(self.lib.nkd_dssum(self.handle, _ptr(u)), "nkd_dssum")
+
+    def dsop(self, u, op):
+        _sync_stream(self.lib)
+        check(self.lib.nkd_dsop(self.handle, _ptr(u), op.encode()[:1]), "nkd_dsop")
+
+    def setprec(self, d, h1, h2):
+        _sync_stream(self.lib)
+        check(self.lib.nkd_setprec(self.handle, _ptr(d), _ptr(h1), _ptr(h2)), "nkd_setprec")
+
+    def hmholtz(self, name, u, rhs, h1, h2, mask, mult, tli, maxit, isd=1, params=None,
+                history=False):
+        """hmholtz.f:2: rhs is dssum'd and masked in place; u gets the solution.
+        Returns a dict with niter, rbn0, rbn2, tol (and history if asked)."""
+        prm = params if params is not None else default_params()
+        res = NkhResult()
+        hist = np.zeros(5 * 900) if history else None
+        _sync_stream(self.lib)
+        check(self.lib.nkd_hmholtz(self.handle, name.encode(), _ptr(u), _ptr(rhs), _ptr(h1),
+                                   _ptr(h2), _ptr(mask), _ptr(mult), float(tli), int(maxit),
+                                   isd, ctypes.byref(prm), ctypes.byref(res),
+                                   ctypes.c_void_p(hist.ctypes.data) if history else None),
+              "nkd_hmholtz")
+        out = dict(niter=res.niter, converged=res.converged, rbn0=res.rbn0, rbn2=res.rbn2,
+                   tol=res.tol)
+        if history:
+            out["history"] = hist.reshape(900, 5)[: max(res.niter + 1, 1)].copy()
+        return out
+
+    def cggo(self, x, f, h1, h2, mask, mult, tin, maxit, params=None, history=False):
+        prm = params if params is not None else default_params()
+        res = NkhResult()
+        hist = np.zeros(5 * 900) if history else None
+        _sync_stream(self.lib)
+        check(self.lib.nkd_cggo(self.handle, _ptr(x), _ptr(f), _ptr(h1), _ptr(h2), _ptr(mask),
+                                _ptr(mult), float(tin), int(maxit), ctypes.byref(prm),
+                                ctypes.byref(res),
+                                ctypes.c_void_p(hist.ctypes.data) if history else None),
+              "nkd_cggo")
+        out = dict(niter=res.niter, converged=res.converged, rbn0=res.rbn0, rbn2=res.rbn2,
+                   tol=res.tol)
+        if history:
+            out["history"] = hist.reshape(900, 5)[: max(res.niter + 1, 1)].copy()
+        return out
+
+    def destroy(self):
+        if self.handle is not None and self.handle >= 0:
+            check(self.lib.nkh_mesh_destroy(self.handle), "nkh_mesh_destroy")
+            self.handle = None
+
+
+def _copy_raw_to_tensor(ptr, t):
+    """Copy n doubles from a library-owned device pointer into tensor t."""
+    import torch
+
+    nbytes = t.numel() * 8
+    hip = ctypes.CDLL("libamdhip64.so")
+    hip.hipMemcpy.restype = ctypes.c_int
+    hip.hipMemcpy.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t, ctypes.c_int]
+    torch.cuda.synchronize()
+    rc = hip.hipMemcpy(ctypes.c_void_p(t.data_ptr()), ctypes.c_void_p(ptr), nbytes, 3)
+    if rc != 0:
+        raise _lib.NekAmdError(f"hipMemcpy failed: {rc}")
+
+
+def box_field(box: BoxMesh, device=0):
+    """Set up a Field for a BoxMesh the way nek_init does (drive1.f:2-179):
+    numbering (setvert3d), geometry (geodat1, on the GPU), gs map, binvm1,
+    Dirichlet mask (bcmask: facev then dsop '*'), vmult = 1/dssum(1)."""
+    f = Field(box.lx1, box.nel, device)
+    glo, ngv = box.glo_num()
+    f.ngv = ngv
+    x, y, z = box.xyz()
+    f.geom_from_xyz(x, y, z)
+    f.gs_setup(glo)
+    mask = f.to_dev(box.face_mask())
+    f.dsop(mask, "*")
+    mult = f.zeros() + 1.0
+    f.dssum(mult)
+    mult = 1.0 / mult
+    f.mask = mask
+    f.mult = mult
+    return f

@@ -1,0 +1,250 @@
+"""ctypes front end of the CPU oracle (oracle/nek_oracle.c).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and
+bench.py's cpu_baseline leg -- never by the nek5000_amd package.
+
+Arrays use Nek's layout: a field is a float64 numpy array of shape
+(nel, lz1, ly1, lx1) in C order (= (lx1,ly1,lz1,nel) column-major).
+Small matrices are numpy arrays D[i, j] = D(i,j).
+"""
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB = os.path.join(HERE, "libnekoracle.so")
+REF_SPECLIB = os.path.join(HERE, "_ref", "libspeclib_ref.so")
+
+c_int = ctypes.c_int
+c_int64 = ctypes.c_int64
+c_double = ctypes.c_double
+vp = ctypes.c_void_p
+
+
+def build(ref=False):
+    """Compile the oracle (and, when the Nek5000 sources are present, the
+    reference speclib.f) with oracle/Makefile."""
+    subprocess.run(["make", "-s", "-C", HERE, "libnekoracle.so"], check=True)
+    if ref and os.path.isdir(os.environ.get("NEKREF", "/root/reference")):
+        subprocess.run(["make", "-s", "-C", HERE, "ref"], check=True)
+
+
+class _Field(ctypes.Structure):
+    _fields_ = [
+        ("n", c_int), ("nel", c_int64),
+        ("dxm1", vp), ("dxtm1", vp),
+        ("g1", vp), ("g2", vp), ("g3", vp), ("g4", vp), ("g5", vp), ("g6", vp), ("bm1", vp),
+        ("binv", vp), ("mask", vp), ("mult", vp),
+        ("vol", c_double),
+        ("gs", vp),
+    ]
+
+
+class CgRec(ctypes.Structure):
+    _fields_ = [("rbn2", c_double), ("rtz1", c_double), ("rho", c_double),
+                ("alpha", c_double), ("beta", c_double)]
+
+
+class Params(ctypes.Structure):
+    _fields_ = [("istep", c_int), ("param18", c_double), ("param22", c_double),
+                ("restol", c_double)]
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB):
+            build()
+        L = ctypes.CDLL(LIB)
+        L.nko_zwgll.argtypes = [vp, vp, c_int]
+        L.nko_dgll.argtypes = [vp, vp, vp, c_int]
+        L.nko_pnleg.argtypes = [c_double, c_int]
+        L.nko_pnleg.restype = c_double
+        L.nko_igllm.argtypes = [vp, vp, vp, vp, c_int, c_int]
+        L.nko_geom.argtypes = [c_int, c_int, vp, vp, vp, vp, vp] + [vp] * 8
+        L.nko_setvert3d.argtypes = [vp, c_int, c_int, vp]
+        L.nko_setvert3d.restype = c_int64
+        L.nko_gs_setup.argtypes = [vp, c_int64]
+        L.nko_gs_setup.restype = vp
+        L.nko_gs_free.argtypes = [vp]
+        L.nko_gs_op.argtypes = [vp, vp, c_int]
+        L.nko_axhelm.argtypes = [ctypes.POINTER(_Field), vp, vp, vp, vp]
+        L.nko_setprec.argtypes = [ctypes.POINTER(_Field), vp, vp, vp]
+        L.nko_chktcg1.argtypes = [ctypes.POINTER(_Field), c_double, vp, vp, vp]
+        L.nko_chktcg1.restype = c_double
+        L.nko_cggo.argtypes = [ctypes.POINTER(_Field), vp, vp, vp, vp, c_double, c_int,
+                               ctypes.POINTER(Params), vp, ctypes.POINTER(c_double)]
+        L.nko_cggo.restype = c_int
+        L.nko_hmholtz.argtypes = [ctypes.POINTER(_Field), vp, vp, vp, vp, c_double, c_int,
+                                  ctypes.POINTER(Params), vp, ctypes.POINTER(c_double)]
+        L.nko_hmholtz.restype = c_int
+        L.nko_num_threads.restype = c_int
+        L.nko_set_num_threads.argtypes = [c_int]
+        _lib = L
+    return _lib
+
+
+def _p(a):
+    return a.ctypes.data
+
+
+def _c(a, dtype=np.float64):
+    return np.ascontiguousarray(a, dtype=dtype)
+
+
+# ---- speclib ----
+def zwgll(np_):
+    z = np.zeros(np_)
+    w = np.zeros(np_)
+    lib().nko_zwgll(_p(z), _p(w), np_)
+    return z, w
+
+
+def dgll(z):
+    n = len(z)
+    d = np.zeros(n * n)
+    dt = np.zeros(n * n)
+    lib().nko_dgll(_p(d), _p(dt), _p(_c(z)), n)
+    return d.reshape(n, n).T.copy()  # D[i, j]
+
+
+def pnleg(z, n):
+    return lib().nko_pnleg(float(z), int(n))
+
+
+def igllm(z1, z2):
+    n1, n2 = len(z1), len(z2)
+    i12 = np.zeros(n1 * n2)
+    it12 = np.zeros(n1 * n2)
+    lib().nko_igllm(_p(i12), _p(it12), _p(_c(z1)), _p(_c(z2)), n1, n2)
+    return i12.reshape(n1, n2).T.copy()  # I[i, j] = h_j(z2_i)
+
+
+def ref_speclib():
+    """The reference's own speclib.f built by flang (oracle/_ref); None if absent."""
+    if not os.path.exists(REF_SPECLIB):
+        return None
+    L = ctypes.CDLL(REF_SPECLIB, mode=os.RTLD_LAZY)
+    return L
+
+
+# ---- geometry ----
+def geom(x, y, z, D, w):
+    """(g[6], bm1, jac) from GLL coordinates (coef.f xyzrst/glmapm1/geodat1)."""
+    x, y, z = _c(x), _c(y), _c(z)
+    nel = x.shape[0]
+    n = x.shape[-1]
+    Dc = _c(np.asarray(D).T)  # column-major
+    out = [np.zeros_like(x) for _ in range(8)]
+    lib().nko_geom(n, nel, _p(x), _p(y), _p(z), _p(Dc), _p(_c(w)), *[_p(a) for a in out])
+    return out[:6], out[6], out[7]
+
+
+# ---- numbering and gather-scatter ----
+def setvert3d(vertex, nx):
+    vertex = _c(vertex, np.int64)
+    nel = vertex.shape[0]
+    glo = np.zeros(nel * nx ** 3, dtype=np.int64)
+    ngv = lib().nko_setvert3d(_p(glo), nx, nel, _p(vertex))
+    return glo, int(ngv)
+
+
+class GS:
+    def __init__(self, glo_num):
+        self.glo = _c(glo_num, np.int64).reshape(-1)
+        self.h = lib().nko_gs_setup(_p(self.glo), self.glo.size)
+
+    def op(self, u, op="+"):
+        """In place on a contiguous float64 array."""
+        k = {"+": 0, "*": 1, "m": 2, "M": 3}[op]
+        assert u.dtype == np.float64 and u.flags.c_contiguous and u.size == self.glo.size
+        lib().nko_gs_op(self.h, _p(u), k)
+        return u
+
+    def __del__(self):
+        try:
+            lib().nko_gs_free(self.h)
+        except Exception:
+            pass
+
+
+class Field:
+    """Everything Nek's commons hold for one field, on the host."""
+
+    def __init__(self, D, g, bm1, glo_num, mask, mult=None):
+        self.D = _c(D)
+        self.Dc = _c(self.D.T)      # dxm1 column-major
+        self.Dtc = _c(self.D)       # dxtm1 column-major
+        self.g = [_c(a) for a in g]
+        self.bm1 = _c(bm1)
+        self.gs = GS(glo_num)
+        self.mask = _c(mask)
+        self.nel = self.bm1.shape[0]
+        self.n = self.bm1.shape[-1]
+        if mult is None:
+            m = np.ones_like(self.bm1)
+            self.gs.op(m, "+")
+            mult = 1.0 / m
+        self.mult = _c(mult)
+        b = self.bm1.copy()
+        self.gs.op(b, "+")
+        self.binv = 1.0 / b
+        self.vol = float(np.sum(self.bm1))
+        self._s = _Field(self.n, self.nel, _p(self.Dc), _p(self.Dtc),
+                         *[_p(a) for a in self.g], _p(self.bm1), _p(self.binv),
+                         _p(self.mask), _p(self.mult), self.vol, self.gs.h)
+
+    def axhelm(self, u, h1, h2):
+        au = np.zeros_like(self.bm1)
+        lib().nko_axhelm(ctypes.byref(self._s), _p(au), _p(_c(u)), _p(_c(h1)), _p(_c(h2)))
+        return au
+
+    def dssum(self, u):
+        u = _c(u).copy()
+        return self.gs.op(u, "+")
+
+    def setprec(self, h1, h2):
+        d = np.zeros_like(self.bm1)
+        lib().nko_setprec(ctypes.byref(self._s), _p(d), _p(_c(h1)), _p(_c(h2)))
+        return d
+
+    def chktcg1(self, tol, res, h1, h2):
+        return lib().nko_chktcg1(ctypes.byref(self._s), float(tol), _p(_c(res)), _p(_c(h1)),
+                                 _p(_c(h2)))
+
+    def cggo(self, f, h1, h2, tin, maxit, params=None):
+        prm = params or Params(1, 0.0, 1e-7, 1e-7)
+        x = np.zeros_like(self.bm1)
+        rec = (CgRec * 901)()
+        rbn0 = c_double(0)
+        niter = lib().nko_cggo(ctypes.byref(self._s), _p(x), _p(_c(f)), _p(_c(h1)),
+                               _p(_c(h2)), float(tin), int(maxit), ctypes.byref(prm),
+                               ctypes.addressof(rec), ctypes.byref(rbn0))
+        hist = np.array([[r.rbn2, r.rtz1, r.rho, r.alpha, r.beta] for r in rec[: niter + 1]])
+        return x, niter, hist, rbn0.value
+
+    def hmholtz(self, rhs, h1, h2, tli, maxit, params=None):
+        """hmholtz.f:2 (non-PRES): returns (u, rhs_after, niter, history, rbn0)."""
+        prm = params or Params(1, 0.0, 1e-7, 1e-7)
+        u = np.zeros_like(self.bm1)
+        r = _c(rhs).copy()
+        rec = (CgRec * 901)()
+        rbn0 = c_double(0)
+        niter = lib().nko_hmholtz(ctypes.byref(self._s), _p(u), _p(r), _p(_c(h1)), _p(_c(h2)),
+                                  float(tli), int(maxit), ctypes.byref(prm),
+                                  ctypes.addressof(rec), ctypes.byref(rbn0))
+        hist = np.array([[q.rbn2, q.rtz1, q.rho, q.alpha, q.beta] for q in rec[: niter + 1]])
+        return u, r, niter, hist, rbn0.value
+
+
+def num_threads():
+    return lib().nko_num_threads()
+
+
+def set_num_threads(t):
+    lib().nko_set_num_threads(int(t))
