@@ -61,8 +61,10 @@ int nkh_init(int device);
 int nkh_finalize(void);
 const char *nkh_last_error(void);
 /* Run subsequent work on the caller's stream (hipStream_t passed as void*);
-   NULL restores the library stream. */
+   NULL selects HIP's null stream.  nkh_init starts on a library-owned
+   stream, which nkh_use_library_stream() selects again. */
 int nkh_set_stream(void *hip_stream);
+int nkh_use_library_stream(void);
 int nkh_synchronize(void);
 
 /* ---- basis and numbering (host) --------------------------------------- */

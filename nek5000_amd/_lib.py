@@ -45,6 +45,7 @@ SIGNATURES = {
     "nkh_finalize": (c_int, []),
     "nkh_last_error": (c_char_p, []),
     "nkh_set_stream": (c_int, [c_void_p]),
+    "nkh_use_library_stream": (c_int, []),
     "nkh_synchronize": (c_int, []),
     "nkh_zwgll": (c_int, [c_void_p, c_void_p, c_int]),
     "nkh_dgll": (c_int, [c_void_p, c_void_p, c_void_p, c_int]),
@@ -92,6 +93,14 @@ def load(path=LIB_PATH):
         raise NekAmdError(
             f"{path} is missing: the HIP library is the only implementation; "
             "build it with `python -m nek5000_amd.build`")
+    # One HIP runtime per process: torch ships its own libamdhip64.so.7 and
+    # loads it by file name, so it must be loaded first; libnekamd's
+    # libamdhip64.so.7 dependency then resolves to that already-loaded copy
+    # instead of bringing up a second runtime.
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
     lib = ctypes.CDLL(path)
     for name, (res, args) in SIGNATURES.items():
         fn = getattr(lib, name)

@@ -528,7 +528,13 @@ int nkh_finalize(void) {
 
 int nkh_set_stream(void *s) {
   NK_CHECK(g_init, NKH_ERR_STATE, "nkh_set_stream before nkh_init");
-  g_stream = s ? (hipStream_t)s : g_lib_stream;
+  g_stream = (hipStream_t)s;  // NULL is HIP's null stream
+  return NKH_OK;
+}
+
+int nkh_use_library_stream(void) {
+  NK_CHECK(g_init, NKH_ERR_STATE, "nkh_use_library_stream before nkh_init");
+  g_stream = g_lib_stream;
   return NKH_OK;
 }
 

