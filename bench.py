@@ -95,6 +95,8 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--cpu-nel", type=int, default=16)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--maxit", type=int, default=900,
+                    help="iteration cap per solve (900 = the reference's maxcg)")
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
@@ -125,7 +127,21 @@ def main():
 
     def solve(r):
         u = torch.zeros_like(rhs0)
-        return F.hmholtz("TEMP", u, r, h1, h2, F.mask, F.mult, -args.tol, 900)
+        return F.hmholtz("TEMP", u, r, h1, h2, F.mask, F.mult, -args.tol, args.maxit)
+
+    # plain copy of 2 GiB by torch: the achievable-bandwidth reference for
+    # the roofline numbers (and the calibration kernel of the PMC byte counts)
+    src = torch.empty(2 ** 28, dtype=torch.float64, device="cuda").fill_(1.0)
+    dst = torch.empty_like(src)
+    dst.copy_(src)
+    torch.cuda.synchronize()
+    tc = time.perf_counter()
+    for _ in range(5):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    copy_gbs = 5 * 2 * src.numel() * 8 / (time.perf_counter() - tc) / 1e9
+    del src, dst
+    log(f"[bench] torch copy: {copy_gbs:.0f} GB/s (read+write)")
 
     for w in range(args.warmup):
         out = solve(rhs0.clone())
@@ -185,6 +201,7 @@ def main():
             "free_dof": free_dof, "iterations_per_solve": iters,
             "axhelm_ms": t_ax * 1e3, "gs_ms": t_gs * 1e3,
             "ms_per_iteration": elapsed / total_iters * 1e3,
+            "copy_GBs": copy_gbs,
         },
         "roofline": {
             "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",

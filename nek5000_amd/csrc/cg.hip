@@ -24,7 +24,7 @@ namespace nekamd {
 
 namespace {
 
-constexpr int kMaxCg = 900;  // hmholtz.f:636 maxcg
+constexpr int kMaxCg = kMaxCgIter;
 
 __device__ __forceinline__ double wave_sum(double v) {
   for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
@@ -167,7 +167,9 @@ __global__ void k_cg_init(double *__restrict__ x, double *__restrict__ r,
 }
 
 // x += alpha p ; r -= alpha w (when an update is pending) ; partial sums of
-// (d*r)*r*mult and binv*mult*r*r (vlsc3 / vlsc32, hmholtz.f:754-759)
+// (d*r)*r*mult and binv*mult*r*r (vlsc3 / vlsc32, hmholtz.f:754-759).
+// VEC: 16-byte loads of node pairs (all six arrays 16-byte aligned).
+template <bool VEC>
 __global__ void __launch_bounds__(256)
     k_cg_update_dots(double *__restrict__ x, double *__restrict__ r,
                      const double *__restrict__ p, const double *__restrict__ w,
@@ -178,8 +180,39 @@ __global__ void __launch_bounds__(256)
   const bool upd = st->pending != 0;
   const double alpha = st->alpha;
   double s1 = 0.0, s2 = 0.0;
-  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
-       i += (int64_t)gridDim.x * blockDim.x) {
+  const int64_t tid = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  int64_t done_upto = 0;
+  if constexpr (VEC) {
+    const int64_t n2 = n >> 1;
+    double2 *x2 = reinterpret_cast<double2 *>(x);
+    double2 *r2 = reinterpret_cast<double2 *>(r);
+    const double2 *p2 = reinterpret_cast<const double2 *>(p);
+    const double2 *w2 = reinterpret_cast<const double2 *>(w);
+    const double2 *dm2 = reinterpret_cast<const double2 *>(dm);
+    const double2 *bm2 = reinterpret_cast<const double2 *>(bm);
+    for (int64_t i = tid; i < n2; i += stride) {
+      double2 ri = r2[i];
+      if (upd) {
+        double2 xi = x2[i];
+        const double2 pi = p2[i], wi = w2[i];
+        xi.x = xi.x + alpha * pi.x;
+        xi.y = xi.y + alpha * pi.y;
+        ri.x = ri.x + (-alpha) * wi.x;
+        ri.y = ri.y + (-alpha) * wi.y;
+        x2[i] = xi;
+        r2[i] = ri;
+      }
+      const double2 d = dm2[i], b = bm2[i];
+      const double rr0 = ri.x * ri.x, rr1 = ri.y * ri.y;
+      s1 += d.x * rr0;
+      s2 += b.x * rr0;
+      s1 += d.y * rr1;
+      s2 += b.y * rr1;
+    }
+    done_upto = n2 * 2;
+  }
+  for (int64_t i = done_upto + tid; i < n; i += stride) {
     double ri = r[i];
     if (upd) {
       x[i] = x[i] + alpha * p[i];
@@ -304,11 +337,12 @@ int grid_for(int64_t n) {
 int reduce_host(const Mesh &m, const double *a, const double *b, const double *c, int64_t n,
                 int kind, double *out, hipStream_t s) {
   NK_CHECK(m.red != nullptr, NKH_ERR_STATE, "reduction buffer not allocated");
-  hipLaunchKernelGGL(k_reduce, dim3(kRedBlocks), dim3(256), 0, s, a, b, c, n, kind, m.red);
-  hipLaunchKernelGGL(k_reduce_final, dim3(1), dim3(256), 0, s, m.red, kRedBlocks, kind,
-                     m.red + 4 * kRedBlocks);
+  hipLaunchKernelGGL(k_reduce, dim3(kRedBlocks), dim3(256), 0, s, a, b, c, n, kind,
+                     m.red + kRedScratch);
+  hipLaunchKernelGGL(k_reduce_final, dim3(1), dim3(256), 0, s, m.red + kRedScratch, kRedBlocks,
+                     kind, m.red + kRedOut);
   NK_HIP(hipGetLastError());
-  NK_HIP(hipMemcpyAsync(out, m.red + 4 * kRedBlocks, sizeof(double), hipMemcpyDeviceToHost, s));
+  NK_HIP(hipMemcpyAsync(out, m.red + kRedOut, sizeof(double), hipMemcpyDeviceToHost, s));
   NK_HIP(hipStreamSynchronize(s));
   return NKH_OK;
 }
@@ -317,7 +351,7 @@ int reduce_host(const Mesh &m, const double *a, const double *b, const double *c
 
 int ensure_small(Mesh &m) {
   if (!m.st) NK_HIP(hipMalloc(&m.st, sizeof(CgState)));
-  if (!m.red) NK_HIP(hipMalloc(&m.red, sizeof(double) * (4 * kRedBlocks + 64 + 5 * kMaxCg)));
+  if (!m.red) NK_HIP(hipMalloc(&m.red, sizeof(double) * kRedSize));
   return NKH_OK;
 }
 
@@ -413,8 +447,8 @@ int run_cggo(Mesh &m, double *x, const double *f, Coef h1, Coef h2, const double
   const int64_t n = m.n;
   double *r = m.ws, *w = m.ws + n, *p = m.ws + 2 * n, *d = m.ws + 3 * n, *dm = m.ws + 4 * n,
          *bm = m.ws + 5 * n;
-  double *part1 = m.red, *part2 = m.red + kRedBlocks, *rho1 = m.red + 2 * kRedBlocks,
-         *rho2 = m.red + 3 * kRedBlocks;
+  double *part1 = m.red + kRedPart1, *part2 = m.red + kRedPart2, *rho1 = m.red + kRedRho1,
+         *rho2 = m.red + kRedRho2;
 
   double tol = std::fabs(tin);
   if (prm.restol != 0.0) tol = prm.restol;
@@ -447,15 +481,21 @@ int run_cggo(Mesh &m, double *x, const double *f, Coef h1, Coef h2, const double
   bool fused = m.gs.interior_zero;
   if (fused) {
     hipLaunchKernelGGL(k_interior_mask_dev, dim3(kRedBlocks), dim3(256), 0, s, mask, n, m.np,
-                       m.red);
-    hipLaunchKernelGGL(k_reduce_final, dim3(1), dim3(256), 0, s, m.red, kRedBlocks, 2,
-                       m.red + 4 * kRedBlocks);
+                       m.red + kRedScratch);
+    hipLaunchKernelGGL(k_reduce_final, dim3(1), dim3(256), 0, s, m.red + kRedScratch,
+                       kRedBlocks, 2, m.red + kRedOut);
     NK_HIP(hipGetLastError());
     double dev;
-    NK_HIP(hipMemcpyAsync(&dev, m.red + 4 * kRedBlocks, sizeof(double), hipMemcpyDeviceToHost, s));
+    NK_HIP(hipMemcpyAsync(&dev, m.red + kRedOut, sizeof(double), hipMemcpyDeviceToHost, s));
     NK_HIP(hipStreamSynchronize(s));
     fused = (dev == 0.0);
   }
+  // continuous mask and mult let the gs kernel read one p per group
+  bool compact = false;
+  if (fused) NK_TRY(gs_prepare(m.gs, mask, mult, m.red + kRedScratch, &compact, s));
+  const int ax_grid_parts = axhelm_cg_grid(m);
+  auto al16 = [](const void *q) { return (reinterpret_cast<uintptr_t>(q) & 15) == 0; };
+  const bool vec_ok = al16(x) && al16(r) && al16(p) && al16(w) && al16(dm) && al16(bm);
 
   CgState h{};
   h.rtz1 = 1.0;
@@ -470,7 +510,7 @@ int run_cggo(Mesh &m, double *x, const double *f, Coef h1, Coef h2, const double
 
   double *hist = nullptr;
   if (history_host) {
-    hist = m.red + 4 * kRedBlocks + 64;
+    hist = m.red + kRedHist;
     hipLaunchKernelGGL(k_fill, dim3(grid_for(5 * kMaxCg)), dim3(256), 0, s, hist, 0.0,
                        (int64_t)5 * kMaxCg);
   }
@@ -492,16 +532,20 @@ int run_cggo(Mesh &m, double *x, const double *f, Coef h1, Coef h2, const double
     NK_HIP(hipEventRecord(t_solve0, s));
   }
   while (true) {
-    hipLaunchKernelGGL(k_cg_update_dots, dim3(kRedBlocks), dim3(256), 0, s, x, r, p, w, dm,
-                       bm, n, m.st, part1, part2);
+    if (vec_ok)
+      hipLaunchKernelGGL(k_cg_update_dots<true>, dim3(kRedBlocks), dim3(256), 0, s, x, r, p, w,
+                         dm, bm, n, m.st, part1, part2);
+    else
+      hipLaunchKernelGGL(k_cg_update_dots<false>, dim3(kRedBlocks), dim3(256), 0, s, x, r, p,
+                         w, dm, bm, n, m.st, part1, part2);
     hipLaunchKernelGGL(k_cg_scalars1, dim3(1), dim3(256), 0, s, m.st, part1, part2, hist);
     if (fused) {
       if (timed) NK_HIP(hipEventRecord(ev[4 * launched + 0], s));
       NK_TRY(launch_axhelm_cg(m, w, p, r, d, h1, h2, mult, m.st, rho1, s));
       if (timed) NK_HIP(hipEventRecord(ev[4 * launched + 1], s));
-      NK_TRY(gs_sum_mask_dot(m.gs, w, mask, p, mult, m.st, rho2, s));
+      NK_TRY(gs_sum_mask_dot(m.gs, w, mask, p, mult, m.st, rho2, compact, s));
       if (timed) NK_HIP(hipEventRecord(ev[4 * launched + 2], s));
-      hipLaunchKernelGGL(k_cg_scalars2, dim3(1), dim3(256), 0, s, m.st, rho1, kRedBlocks,
+      hipLaunchKernelGGL(k_cg_scalars2, dim3(1), dim3(256), 0, s, m.st, rho1, ax_grid_parts,
                          rho2, kRedBlocks, hist);
     } else {
       NK_TRY(launch_axhelm_cg(m, w, p, r, d, h1, h2, mult, m.st, nullptr, s));

@@ -175,7 +175,7 @@ static int axhelm_dispatch(const Mesh &m, double *au, const double *u, Coef h1, 
   constexpr int EPB = epb<NP>();
   const int64_t ngroups = (m.nel + EPB - 1) / EPB;
   int grid;
-  if (MODE == 1) grid = kRedBlocks;
+  if (MODE == 1) grid = axhelm_cg_grid(m);
   else grid = (int)((ngroups < (int64_t)1 << 20) ? ngroups : ((int64_t)1 << 20));
   if (grid < 1) grid = 1;
   const dim3 block(NP * NP * EPB);

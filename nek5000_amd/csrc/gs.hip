@@ -107,6 +107,7 @@ int gs_build(GsMap &gs, const int64_t *glo, int64_t n, int np, int64_t nel) {
 }
 
 void gs_free(GsMap &gs) {
+  if (gs.coef) (void)hipFree(gs.coef);
   if (gs.off) (void)hipFree(gs.off);
   if (gs.idx) (void)hipFree(gs.idx);
   if (gs.single) (void)hipFree(gs.single);
@@ -185,10 +186,104 @@ __global__ void __launch_bounds__(256)
   if (threadIdx.x == 0) rho_part[blockIdx.x] = sbuf[0] + sbuf[1] + sbuf[2] + sbuf[3];
 }
 
+// Groups with coefficients: every member holds the same mask, mult and p
+// (all three are continuous), so w <- mask*sum(w) and the rho contribution
+// sum_m w*p_m*mult_m = w * p_0 * sum_m mult_m need only the first member's p.
+__global__ void __launch_bounds__(256)
+    k_gs_sum_mask_dot_c(double *__restrict__ w, const double *__restrict__ p,
+                        const int32_t *__restrict__ off, const int32_t *__restrict__ idx,
+                        int64_t nseg, const int32_t *__restrict__ single, int64_t nsingle,
+                        const double2 *__restrict__ coef, const CgState *__restrict__ st,
+                        double *__restrict__ rho_part) {
+  __shared__ double sbuf[4];
+  if (st->done) return;
+  double rho = 0.0;
+  const int64_t total = nseg + nsingle;
+  for (int64_t s = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; s < total;
+       s += (int64_t)gridDim.x * blockDim.x) {
+    const double2 c = coef[s];
+    if (s < nseg) {
+      const int32_t a = off[s], b = off[s + 1];
+      const int32_t l0 = idx[a];
+      double acc = w[l0];
+      for (int32_t q = a + 1; q < b; ++q) acc += w[idx[q]];
+      const double v = acc * c.x;
+      w[l0] = v;
+      for (int32_t q = a + 1; q < b; ++q) w[idx[q]] = v;
+      rho += v * p[l0] * c.y;
+    } else {
+      const int32_t l = single[s - nseg];
+      const double v = w[l] * c.x;
+      w[l] = v;
+      rho += v * p[l] * c.y;
+    }
+  }
+  for (int o = 32; o > 0; o >>= 1) rho += __shfl_down(rho, o, 64);
+  if ((threadIdx.x & 63) == 0) sbuf[threadIdx.x >> 6] = rho;
+  __syncthreads();
+  if (threadIdx.x == 0) rho_part[blockIdx.x] = ((sbuf[0] + sbuf[1]) + sbuf[2]) + sbuf[3];
+}
+
+__global__ void __launch_bounds__(256)
+    k_gs_prepare(const double *__restrict__ mask, const double *__restrict__ mult,
+                 const int32_t *__restrict__ off, const int32_t *__restrict__ idx, int64_t nseg,
+                 const int32_t *__restrict__ single, int64_t nsingle, double2 *__restrict__ coef,
+                 double *__restrict__ part) {
+  __shared__ double sbuf[4];
+  double dev = 0.0;
+  const int64_t total = nseg + nsingle;
+  for (int64_t s = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; s < total;
+       s += (int64_t)gridDim.x * blockDim.x) {
+    if (s < nseg) {
+      const int32_t a = off[s], b = off[s + 1];
+      const double m0 = mask[idx[a]], u0 = mult[idx[a]];
+      double sm = u0;
+      for (int32_t q = a + 1; q < b; ++q) {
+        const int32_t l = idx[q];
+        dev = fmax(dev, fabs(mask[l] - m0) + fabs(mult[l] - u0));
+        sm += mult[l];
+      }
+      coef[s] = make_double2(m0, sm);
+    } else {
+      const int32_t l = single[s - nseg];
+      coef[s] = make_double2(mask[l], mult[l]);
+    }
+  }
+  for (int o = 32; o > 0; o >>= 1) dev = fmax(dev, __shfl_down(dev, o, 64));
+  if ((threadIdx.x & 63) == 0) sbuf[threadIdx.x >> 6] = dev;
+  __syncthreads();
+  if (threadIdx.x == 0)
+    part[blockIdx.x] = fmax(fmax(sbuf[0], sbuf[1]), fmax(sbuf[2], sbuf[3]));
+}
+
+int gs_prepare(GsMap &gs, const double *mask, const double *mult, double *red,
+               bool *continuous, hipStream_t s) {
+  const int64_t total = gs.nseg + gs.nsingle;
+  *continuous = false;
+  if (total == 0) return NKH_OK;
+  if (!gs.coef) NK_HIP(hipMalloc(&gs.coef, sizeof(double2) * total));
+  hipLaunchKernelGGL(k_gs_prepare, dim3(kRedBlocks), dim3(256), 0, s, mask, mult, gs.off,
+                     gs.idx, gs.nseg, gs.single, gs.nsingle, gs.coef, red);
+  NK_HIP(hipGetLastError());
+  std::vector<double> h(kRedBlocks);
+  NK_HIP(hipMemcpyAsync(h.data(), red, sizeof(double) * kRedBlocks, hipMemcpyDeviceToHost, s));
+  NK_HIP(hipStreamSynchronize(s));
+  double dev = 0.0;
+  for (double v : h) dev = std::max(dev, v);
+  *continuous = (dev == 0.0);
+  return NKH_OK;
+}
+
 int gs_sum_mask_dot(const GsMap &gs, double *w, const double *mask, const double *p,
-                    const double *mult, const CgState *st, double *rho_part, hipStream_t s) {
-  hipLaunchKernelGGL(k_gs_sum_mask_dot, dim3(kRedBlocks), dim3(256), 0, s, w, mask, p, mult,
-                     gs.off, gs.idx, gs.nseg, gs.single, gs.nsingle, st, rho_part);
+                    const double *mult, const CgState *st, double *rho_part, bool compact,
+                    hipStream_t s) {
+  if (compact) {
+    hipLaunchKernelGGL(k_gs_sum_mask_dot_c, dim3(kRedBlocks), dim3(256), 0, s, w, p, gs.off,
+                       gs.idx, gs.nseg, gs.single, gs.nsingle, gs.coef, st, rho_part);
+  } else {
+    hipLaunchKernelGGL(k_gs_sum_mask_dot, dim3(kRedBlocks), dim3(256), 0, s, w, mask, p, mult,
+                       gs.off, gs.idx, gs.nseg, gs.single, gs.nsingle, st, rho_part);
+  }
   NK_HIP(hipGetLastError());
   return NKH_OK;
 }

@@ -14,6 +14,18 @@ constexpr int kMaxNP = 16;
 // Fixed grid of the reduction kernels: partial sums have this many slots, so
 // every reduction sums the same partials in the same order (deterministic).
 constexpr int kRedBlocks = 2048;
+// Partial-sum slots of the CG-fused axhelm (one per block of its fixed grid).
+constexpr int kAxBlocks = 16384;
+constexpr int kMaxCgIter = 900;  // hmholtz.f:636 maxcg
+// Layout of Mesh::red (doubles).
+constexpr int kRedPart1 = 0;                         // (z,r) partials
+constexpr int kRedPart2 = kRedBlocks;                // (r,r)_binv partials
+constexpr int kRedRho2 = 2 * kRedBlocks;             // rho partials of the gs kernel
+constexpr int kRedScratch = 3 * kRedBlocks;          // setup reductions
+constexpr int kRedOut = 4 * kRedBlocks;              // 64 result slots
+constexpr int kRedRho1 = 4 * kRedBlocks + 64;        // rho partials of axhelm
+constexpr int kRedHist = kRedRho1 + kAxBlocks;       // 5*kMaxCgIter history
+constexpr int kRedSize = kRedHist + 5 * kMaxCgIter;
 
 void set_error(const char *fmt, ...);
 
@@ -55,6 +67,9 @@ struct GsMap {
   int32_t *single = nullptr;  // device, nsingle
   bool interior_zero = false; // glo_num == 0 exactly at element-interior nodes
   int max_mult = 0;
+  // per group (segments, then singles): {mask, sum of mult over members},
+  // filled once per solve by gs_prepare when mask and mult are continuous
+  double2 *coef = nullptr;
 };
 
 // State of one PCG solve, resident on the device.  The scalar kernels update
@@ -104,6 +119,14 @@ struct Timers {
 };
 Timers &timers();
 
+// Elements per block of the element kernels (blocks of at least 64 lanes)
+// and the fixed grid of the CG-fused axhelm (its number of rho partials).
+inline int elems_per_block(int np) { return np * np >= 64 ? 1 : 64 / (np * np); }
+inline int axhelm_cg_grid(const Mesh &m) {
+  const int64_t g = (m.nel + elems_per_block(m.np) - 1) / elems_per_block(m.np);
+  return (int)(g < kAxBlocks ? g : kAxBlocks);
+}
+
 Mesh *get_mesh(int id);
 hipStream_t cur_stream();
 int ensure_workspace(Mesh &m);  // solve vectors + state + partial sums
@@ -132,10 +155,16 @@ int gs_build(GsMap &gs, const int64_t *glo_num_host, int64_t n, int np, int64_t 
 void gs_free(GsMap &gs);
 int gs_op(const GsMap &gs, double *u, int op, hipStream_t s);
 // gs '+' on w fused with w*=mask on all nonzero-id nodes and the rho partial
-// sum over them (w*p*mult); skipped when st->done.
+// sum over them (w*p*mult); skipped when st->done.  With `compact` the
+// per-group coefficients from gs_prepare replace the per-member mask/mult/p
+// reads (valid when those fields are continuous).
 int gs_sum_mask_dot(const GsMap &gs, double *w, const double *mask, const double *p,
-                    const double *mult, const CgState *st, double *rho_part,
+                    const double *mult, const CgState *st, double *rho_part, bool compact,
                     hipStream_t s);
+// Fill gs.coef from mask and mult; *continuous tells whether every member of
+// every group carries the same mask and mult value (red: kRedBlocks scratch).
+int gs_prepare(GsMap &gs, const double *mask, const double *mult, double *red,
+               bool *continuous, hipStream_t s);
 
 // ---- cg.hip ----
 int run_cggo(Mesh &m, double *x, const double *f, Coef h1, Coef h2, const double *mask,
