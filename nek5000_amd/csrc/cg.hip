@@ -8,9 +8,14 @@
 // runs as five launches:
 //     k_cg_update_dots : x/r update of the previous iteration + both dots
 //     k_cg_scalars1    : one block: sums the dot partials, exit test, beta
-//     axhelm (MODE 1)  : p update fused in, w = A p, rho over interior nodes
-//     gs_sum_mask_dot  : dssum + mask + rho over element-boundary nodes
+//     axhelm (MODE 1)  : p update fused in, w = A_L p, rho = sum p.w
+//     gs_assemble      : w <- mask * dssum(w)
 //     k_cg_scalars2    : one block: rho, alpha
+// rho is taken before the assembly: with p continuous and zero where the
+// mask is, and mult summing to one over each group of shared nodes,
+// (mask*QQ^T w, p)_mult = sum_l p_l w_l exactly in exact arithmetic.  The
+// solve checks those conditions once and otherwise runs the unfused
+// sequence (axhelm, dssum, mask, dot).
 // Scalars stay on the device (CgState); the host enqueues iterations ahead
 // and reads the state back every few iterations.  Kernels of iterations past
 // the exit return immediately.
@@ -230,8 +235,8 @@ __global__ void __launch_bounds__(256)
     k_cg_scalars1(CgState *__restrict__ st, const double *__restrict__ part1,
                   const double *__restrict__ part2, double *__restrict__ hist) {
   if (st->done) return;
-  const double s1 = block_sum_array(part1, kRedBlocks);
-  const double s2 = block_sum_array(part2, kRedBlocks);
+  const double s1 = block_sum_array(part1, kUpdBlocks);
+  const double s2 = block_sum_array(part2, kUpdBlocks);
   if (threadIdx.x != 0) return;
   st->pending = 0;
   const int iter = st->iter + 1;
@@ -306,10 +311,12 @@ __global__ void k_mask_guarded(double *__restrict__ w, const double *__restrict_
     w[i] = w[i] * mask[i];
 }
 
-// max |mask - 1| over element-interior nodes, one partial per block
+// max(|mask-1|, |mult-1|) over element-interior nodes and max |f| over
+// masked nodes, one partial per block
 __global__ void __launch_bounds__(256)
-    k_interior_mask_dev(const double *__restrict__ mask, int64_t n, int np,
-                        double *__restrict__ part) {
+    k_fused_checks(const double *__restrict__ mask, const double *__restrict__ mult,
+                   const double *__restrict__ f, int64_t n, int np,
+                   double *__restrict__ part) {
   __shared__ double sb[4];
   const int64_t np3 = (int64_t)np * np * np;
   double v = 0.0;
@@ -318,7 +325,8 @@ __global__ void __launch_bounds__(256)
     const int q = (int)(l % np3);
     const int i = q % np, j = (q / np) % np, k = q / (np * np);
     if (i > 0 && i < np - 1 && j > 0 && j < np - 1 && k > 0 && k < np - 1)
-      v = fmax(v, fabs(mask[l] - 1.0));
+      v = fmax(v, fmax(fabs(mask[l] - 1.0), fabs(mult[l] - 1.0)));
+    if (mask[l] == 0.0) v = fmax(v, fabs(f[l]));
   }
   for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_down(v, o, 64));
   if ((threadIdx.x & 63) == 0) sb[threadIdx.x >> 6] = v;
@@ -475,13 +483,14 @@ int run_cggo(Mesh &m, double *x, const double *f, Coef h1, Coef h2, const double
   NK_CHECK(!(skmin > 0 && h2max == 0), NKH_ERR_UNSUPPORTED,
            "cggo: pure-Neumann mean correction (ifmcor) is not implemented");
 
-  // The fused kernels finish element-interior nodes inside axhelm (no dssum
-  // there, mult taken from the array, mask assumed 1).  Use them only when the
-  // ids are zero exactly at interiors and the mask is 1 at every interior node.
+  // Fused iteration (rho inside axhelm, mask inside the assembly) when its
+  // conditions hold: ids zero exactly at element interiors, mask 1 and mult 1
+  // there, f zero where masked, and per group a 0/1 mask and mult summing
+  // to 1.  Checked once per solve.
   bool fused = m.gs.interior_zero;
   if (fused) {
-    hipLaunchKernelGGL(k_interior_mask_dev, dim3(kRedBlocks), dim3(256), 0, s, mask, n, m.np,
-                       m.red + kRedScratch);
+    hipLaunchKernelGGL(k_fused_checks, dim3(kRedBlocks), dim3(256), 0, s, mask, mult, f, n,
+                       m.np, m.red + kRedScratch);
     hipLaunchKernelGGL(k_reduce_final, dim3(1), dim3(256), 0, s, m.red + kRedScratch,
                        kRedBlocks, 2, m.red + kRedOut);
     NK_HIP(hipGetLastError());
@@ -490,9 +499,11 @@ int run_cggo(Mesh &m, double *x, const double *f, Coef h1, Coef h2, const double
     NK_HIP(hipStreamSynchronize(s));
     fused = (dev == 0.0);
   }
-  // continuous mask and mult let the gs kernel read one p per group
-  bool compact = false;
-  if (fused) NK_TRY(gs_prepare(m.gs, mask, mult, m.red + kRedScratch, &compact, s));
+  if (fused) {
+    bool ok = false;
+    NK_TRY(gs_prepare(m.gs, mask, mult, m.red + kRedScratch, &ok, s));
+    fused = ok;
+  }
   const int ax_grid_parts = axhelm_cg_grid(m);
   auto al16 = [](const void *q) { return (reinterpret_cast<uintptr_t>(q) & 15) == 0; };
   const bool vec_ok = al16(x) && al16(r) && al16(p) && al16(w) && al16(dm) && al16(bm);
@@ -533,20 +544,20 @@ int run_cggo(Mesh &m, double *x, const double *f, Coef h1, Coef h2, const double
   }
   while (true) {
     if (vec_ok)
-      hipLaunchKernelGGL(k_cg_update_dots<true>, dim3(kRedBlocks), dim3(256), 0, s, x, r, p, w,
+      hipLaunchKernelGGL(k_cg_update_dots<true>, dim3(kUpdBlocks), dim3(256), 0, s, x, r, p, w,
                          dm, bm, n, m.st, part1, part2);
     else
-      hipLaunchKernelGGL(k_cg_update_dots<false>, dim3(kRedBlocks), dim3(256), 0, s, x, r, p,
+      hipLaunchKernelGGL(k_cg_update_dots<false>, dim3(kUpdBlocks), dim3(256), 0, s, x, r, p,
                          w, dm, bm, n, m.st, part1, part2);
     hipLaunchKernelGGL(k_cg_scalars1, dim3(1), dim3(256), 0, s, m.st, part1, part2, hist);
     if (fused) {
       if (timed) NK_HIP(hipEventRecord(ev[4 * launched + 0], s));
       NK_TRY(launch_axhelm_cg(m, w, p, r, d, h1, h2, mult, m.st, rho1, s));
       if (timed) NK_HIP(hipEventRecord(ev[4 * launched + 1], s));
-      NK_TRY(gs_sum_mask_dot(m.gs, w, mask, p, mult, m.st, rho2, compact, s));
+      NK_TRY(gs_assemble(m.gs, w, m.st, s));
       if (timed) NK_HIP(hipEventRecord(ev[4 * launched + 2], s));
       hipLaunchKernelGGL(k_cg_scalars2, dim3(1), dim3(256), 0, s, m.st, rho1, ax_grid_parts,
-                         rho2, kRedBlocks, hist);
+                         rho2, 0, hist);
     } else {
       NK_TRY(launch_axhelm_cg(m, w, p, r, d, h1, h2, mult, m.st, nullptr, s));
       NK_TRY(gs_op(m.gs, w, 0, s));

@@ -46,8 +46,9 @@ __device__ __forceinline__ double block_sum(double v, double *sbuf) {
 //   tmp3 = h1*(g3*ut + g5*ur + g6*us)
 //   au   = D^T_r tmp1 + D^T_s tmp2 + D^T_t tmp3  (+ (h2*bm1)*u)
 // MODE 0: au = A u.  MODE 1 (CG): p = d*r + beta*p is formed on the fly and
-// stored, w = A p, and sum(w*p*mult) over element-interior nodes is written
-// to one slot per block.
+// stored, w = A_L p (unassembled), and sum_l p_l w_l -- the CG denominator
+// rho = (Q Q^T w, p)_mult for a continuous, masked p -- is written to one
+// slot per block when rho_part is given.
 // ---------------------------------------------------------------------------
 template <int NP, int MODE, bool H1F, int H2M>
 __global__ void __launch_bounds__(NP * NP * epb<NP>())
@@ -149,11 +150,7 @@ __global__ void __launch_bounds__(NP * NP * epb<NP>())
         if constexpr (H2M == 1) a = a + (h2s * bm1[l]) * r_u[k];
         if constexpr (H2M == 2) a = a + (h2f[l] * bm1[l]) * r_u[k];
         au[l] = a;
-        if constexpr (MODE == 1) {
-          if (rho_part != nullptr && i > 0 && i < NP - 1 && j > 0 && j < NP - 1 && k > 0 &&
-              k < NP - 1)
-            rho += a * r_u[k] * mult[l];
-        }
+        if constexpr (MODE == 1) rho += a * r_u[k];  // p . (A_L p), element-local
       }
     }
   }

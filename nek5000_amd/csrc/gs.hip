@@ -24,12 +24,14 @@ int gs_build(GsMap &gs, const int64_t *glo, int64_t n, int np, int64_t nel) {
   NK_CHECK(n < ((int64_t)1 << 31), NKH_ERR_ARG, "gs: %lld local nodes exceed int32 indexing",
            (long long)n);
   gs_free(gs);
-  // Group nonzero ids.  Ids from setvert3d are dense in [1, ngv], so a
-  // counting sort (stable in the local index) does it in O(n); arbitrary ids
-  // fall back to a comparison sort of (id, index) pairs.  Both give groups in
-  // increasing id order with members in increasing local index.
-  std::vector<int32_t> off, idx, single;
-  int maxm = 1;
+  // Group the nonzero ids (members in increasing local index) and list the
+  // groups in the order of their first member, so that the groups one block
+  // of threads visits lie in the same few elements.  Ids from setvert3d are
+  // dense in [1, ngv]: a counting sort does it in O(n); arbitrary ids fall
+  // back to a comparison sort.
+  std::vector<int32_t> off, idx;
+  int maxm = 0;
+  int64_t nsingle = 0;
   int64_t gmax = 0, gmin = INT64_MAX, nnz = 0;
   for (int64_t i = 0; i < n; ++i)
     if (glo[i] != 0) {
@@ -37,14 +39,13 @@ int gs_build(GsMap &gs, const int64_t *glo, int64_t n, int np, int64_t nel) {
       gmin = std::min(gmin, glo[i]);
       ++nnz;
     }
+  off.reserve((size_t)nnz / 2 + 2);
+  idx.reserve((size_t)nnz);
   auto emit = [&](const int32_t *mem, int64_t cnt) {
-    if (cnt >= 2) {
-      off.push_back((int32_t)idx.size());
-      idx.insert(idx.end(), mem, mem + cnt);
-      maxm = std::max(maxm, (int)cnt);
-    } else if (cnt == 1) {
-      single.push_back(mem[0]);
-    }
+    off.push_back((int32_t)idx.size());
+    idx.insert(idx.end(), mem, mem + cnt);
+    maxm = std::max(maxm, (int)cnt);
+    if (cnt == 1) ++nsingle;
   };
   if (nnz > 0 && gmin > 0 && gmax <= 4 * n + 16) {
     std::vector<int32_t> cnt((size_t)gmax + 2, 0);
@@ -52,25 +53,36 @@ int gs_build(GsMap &gs, const int64_t *glo, int64_t n, int np, int64_t nel) {
       if (glo[i] != 0) ++cnt[(size_t)glo[i] + 1];
     for (size_t k = 1; k < cnt.size(); ++k) cnt[k] += cnt[k - 1];
     std::vector<int32_t> perm((size_t)nnz);
-    std::vector<int32_t> pos(cnt.begin(), cnt.end() - 1);
-    for (int64_t i = 0; i < n; ++i)
-      if (glo[i] != 0) perm[(size_t)pos[(size_t)glo[i]]++] = (int32_t)i;
-    idx.reserve((size_t)nnz);
-    for (int64_t g = 1; g <= gmax; ++g)
-      emit(perm.data() + cnt[(size_t)g], cnt[(size_t)g + 1] - cnt[(size_t)g]);
+    {
+      std::vector<int32_t> pos(cnt.begin(), cnt.end() - 1);
+      for (int64_t i = 0; i < n; ++i)
+        if (glo[i] != 0) perm[(size_t)pos[(size_t)glo[i]]++] = (int32_t)i;
+    }
+    for (int64_t l = 0; l < n; ++l) {
+      const int64_t g = glo[l];
+      if (g != 0 && perm[(size_t)cnt[(size_t)g]] == (int32_t)l)
+        emit(perm.data() + cnt[(size_t)g], cnt[(size_t)g + 1] - cnt[(size_t)g]);
+    }
   } else if (nnz > 0) {
     std::vector<std::pair<int64_t, int32_t>> p;
     p.reserve((size_t)nnz);
     for (int64_t i = 0; i < n; ++i)
       if (glo[i] != 0) p.emplace_back(glo[i], (int32_t)i);
     std::sort(p.begin(), p.end());
-    std::vector<int32_t> mem;
+    std::vector<std::pair<int32_t, size_t>> starts;  // (first member, position in p)
     for (size_t a = 0; a < p.size();) {
       size_t b = a;
-      mem.clear();
-      while (b < p.size() && p[b].first == p[a].first) mem.push_back(p[b++].second);
-      emit(mem.data(), (int64_t)mem.size());
+      while (b < p.size() && p[b].first == p[a].first) ++b;
+      starts.emplace_back(p[a].second, a);
       a = b;
+    }
+    std::sort(starts.begin(), starts.end());
+    std::vector<int32_t> mem;
+    for (auto &st : starts) {
+      mem.clear();
+      for (size_t b = st.second; b < p.size() && p[b].first == p[st.second].first; ++b)
+        mem.push_back(p[b].second);
+      emit(mem.data(), (int64_t)mem.size());
     }
   }
   off.push_back((int32_t)idx.size());
@@ -87,9 +99,10 @@ int gs_build(GsMap &gs, const int64_t *glo, int64_t n, int np, int64_t nel) {
     }
   }
   gs.n = n;
-  gs.nseg = (int64_t)off.size() - 1;
+  gs.ngroups = (int64_t)off.size() - 1;
   gs.nmem = (int64_t)idx.size();
-  gs.nsingle = (int64_t)single.size();
+  gs.nseg = gs.ngroups - nsingle;
+  gs.nsingle = nsingle;
   gs.interior_zero = iz;
   gs.max_mult = maxm;
   NK_HIP(hipMalloc(&gs.off, sizeof(int32_t) * off.size()));
@@ -98,19 +111,13 @@ int gs_build(GsMap &gs, const int64_t *glo, int64_t n, int np, int64_t nel) {
     NK_HIP(hipMalloc(&gs.idx, sizeof(int32_t) * idx.size()));
     NK_HIP(hipMemcpy(gs.idx, idx.data(), sizeof(int32_t) * idx.size(), hipMemcpyHostToDevice));
   }
-  if (!single.empty()) {
-    NK_HIP(hipMalloc(&gs.single, sizeof(int32_t) * single.size()));
-    NK_HIP(hipMemcpy(gs.single, single.data(), sizeof(int32_t) * single.size(),
-                     hipMemcpyHostToDevice));
-  }
   return NKH_OK;
 }
 
 void gs_free(GsMap &gs) {
-  if (gs.coef) (void)hipFree(gs.coef);
+  if (gs.flag) (void)hipFree(gs.flag);
   if (gs.off) (void)hipFree(gs.off);
   if (gs.idx) (void)hipFree(gs.idx);
-  if (gs.single) (void)hipFree(gs.single);
   gs = GsMap();
 }
 
@@ -128,6 +135,7 @@ __global__ void k_gs_op(double *__restrict__ u, const int32_t *__restrict__ off,
   for (int64_t s = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; s < nseg;
        s += (int64_t)gridDim.x * blockDim.x) {
     const int32_t a = off[s], b = off[s + 1];
+    if (b - a < 2) continue;
     double acc = u[idx[a]];
     for (int32_t c = a + 1; c < b; ++c) acc = gs_combine<OP>(acc, u[idx[c]]);
     for (int32_t c = a; c < b; ++c) u[idx[c]] = acc;
@@ -136,154 +144,103 @@ __global__ void k_gs_op(double *__restrict__ u, const int32_t *__restrict__ off,
 
 int gs_op(const GsMap &gs, double *u, int op, hipStream_t s) {
   if (gs.nseg == 0) return NKH_OK;
-  int grid = (int)((gs.nseg + 255) / 256);
+  const int64_t ng = gs.ngroups;
+  int grid = (int)((ng + 255) / 256);
   if (grid > 65536) grid = 65536;
   switch (op) {
-    case 0: hipLaunchKernelGGL(k_gs_op<0>, dim3(grid), dim3(256), 0, s, u, gs.off, gs.idx, gs.nseg); break;
-    case 1: hipLaunchKernelGGL(k_gs_op<1>, dim3(grid), dim3(256), 0, s, u, gs.off, gs.idx, gs.nseg); break;
-    case 2: hipLaunchKernelGGL(k_gs_op<2>, dim3(grid), dim3(256), 0, s, u, gs.off, gs.idx, gs.nseg); break;
-    case 3: hipLaunchKernelGGL(k_gs_op<3>, dim3(grid), dim3(256), 0, s, u, gs.off, gs.idx, gs.nseg); break;
+    case 0: hipLaunchKernelGGL(k_gs_op<0>, dim3(grid), dim3(256), 0, s, u, gs.off, gs.idx, ng); break;
+    case 1: hipLaunchKernelGGL(k_gs_op<1>, dim3(grid), dim3(256), 0, s, u, gs.off, gs.idx, ng); break;
+    case 2: hipLaunchKernelGGL(k_gs_op<2>, dim3(grid), dim3(256), 0, s, u, gs.off, gs.idx, ng); break;
+    case 3: hipLaunchKernelGGL(k_gs_op<3>, dim3(grid), dim3(256), 0, s, u, gs.off, gs.idx, ng); break;
     default: set_error("gs_op: unknown op %d", op); return NKH_ERR_ARG;
   }
   NK_HIP(hipGetLastError());
   return NKH_OK;
 }
 
-// dssum + mask + rho partial (hmholtz.f:797-801): for every nonzero-id node
-// w <- mask * sum_{same id} w, and sum(w*p*mult) over those nodes.
+// CG assembly of w (hmholtz.f:797-798, dssum then mask), with the group
+// flags of gs_prepare: 0 sum the members, 1 masked group (write zeros),
+// 2 nothing to do (one member, mask 1).  Each block takes a contiguous chunk
+// of groups; chunks are handed out so that the blocks resident on one XCD
+// (workgroups are dealt to the 8 XCDs round-robin) walk neighbouring chunks
+// and share their L2 lines.
 __global__ void __launch_bounds__(256)
-    k_gs_sum_mask_dot(double *__restrict__ w, const double *__restrict__ mask,
-                      const double *__restrict__ p, const double *__restrict__ mult,
-                      const int32_t *__restrict__ off, const int32_t *__restrict__ idx,
-                      int64_t nseg, const int32_t *__restrict__ single, int64_t nsingle,
-                      const CgState *__restrict__ st, double *__restrict__ rho_part) {
-  __shared__ double sbuf[4];
+    k_gs_assemble(double *__restrict__ w, const int32_t *__restrict__ off,
+                  const int32_t *__restrict__ idx, const uint8_t *__restrict__ flag,
+                  int64_t ngroups, int64_t chunk, const CgState *__restrict__ st) {
   if (st->done) return;
-  double rho = 0.0;
-  const int64_t total = nseg + nsingle;
-  for (int64_t s = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; s < total;
-       s += (int64_t)gridDim.x * blockDim.x) {
-    if (s < nseg) {
-      const int32_t a = off[s], b = off[s + 1];
+  const int nb = gridDim.x, b = blockIdx.x;
+  const int64_t lb = (int64_t)(b % 8) * (nb / 8) + b / 8;
+  const int64_t s0 = lb * chunk;
+  const int64_t s1 = (s0 + chunk < ngroups) ? s0 + chunk : ngroups;
+  for (int64_t s = s0 + threadIdx.x; s < s1; s += blockDim.x) {
+    const uint8_t f = flag[s];
+    if (f == 2) continue;
+    const int32_t a = off[s], e = off[s + 1];
+    if (f == 1) {
+      for (int32_t q = a; q < e; ++q) w[idx[q]] = 0.0;
+    } else {
       double acc = w[idx[a]];
-      for (int32_t c = a + 1; c < b; ++c) acc += w[idx[c]];
-      for (int32_t c = a; c < b; ++c) {
-        const int32_t l = idx[c];
-        const double v = acc * mask[l];
-        w[l] = v;
-        rho += v * p[l] * mult[l];
-      }
-    } else {
-      const int32_t l = single[s - nseg];
-      const double v = w[l] * mask[l];
-      w[l] = v;
-      rho += v * p[l] * mult[l];
+      for (int32_t q = a + 1; q < e; ++q) acc += w[idx[q]];
+      for (int32_t q = a; q < e; ++q) w[idx[q]] = acc;
     }
   }
-  for (int o = 32; o > 0; o >>= 1) rho += __shfl_down(rho, o, 64);
-  if ((threadIdx.x & 63) == 0) sbuf[threadIdx.x >> 6] = rho;
-  __syncthreads();
-  if (threadIdx.x == 0) rho_part[blockIdx.x] = sbuf[0] + sbuf[1] + sbuf[2] + sbuf[3];
 }
 
-// Groups with coefficients: every member holds the same mask, mult and p
-// (all three are continuous), so w <- mask*sum(w) and the rho contribution
-// sum_m w*p_m*mult_m = w * p_0 * sum_m mult_m need only the first member's p.
-__global__ void __launch_bounds__(256)
-    k_gs_sum_mask_dot_c(double *__restrict__ w, const double *__restrict__ p,
-                        const int32_t *__restrict__ off, const int32_t *__restrict__ idx,
-                        int64_t nseg, const int32_t *__restrict__ single, int64_t nsingle,
-                        const double2 *__restrict__ coef, const CgState *__restrict__ st,
-                        double *__restrict__ rho_part) {
-  __shared__ double sbuf[4];
-  if (st->done) return;
-  double rho = 0.0;
-  const int64_t total = nseg + nsingle;
-  for (int64_t s = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; s < total;
-       s += (int64_t)gridDim.x * blockDim.x) {
-    const double2 c = coef[s];
-    if (s < nseg) {
-      const int32_t a = off[s], b = off[s + 1];
-      const int32_t l0 = idx[a];
-      double acc = w[l0];
-      for (int32_t q = a + 1; q < b; ++q) acc += w[idx[q]];
-      const double v = acc * c.x;
-      w[l0] = v;
-      for (int32_t q = a + 1; q < b; ++q) w[idx[q]] = v;
-      rho += v * p[l0] * c.y;
-    } else {
-      const int32_t l = single[s - nseg];
-      const double v = w[l] * c.x;
-      w[l] = v;
-      rho += v * p[l] * c.y;
-    }
-  }
-  for (int o = 32; o > 0; o >>= 1) rho += __shfl_down(rho, o, 64);
-  if ((threadIdx.x & 63) == 0) sbuf[threadIdx.x >> 6] = rho;
-  __syncthreads();
-  if (threadIdx.x == 0) rho_part[blockIdx.x] = ((sbuf[0] + sbuf[1]) + sbuf[2]) + sbuf[3];
-}
-
+// Group flags from mask and mult, plus the check that makes the fused CG
+// exact: in every group mask is 0 or 1 and equal at all members, and the
+// members' mult sum to 1 (then sum_l mult*w*p over the assembled, masked w
+// equals sum_l w_local*p for a masked, continuous p).  part[b] = 0 when the
+// block's groups pass.
 __global__ void __launch_bounds__(256)
     k_gs_prepare(const double *__restrict__ mask, const double *__restrict__ mult,
-                 const int32_t *__restrict__ off, const int32_t *__restrict__ idx, int64_t nseg,
-                 const int32_t *__restrict__ single, int64_t nsingle, double2 *__restrict__ coef,
-                 double *__restrict__ part) {
+                 const int32_t *__restrict__ off, const int32_t *__restrict__ idx,
+                 int64_t ngroups, uint8_t *__restrict__ flag, double *__restrict__ part) {
   __shared__ double sbuf[4];
-  double dev = 0.0;
-  const int64_t total = nseg + nsingle;
-  for (int64_t s = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; s < total;
+  double bad = 0.0;
+  for (int64_t s = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; s < ngroups;
        s += (int64_t)gridDim.x * blockDim.x) {
-    if (s < nseg) {
-      const int32_t a = off[s], b = off[s + 1];
-      const double m0 = mask[idx[a]], u0 = mult[idx[a]];
-      double sm = u0;
-      for (int32_t q = a + 1; q < b; ++q) {
-        const int32_t l = idx[q];
-        dev = fmax(dev, fabs(mask[l] - m0) + fabs(mult[l] - u0));
-        sm += mult[l];
-      }
-      coef[s] = make_double2(m0, sm);
-    } else {
-      const int32_t l = single[s - nseg];
-      coef[s] = make_double2(mask[l], mult[l]);
+    const int32_t a = off[s], e = off[s + 1];
+    const double m0 = mask[idx[a]];
+    double sm = 0.0;
+    for (int32_t q = a; q < e; ++q) {
+      const int32_t l = idx[q];
+      if (mask[l] != m0) bad = 1.0;
+      sm += mult[l];
     }
+    if (!(m0 == 0.0 || m0 == 1.0) || sm != 1.0) bad = 1.0;
+    flag[s] = (m0 == 0.0) ? 1 : ((e - a >= 2) ? 0 : 2);
   }
-  for (int o = 32; o > 0; o >>= 1) dev = fmax(dev, __shfl_down(dev, o, 64));
-  if ((threadIdx.x & 63) == 0) sbuf[threadIdx.x >> 6] = dev;
+  for (int o = 32; o > 0; o >>= 1) bad = fmax(bad, __shfl_down(bad, o, 64));
+  if ((threadIdx.x & 63) == 0) sbuf[threadIdx.x >> 6] = bad;
   __syncthreads();
   if (threadIdx.x == 0)
     part[blockIdx.x] = fmax(fmax(sbuf[0], sbuf[1]), fmax(sbuf[2], sbuf[3]));
 }
 
-int gs_prepare(GsMap &gs, const double *mask, const double *mult, double *red,
-               bool *continuous, hipStream_t s) {
-  const int64_t total = gs.nseg + gs.nsingle;
-  *continuous = false;
-  if (total == 0) return NKH_OK;
-  if (!gs.coef) NK_HIP(hipMalloc(&gs.coef, sizeof(double2) * total));
+int gs_prepare(GsMap &gs, const double *mask, const double *mult, double *red, bool *ok,
+               hipStream_t s) {
+  *ok = false;
+  if (gs.ngroups == 0) return NKH_OK;
+  if (!gs.flag) NK_HIP(hipMalloc(&gs.flag, (size_t)gs.ngroups));
   hipLaunchKernelGGL(k_gs_prepare, dim3(kRedBlocks), dim3(256), 0, s, mask, mult, gs.off,
-                     gs.idx, gs.nseg, gs.single, gs.nsingle, gs.coef, red);
+                     gs.idx, gs.ngroups, gs.flag, red);
   NK_HIP(hipGetLastError());
   std::vector<double> h(kRedBlocks);
   NK_HIP(hipMemcpyAsync(h.data(), red, sizeof(double) * kRedBlocks, hipMemcpyDeviceToHost, s));
   NK_HIP(hipStreamSynchronize(s));
-  double dev = 0.0;
-  for (double v : h) dev = std::max(dev, v);
-  *continuous = (dev == 0.0);
+  double bad = 0.0;
+  for (double v : h) bad = std::max(bad, v);
+  *ok = (bad == 0.0);
   return NKH_OK;
 }
 
-int gs_sum_mask_dot(const GsMap &gs, double *w, const double *mask, const double *p,
-                    const double *mult, const CgState *st, double *rho_part, bool compact,
-                    hipStream_t s) {
-  if (compact) {
-    hipLaunchKernelGGL(k_gs_sum_mask_dot_c, dim3(kRedBlocks), dim3(256), 0, s, w, p, gs.off,
-                       gs.idx, gs.nseg, gs.single, gs.nsingle, gs.coef, st, rho_part);
-  } else {
-    hipLaunchKernelGGL(k_gs_sum_mask_dot, dim3(kRedBlocks), dim3(256), 0, s, w, mask, p, mult,
-                       gs.off, gs.idx, gs.nseg, gs.single, gs.nsingle, st, rho_part);
-  }
+int gs_assemble(const GsMap &gs, double *w, const CgState *st, hipStream_t s) {
+  if (gs.ngroups == 0) return NKH_OK;
+  const int nb = kGsBlocks;
+  const int64_t chunk = (gs.ngroups + nb - 1) / nb;
+  hipLaunchKernelGGL(k_gs_assemble, dim3(nb), dim3(256), 0, s, w, gs.off, gs.idx, gs.flag,
+                     gs.ngroups, chunk, st);
   NK_HIP(hipGetLastError());
   return NKH_OK;
 }

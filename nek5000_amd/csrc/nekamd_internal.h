@@ -16,14 +16,20 @@ constexpr int kMaxNP = 16;
 constexpr int kRedBlocks = 2048;
 // Partial-sum slots of the CG-fused axhelm (one per block of its fixed grid).
 constexpr int kAxBlocks = 16384;
+// Blocks of the CG assembly kernel (a multiple of the 8 XCDs).
+constexpr int kGsBlocks = 8192;
 constexpr int kMaxCgIter = 900;  // hmholtz.f:636 maxcg
+// Blocks of the CG vector-update kernel (more blocks than the other
+// reductions: the 6-read/2-write stream reaches its bandwidth with more
+// waves in flight).
+constexpr int kUpdBlocks = 16384;
 // Layout of Mesh::red (doubles).
 constexpr int kRedPart1 = 0;                         // (z,r) partials
-constexpr int kRedPart2 = kRedBlocks;                // (r,r)_binv partials
-constexpr int kRedRho2 = 2 * kRedBlocks;             // rho partials of the gs kernel
-constexpr int kRedScratch = 3 * kRedBlocks;          // setup reductions
-constexpr int kRedOut = 4 * kRedBlocks;              // 64 result slots
-constexpr int kRedRho1 = 4 * kRedBlocks + 64;        // rho partials of axhelm
+constexpr int kRedPart2 = kUpdBlocks;                // (r,r)_binv partials
+constexpr int kRedRho2 = 2 * kUpdBlocks;             // rho partials (unfused path)
+constexpr int kRedScratch = kRedRho2 + kRedBlocks;   // setup reductions
+constexpr int kRedOut = kRedScratch + kRedBlocks;    // 64 result slots
+constexpr int kRedRho1 = kRedOut + 64;               // rho partials of axhelm
 constexpr int kRedHist = kRedRho1 + kAxBlocks;       // 5*kMaxCgIter history
 constexpr int kRedSize = kRedHist + 5 * kMaxCgIter;
 
@@ -54,22 +60,20 @@ void set_error(const char *fmt, ...);
   } while (0)
 
 // Gather-scatter map over the local nodes of one mesh (gslib gs_op semantics
-// on one rank).  Nodes with a nonzero global id are grouped by id, groups in
-// increasing id order.  off/idx list the groups of two or more members;
-// single lists the nonzero-id nodes without partner.
+// on one rank).  Nodes with a nonzero global id form groups (members in
+// increasing local index); groups are listed in the order of their first
+// member.  Groups of one member (unshared ids) are listed too.
 struct GsMap {
   int64_t n = 0;
-  int64_t nseg = 0;
+  int64_t ngroups = 0;    // all nonzero ids
   int64_t nmem = 0;
-  int64_t nsingle = 0;
-  int32_t *off = nullptr;     // device, nseg+1
+  int64_t nseg = 0;       // groups with >= 2 members
+  int64_t nsingle = 0;    // groups with one member
+  int32_t *off = nullptr;     // device, ngroups+1
   int32_t *idx = nullptr;     // device, nmem
-  int32_t *single = nullptr;  // device, nsingle
+  uint8_t *flag = nullptr;    // device, ngroups (gs_prepare)
   bool interior_zero = false; // glo_num == 0 exactly at element-interior nodes
   int max_mult = 0;
-  // per group (segments, then singles): {mask, sum of mult over members},
-  // filled once per solve by gs_prepare when mask and mult are continuous
-  double2 *coef = nullptr;
 };
 
 // State of one PCG solve, resident on the device.  The scalar kernels update
@@ -154,17 +158,13 @@ int launch_axhelm_cg(const Mesh &m, double *w, double *p, const double *r, const
 int gs_build(GsMap &gs, const int64_t *glo_num_host, int64_t n, int np, int64_t nel);
 void gs_free(GsMap &gs);
 int gs_op(const GsMap &gs, double *u, int op, hipStream_t s);
-// gs '+' on w fused with w*=mask on all nonzero-id nodes and the rho partial
-// sum over them (w*p*mult); skipped when st->done.  With `compact` the
-// per-group coefficients from gs_prepare replace the per-member mask/mult/p
-// reads (valid when those fields are continuous).
-int gs_sum_mask_dot(const GsMap &gs, double *w, const double *mask, const double *p,
-                    const double *mult, const CgState *st, double *rho_part, bool compact,
-                    hipStream_t s);
-// Fill gs.coef from mask and mult; *continuous tells whether every member of
-// every group carries the same mask and mult value (red: kRedBlocks scratch).
-int gs_prepare(GsMap &gs, const double *mask, const double *mult, double *red,
-               bool *continuous, hipStream_t s);
+// CG assembly: w <- mask * dssum(w) using the flags of gs_prepare; skipped
+// when st->done.
+int gs_assemble(const GsMap &gs, double *w, const CgState *st, hipStream_t s);
+// Flags per group from mask/mult; *ok when every group has a 0/1 mask equal
+// at all members and mult summing to 1 (red: kRedBlocks scratch).
+int gs_prepare(GsMap &gs, const double *mask, const double *mult, double *red, bool *ok,
+               hipStream_t s);
 
 // ---- cg.hip ----
 int run_cggo(Mesh &m, double *x, const double *f, Coef h1, Coef h2, const double *mask,

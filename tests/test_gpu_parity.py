@@ -212,3 +212,20 @@ def test_large_mesh_residual(gpu):
     say(f"[large] niter={out['niter']} rn/rbn0={rn / out['rbn0']:.3e}")
     assert out["converged"] == 1
     assert rn <= 1.0001e-8 * out["rbn0"]
+
+
+def test_cggo_unfused_path(gpu):
+    """cggo on an rhs that is neither assembled nor masked: the fused
+    iteration's preconditions fail, so the unfused sequence (axhelm, dssum,
+    mask, weighted dot) runs; it must still match the oracle."""
+    P, F = setup(gpu, box((3, 3, 2), 6, deform=0.1))
+    f = P.bm1 * P.uniform_field(21)
+    f = f * (1.0 + 0.1 * np.cos(np.arange(f.size)).reshape(f.shape))  # discontinuous
+    h1 = np.ones(P.shape)
+    h2 = np.full(P.shape, 2.0)
+    x_o, n_o, hist_o, _ = P.field.cggo(f, h1, h2, -1e-8, 900)
+    x = F.zeros()
+    out = F.cggo(x, F.to_dev(f), F.to_dev(h1), F.to_dev(h2), F.mask, F.mult, -1e-8, 900)
+    say(f"[cggo-unfused] oracle={n_o} gpu={out['niter']}")
+    assert abs(out["niter"] - n_o) <= 1
+    assert l2rel(host(x), x_o) < 1e-9
