@@ -42,10 +42,9 @@ def make_rhs(box, glo, ngv, bm1_host=None, seed=2024):
 
 
 def fused_axhelm_bytes_per_node(lx1, h1_field=False, h2_kind="uniform"):
-    """Algorithmic HBM bytes per local node of the fused CG axhelm kernel:
-    reads r, d, p (3 words), g1..g6 (6), bm1 when h2 != 0 (1), h1 when it is
-    a field (1), mult at element-interior nodes (fraction ((N-1)/(N+1))^3);
-    writes p and w (2)."""
+    """Algorithmic HBM bytes per local node of the CG-fused axhelm kernel:
+    reads r, d, p (3 words), g1..g6 (6), bm1 when h2 != 0 (1), h1 and h2
+    when they are fields; writes p and w (2)."""
     words = 3 + 6 + 2
     if h2_kind == "uniform":
         words += 1
@@ -53,8 +52,7 @@ def fused_axhelm_bytes_per_node(lx1, h1_field=False, h2_kind="uniform"):
         words += 2
     if h1_field:
         words += 1
-    fint = ((lx1 - 2) / lx1) ** 3
-    return 8.0 * (words + fint)
+    return 8.0 * words
 
 
 def cpu_baseline(nel1d, lx1, deform, target_seconds):

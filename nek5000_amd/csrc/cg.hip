@@ -36,17 +36,19 @@ __device__ __forceinline__ double wave_sum(double v) {
   return v;
 }
 
-// Sum of `count` partials laid out with stride 1, by one 256-thread block in
-// a fixed order.  Result valid on thread 0.
+// Sum of `count` partials by one block of kScalThreads threads in a fixed
+// order.  Result valid on thread 0.
+constexpr int kScalThreads = 1024;
 __device__ double block_sum_array(const double *__restrict__ part, int count) {
-  __shared__ double sb[4];
+  __shared__ double sb[kScalThreads / 64];
   double v = 0.0;
   for (int q = threadIdx.x; q < count; q += blockDim.x) v += part[q];
   v = wave_sum(v);
   if ((threadIdx.x & 63) == 0) sb[threadIdx.x >> 6] = v;
   __syncthreads();
   double t = 0.0;
-  if (threadIdx.x == 0) t = ((sb[0] + sb[1]) + sb[2]) + sb[3];
+  if (threadIdx.x == 0)
+    for (int q = 0; q < (int)(blockDim.x >> 6); ++q) t += sb[q];
   __syncthreads();
   return t;
 }
@@ -174,7 +176,7 @@ __global__ void k_cg_init(double *__restrict__ x, double *__restrict__ r,
 // x += alpha p ; r -= alpha w (when an update is pending) ; partial sums of
 // (d*r)*r*mult and binv*mult*r*r (vlsc3 / vlsc32, hmholtz.f:754-759).
 // VEC: 16-byte loads of node pairs (all six arrays 16-byte aligned).
-template <bool VEC>
+template <bool VEC, bool UPD>
 __global__ void __launch_bounds__(256)
     k_cg_update_dots(double *__restrict__ x, double *__restrict__ r,
                      const double *__restrict__ p, const double *__restrict__ w,
@@ -182,7 +184,9 @@ __global__ void __launch_bounds__(256)
                      const CgState *__restrict__ st, double *__restrict__ part1,
                      double *__restrict__ part2) {
   if (st->done) return;
-  const bool upd = st->pending != 0;
+  // UPD: the host launches the update variant for every iteration after the
+  // first; a pending update exists exactly then (or the solve is done).
+  constexpr bool upd = UPD;
   const double alpha = st->alpha;
   double s1 = 0.0, s2 = 0.0;
   const int64_t tid = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
@@ -198,7 +202,7 @@ __global__ void __launch_bounds__(256)
     const double2 *bm2 = reinterpret_cast<const double2 *>(bm);
     for (int64_t i = tid; i < n2; i += stride) {
       double2 ri = r2[i];
-      if (upd) {
+      if constexpr (upd) {
         double2 xi = x2[i];
         const double2 pi = p2[i], wi = w2[i];
         xi.x = xi.x + alpha * pi.x;
@@ -219,7 +223,7 @@ __global__ void __launch_bounds__(256)
   }
   for (int64_t i = done_upto + tid; i < n; i += stride) {
     double ri = r[i];
-    if (upd) {
+    if constexpr (upd) {
       x[i] = x[i] + alpha * p[i];
       ri = ri + (-alpha) * w[i];
       r[i] = ri;
@@ -231,7 +235,7 @@ __global__ void __launch_bounds__(256)
   block_store2(s1, s2, part1, part2);
 }
 
-__global__ void __launch_bounds__(256)
+__global__ void __launch_bounds__(kScalThreads)
     k_cg_scalars1(CgState *__restrict__ st, const double *__restrict__ part1,
                   const double *__restrict__ part2, double *__restrict__ hist) {
   if (st->done) return;
@@ -266,7 +270,7 @@ __global__ void __launch_bounds__(256)
   st->beta = (iter == 1) ? 0.0 : st->rtz1 / st->rtz2;
 }
 
-__global__ void __launch_bounds__(256)
+__global__ void __launch_bounds__(kScalThreads)
     k_cg_scalars2(CgState *__restrict__ st, const double *__restrict__ rp1, int n1,
                   const double *__restrict__ rp2, int n2, double *__restrict__ hist) {
   if (st->done) return;
@@ -543,20 +547,22 @@ int run_cggo(Mesh &m, double *x, const double *f, Coef h1, Coef h2, const double
     NK_HIP(hipEventRecord(t_solve0, s));
   }
   while (true) {
-    if (vec_ok)
-      hipLaunchKernelGGL(k_cg_update_dots<true>, dim3(kUpdBlocks), dim3(256), 0, s, x, r, p, w,
-                         dm, bm, n, m.st, part1, part2);
-    else
-      hipLaunchKernelGGL(k_cg_update_dots<false>, dim3(kUpdBlocks), dim3(256), 0, s, x, r, p,
-                         w, dm, bm, n, m.st, part1, part2);
-    hipLaunchKernelGGL(k_cg_scalars1, dim3(1), dim3(256), 0, s, m.st, part1, part2, hist);
+    {
+      auto kern = vec_ok ? (launched == 0 ? k_cg_update_dots<true, false>
+                                          : k_cg_update_dots<true, true>)
+                         : (launched == 0 ? k_cg_update_dots<false, false>
+                                          : k_cg_update_dots<false, true>);
+      hipLaunchKernelGGL(kern, dim3(kUpdBlocks), dim3(256), 0, s, x, r, p, w, dm, bm, n, m.st,
+                         part1, part2);
+    }
+    hipLaunchKernelGGL(k_cg_scalars1, dim3(1), dim3(kScalThreads), 0, s, m.st, part1, part2, hist);
     if (fused) {
       if (timed) NK_HIP(hipEventRecord(ev[4 * launched + 0], s));
       NK_TRY(launch_axhelm_cg(m, w, p, r, d, h1, h2, mult, m.st, rho1, s));
       if (timed) NK_HIP(hipEventRecord(ev[4 * launched + 1], s));
       NK_TRY(gs_assemble(m.gs, w, m.st, s));
       if (timed) NK_HIP(hipEventRecord(ev[4 * launched + 2], s));
-      hipLaunchKernelGGL(k_cg_scalars2, dim3(1), dim3(256), 0, s, m.st, rho1, ax_grid_parts,
+      hipLaunchKernelGGL(k_cg_scalars2, dim3(1), dim3(kScalThreads), 0, s, m.st, rho1, ax_grid_parts,
                          rho2, 0, hist);
     } else {
       NK_TRY(launch_axhelm_cg(m, w, p, r, d, h1, h2, mult, m.st, nullptr, s));
@@ -564,7 +570,7 @@ int run_cggo(Mesh &m, double *x, const double *f, Coef h1, Coef h2, const double
       hipLaunchKernelGGL(k_mask_guarded, dim3(grid_for(n)), dim3(256), 0, s, w, mask, n, m.st);
       hipLaunchKernelGGL(k_dot3_guarded, dim3(kRedBlocks), dim3(256), 0, s, w, p, mult, n,
                          m.st, rho1);
-      hipLaunchKernelGGL(k_cg_scalars2, dim3(1), dim3(256), 0, s, m.st, rho1, kRedBlocks,
+      hipLaunchKernelGGL(k_cg_scalars2, dim3(1), dim3(kScalThreads), 0, s, m.st, rho1, kRedBlocks,
                          rho2, 0, hist);
     }
     NK_HIP(hipGetLastError());
