@@ -83,6 +83,28 @@ def lib():
         L.nko_hmholtz.argtypes = [ctypes.POINTER(_Field), vp, vp, vp, vp, c_double, c_int,
                                   ctypes.POINTER(Params), vp, ctypes.POINTER(c_double)]
         L.nko_hmholtz.restype = c_int
+        L.nko_fd_weights_full.argtypes = [c_double, vp, c_int, c_int, vp]
+        L.nko_semhat.argtypes = [c_int, vp, vp, vp, vp]
+        L.nko_generalev.argtypes = [vp, vp, vp, c_int]
+        L.nko_mg_setup.argtypes = [ctypes.POINTER(_Field), vp, vp, vp, vp, vp, vp]
+        L.nko_mg_setup.restype = vp
+        L.nko_mg_levels.argtypes = [vp, vp]
+        L.nko_mg_levels.restype = c_int
+        L.nko_mg_solve.argtypes = [vp, vp, vp]
+        L.nko_flex_cg.argtypes = [ctypes.POINTER(_Field), vp, vp, vp, vp, c_int, c_double,
+                                  c_int, vp, ctypes.POINTER(c_double)]
+        L.nko_flex_cg.restype = c_int
+        for nm in ("nko_mg_fdm_s", "nko_mg_fdm_d", "nko_mg_mask", "nko_mg_swt", "nko_mg_rwt",
+                   "nko_mg_jh"):
+            getattr(L, nm).argtypes = [vp, c_int]
+            getattr(L, nm).restype = ctypes.POINTER(c_double)
+        L.nko_mg_lengths.argtypes = [vp, c_int]
+        L.nko_mg_lengths.restype = ctypes.POINTER(c_double)
+        L.nko_mg_ncrs.argtypes = [vp]
+        L.nko_mg_ncrs.restype = c_int
+        L.nko_mg_coarse_solve.argtypes = [vp, vp, vp]
+        L.nko_mg_schwarz.argtypes = [vp, c_int, vp, vp]
+        L.nko_mg_apply_fdm.argtypes = [vp, c_int, vp, vp]
         L.nko_num_threads.restype = c_int
         L.nko_set_num_threads.argtypes = [c_int]
         _lib = L
@@ -240,6 +262,86 @@ class Field:
                                   ctypes.addressof(rec), ctypes.byref(rbn0))
         hist = np.array([[q.rbn2, q.rtz1, q.rho, q.alpha, q.beta] for q in rec[: niter + 1]])
         return u, r, niter, hist, rbn0.value
+
+
+# ---- fast3d.f / hsmg.f ----
+def fd_weights_full(xx, x, m=1):
+    x = _c(x)
+    n = len(x) - 1
+    c = np.zeros((n + 1) * (m + 1))
+    lib().nko_fd_weights_full(float(xx), _p(x), n, m, _p(c))
+    return c.reshape(m + 1, n + 1)  # c[k, j]: k-th derivative weight of x_j
+
+
+def semhat(n):
+    a = np.zeros((n + 1) ** 2)
+    b = np.zeros(n + 1)
+    d = np.zeros((n + 1) ** 2)
+    z = np.zeros(n + 1)
+    lib().nko_semhat(n, _p(a), _p(b), _p(d), _p(z))
+    return a.reshape(n + 1, n + 1).T.copy(), b, d.reshape(n + 1, n + 1).T.copy(), z
+
+
+def generalev(A, B):
+    n = A.shape[0]
+    a = _c(np.asarray(A).T).copy()
+    lam = np.zeros(n)
+    lib().nko_generalev(_p(a), _p(_c(np.asarray(B).T)), _p(lam), n)
+    return a.reshape(n, n).T.copy(), lam
+
+
+# face order: symmetric notation (r-, r+, s-, s+, t-, t+); preprocessor
+# notation (connect1.f facind) is (s-, r+, s+, r-, t-, t+)
+SYM_TO_PREP = [3, 1, 0, 2, 4, 5]
+
+
+def fbc_from_box(box, code=1):
+    """BC codes (get_fast_bc) per element face in symmetric notation: `code`
+    (1 = outflow 'O', Dirichlet for the solve) on domain-boundary faces."""
+    prep = box.dirichlet_faces()
+    return np.ascontiguousarray(prep[:, SYM_TO_PREP].astype(np.int32) * code)
+
+
+class MG:
+    """hsmg.f h1mg_setup / h1mg_solve and hmh_flex_cg on a Field."""
+
+    def __init__(self, field, vertex, fbc, x, y, z, w):
+        self.field = field
+        self._keep = [_c(vertex, np.int64), _c(fbc, np.int32), _c(x), _c(y), _c(z), _c(w)]
+        v, f, xx, yy, zz, ww = self._keep
+        self.h = lib().nko_mg_setup(ctypes.byref(field._s), _p(v), _p(f), _p(xx), _p(yy),
+                                    _p(zz), _p(ww))
+        if not self.h:
+            raise RuntimeError("oracle mg setup failed")
+        nh = np.zeros(4, dtype=np.int32)
+        self.lmax = lib().nko_mg_levels(self.h, _p(nh))
+        self.nh = [int(v) for v in nh[: self.lmax]]  # nh[l-1] for level l
+
+    def level_shape(self, l):
+        n = self.nh[l - 1]
+        return (self.field.nel, n, n, n)
+
+    def _arr(self, fn, l, count):
+        ptr = getattr(lib(), fn)(self.h, l)
+        return np.ctypeslib.as_array(ptr, shape=(count,)).copy()
+
+    def mask(self, l):
+        return self._arr("nko_mg_mask", l, int(np.prod(self.level_shape(l)))).reshape(self.level_shape(l))
+
+    def solve(self, rhs):
+        r = _c(rhs).copy()
+        z = np.zeros_like(r)
+        lib().nko_mg_solve(self.h, _p(z), _p(r))
+        return z
+
+    def flex_cg(self, rhs, h1, h2, maxit, param21, istep=1):
+        res = _c(rhs).copy()
+        hist = np.zeros(maxit)
+        div0 = c_double(0)
+        it = lib().nko_flex_cg(ctypes.byref(self.field._s), self.h, _p(res), _p(_c(h1)),
+                               _p(_c(h2)), int(maxit), float(param21), int(istep),
+                               _p(hist), ctypes.byref(div0))
+        return res, it, hist[:it], div0.value
 
 
 def num_threads():

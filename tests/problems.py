@@ -51,3 +51,9 @@ def box(nel, lx1, **kw):
     if isinstance(nel, int):
         nel = (nel, nel, nel)
     return BoxMesh(nel[0], nel[1], nel[2], lx1, **kw)
+
+
+def mg_for(P, code=1):
+    """Oracle multigrid hierarchy for a Problem (Dirichlet via outflow faces)."""
+    fbc = O.fbc_from_box(P.box, code)
+    return O.MG(P.field, P.box.vertex, fbc, P.x, P.y, P.zc, P.w)
