@@ -41,7 +41,8 @@ typedef struct {
   double param22;  /* param(22): chktcg1 gate (hmholtz.f:54) and relative tol when <0 (:764) */
   double restol;   /* restol(ifield): overrides tin when nonzero and tin>0 (hmholtz.f:677) */
   int precision;   /* 0: FP64 (the drop-in); 1: FP32 operator inside FP64 refinement */
-  int reserved;
+  int param42;     /* param(42): PRES solver, 2 -> hmh_flex_cg with h1mg (hmholtz.f:646-653) */
+  double param21;  /* param(21): PRES tolerance cap (>0) or relative tolerance (<0) (hmholtz.f:2206-2220) */
 } nkh_params;
 
 /* What the reference leaves behind after a solve. */
@@ -112,14 +113,43 @@ int nkd_dssum(int mesh, double *u);
 int nkd_dsop(int mesh, double *u, char op);
 /* hmholtz.f:380-524 SETPREC: inverse of the assembled Jacobi diagonal. */
 int nkd_setprec(int mesh, double *d, const double *h1, const double *h2);
-/* hmholtz.f:2-69 HMHOLTZ (non-PRES field): dssum(rhs); rhs*=mask; chktcg1;
-   cggo.  rhs is modified in place as in the reference; u receives the
-   solution.  history (optional, 5*min(maxit,900) doubles) receives per
-   iteration (rbn2, rtz1, rho, alpha, beta). */
+/* hmholtz.f:2-69 HMHOLTZ: dssum(rhs); rhs*=mask; chktcg1; cggo.  name
+   "PRES" with prm->param42 == 2 takes cggo's branch to hmh_flex_cg
+   (hmholtz.f:646-653; needs nkh_mg_setup); other PRES solvers (GMRES,
+   param42 0) return NKH_ERR_UNSUPPORTED.  rhs is modified in place as in
+   the reference; u receives the solution.  history (optional) receives per
+   Jacobi-PCG iteration (rbn2, rtz1, rho, alpha, beta), 5*min(maxit,900)
+   doubles, or per flexible-CG iteration the residual norm, maxit doubles. */
 int nkd_hmholtz(int mesh, const char *name, double *u, double *rhs, const double *h1,
                 const double *h2, const double *mask, const double *mult, double tli,
                 int maxit, int isd, const nkh_params *prm, nkh_result *res,
                 double *history);
+/* ---- H1 multigrid (PnPn pressure) ------------------------------------- */
+
+/* hsmg.f:2234-2270 H1MG_SETUP (with hsmg.f:2272-2323 level orders, :2361-2394
+   level numbering, :633-665 FDM, :3048-3104 Schwarz weights, fast3d.f:1542-1617
+   swap_lengths and navier8.f:83-233 set_up_h1_crs) for a mesh that has its
+   geometry and gs map.  Host arrays: fbc = 6 codes per element in symmetric
+   face order (r-,r+,s-,s+,t-,t+) with get_fast_bc's meaning (fast3d.f:802-877:
+   0 interior or periodic, 1 Dirichlet, 2 Neumann); vertex = the 8 corner ids
+   per element (Nek's vertex(8,nel)); xm1/ym1/zm1 = GLL coordinates. */
+int nkh_mg_setup(int mesh, const int *fbc, const int64_t *vertex, const double *xm1,
+                 const double *ym1, const double *zm1);
+/* hsmg.f:1855-1949 H1MG_SOLVE: z = M rhs.  rhs is masked in place (h1mg_mask
+   at the fine level, as in the reference). */
+int nkd_mg_solve(int mesh, double *z, double *rhs);
+/* hmholtz.f:2164-2281 HMH_FLEX_CG: res holds the rhs on entry and the
+   solution on return; wt = the multiplicity weight (vmult at the reference's
+   call site hmholtz.f:653), pmask = the pressure mask.  Tolerance from
+   chktcg1, param21 and istep as in the reference.  rnorm_history (optional,
+   maxit doubles) receives the residual norm of every iteration. */
+int nkd_hmh_flex_cg(int mesh, double *res, const double *h1, const double *h2,
+                    const double *wt, const double *pmask, int maxit, const nkh_params *prm,
+                    nkh_result *res_out, double *rnorm_history);
+/* Hierarchy facts: number of levels, points per direction per level (nh[0]
+   = level 1), coarse unknowns, mean coarse CG iterations per solve so far. */
+int nkh_mg_info(int mesh, int *lmax, int *nh, int *ncrs, double *crs_avg_iters);
+
 /* hmholtz.f:611-846 CGGO (Jacobi branch): f is the assembled, masked rhs
    (left unchanged), binv = binvm1/bintm1 of the mesh. */
 int nkd_cggo(int mesh, double *x, const double *f, const double *h1, const double *h2,

@@ -25,7 +25,8 @@ class NkhParams(ctypes.Structure):
         ("param22", c_double),
         ("restol", c_double),
         ("precision", c_int),
-        ("reserved", c_int),
+        ("param42", c_int),
+        ("param21", c_double),
     ]
 
 
@@ -67,6 +68,11 @@ SIGNATURES = {
                             c_void_p]),
     "nkd_cggo": (c_int, [c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
                          c_double, c_int, P(NkhParams), P(NkhResult), c_void_p]),
+    "nkh_mg_setup": (c_int, [c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]),
+    "nkd_mg_solve": (c_int, [c_int, c_void_p, c_void_p]),
+    "nkd_hmh_flex_cg": (c_int, [c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_int,
+                                c_void_p, c_void_p, c_void_p]),
+    "nkh_mg_info": (c_int, [c_int, P(c_int), c_void_p, P(c_int), P(c_double)]),
     "nkh_timers_enable": (c_int, [c_int]),
     "nkh_timers_reset": (c_int, []),
     "nkh_timers_read": (c_int, [c_char_p, P(c_double), P(c_int64)]),

@@ -592,6 +592,7 @@ int nkh_mesh_destroy(int id) {
   Mesh *m = get_mesh(id);
   NK_CHECK(m, NKH_ERR_ARG, "nkh_mesh_destroy: bad mesh %d", id);
   NK_HIP(hipStreamSynchronize(g_stream));
+  mg_free(*m);
   gs_free(m->gs);
   if (m->Dd) (void)hipFree(m->Dd);
   if (m->g) (void)hipFree(m->g);
@@ -684,6 +685,7 @@ int nkh_mesh_set_geom(int id, const double *g1, const double *g2, const double *
 int nkh_mesh_gs_setup(int id, const int64_t *glo_num) {
   Mesh *m = get_mesh(id);
   NK_CHECK(m && glo_num, NKH_ERR_ARG, "nkh_mesh_gs_setup: bad args");
+  mg_free(*m);  // the hierarchy shares the fine map
   NK_TRY(gs_build(m->gs, glo_num, m->n, m->np, m->nel));
   m->has_gs = true;
   return finish_mass(*m);
@@ -760,7 +762,9 @@ int nkd_setprec(int id, double *d, const double *h1, const double *h2) {
   return run_setprec(*m, d, c1, c2, g_stream);
 }
 
-static const nkh_params kDefaultParams = {1, 0, 1.0e-7, 1.0e-7, 0, 0};
+// Nek's defaults (reader_par.f:26-27,76-79): param(22) = restol = 1e-7,
+// param(42) = 0, param(21) = 0.
+static const nkh_params kDefaultParams = {1, 0, 1.0e-7, 1.0e-7, 0, 0, 0.0};
 
 int nkd_cggo(int id, double *x, const double *f, const double *h1, const double *h2,
              const double *mask, const double *mult, double tin, int maxit,
@@ -783,20 +787,64 @@ int nkd_hmholtz(int id, const char *name, double *u, double *rhs, const double *
   NK_TRY(ready(m, id, true));
   NK_CHECK(u && rhs && h1 && h2 && mask && mult, NKH_ERR_ARG, "nkd_hmholtz: null array");
   NK_CHECK(isd == 1 || isd == 2, NKH_ERR_ARG, "nkd_hmholtz: isd=%d", isd);
-  NK_CHECK(!(name && std::strncmp(name, "PRES", 4) == 0), NKH_ERR_UNSUPPORTED,
-           "nkd_hmholtz: the PRES (pressure) branch is not part of this path");
   const nkh_params &p = prm ? *prm : kDefaultParams;
   NK_CHECK(p.precision == 0, NKH_ERR_UNSUPPORTED, "nkd_hmholtz: precision=%d", p.precision);
+  const bool pres = name && std::strncmp(name, "PRES", 4) == 0;
+  NK_CHECK(!pres || p.param42 == 2, NKH_ERR_UNSUPPORTED,
+           "nkd_hmholtz: PRES with param42=%d (only 2, flexible CG with h1mg, is built)",
+           p.param42);
   hipStream_t s = g_stream;
   Coef c1, c2;
   NK_TRY(classify_coefs(*m, h1, h2, &c1, &c2, s));
   // hmholtz.f:50-51: dssum(rhs); rhs *= mask
   NK_TRY(gs_op(m->gs, rhs, 0, s));
   NK_TRY(mul_inplace(rhs, mask, m->n, s));
+  if (pres) {
+    // hmholtz.f:52-55's chktcg1 only sets cggo's tolerance, which the
+    // flexible-CG branch (hmholtz.f:646-653) does not read: x = f, then
+    // hmh_flex_cg(x, h1, h2, mult, maxit)
+    NK_HIP(hipMemcpyAsync(u, rhs, sizeof(double) * m->n, hipMemcpyDeviceToDevice, s));
+    return run_flex_cg(*m, u, c1, c2, mult, mask, maxit, p, res, history);
+  }
   double tol = std::fabs(tli);
   if (p.param22 == 0.0 || p.istep <= 10) NK_TRY(run_chktcg1(*m, &tol, rhs, c1, c2, mask, mult));
   if (tli < 0) tol = tli;
   return run_cggo(*m, u, rhs, c1, c2, mask, mult, tol, maxit, p, res, history);
+}
+
+int nkh_mg_setup(int id, const int *fbc, const int64_t *vertex, const double *xm1,
+                 const double *ym1, const double *zm1) {
+  Mesh *m = get_mesh(id);
+  NK_TRY(ready(m, id, true));
+  NK_CHECK(fbc && vertex && xm1 && ym1 && zm1, NKH_ERR_ARG, "nkh_mg_setup: null array");
+  return mg_setup(*m, fbc, vertex, xm1, ym1, zm1);
+}
+
+int nkd_mg_solve(int id, double *z, double *rhs) {
+  Mesh *m = get_mesh(id);
+  NK_TRY(ready(m, id, true));
+  NK_CHECK(z && rhs, NKH_ERR_ARG, "nkd_mg_solve: null array");
+  return mg_solve(*m, z, rhs, g_stream);
+}
+
+int nkd_hmh_flex_cg(int id, double *res, const double *h1, const double *h2, const double *wt,
+                    const double *pmask, int maxit, const nkh_params *prm, nkh_result *out,
+                    double *rnorm_history) {
+  Mesh *m = get_mesh(id);
+  NK_TRY(ready(m, id, true));
+  NK_CHECK(res && h1 && h2 && wt && pmask, NKH_ERR_ARG, "nkd_hmh_flex_cg: null array");
+  NK_CHECK(maxit >= 0, NKH_ERR_ARG, "nkd_hmh_flex_cg: maxit=%d", maxit);
+  const nkh_params &p = prm ? *prm : kDefaultParams;
+  NK_CHECK(p.precision == 0, NKH_ERR_UNSUPPORTED, "nkd_hmh_flex_cg: precision=%d", p.precision);
+  Coef c1, c2;
+  NK_TRY(classify_coefs(*m, h1, h2, &c1, &c2, g_stream));
+  return run_flex_cg(*m, res, c1, c2, wt, pmask, maxit, p, out, rnorm_history);
+}
+
+int nkh_mg_info(int id, int *lmax, int *nh, int *ncrs, double *crs_avg_iters) {
+  Mesh *m = get_mesh(id);
+  NK_CHECK(m, NKH_ERR_ARG, "nkh_mg_info: bad mesh %d", id);
+  return mg_info(*m, lmax, nh, ncrs, crs_avg_iters);
 }
 
 int nkh_timers_enable(int on) {

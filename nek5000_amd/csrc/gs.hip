@@ -123,7 +123,7 @@ void gs_free(GsMap &gs) {
 
 template <int OP>
 __device__ __forceinline__ double gs_combine(double a, double b) {
-  if constexpr (OP == 0) return a + b;
+  if constexpr (OP == 0 || OP == 4) return a + b;
   else if constexpr (OP == 1) return a * b;
   else if constexpr (OP == 2) return b < a ? b : a;
   else return b > a ? b : a;
@@ -138,6 +138,8 @@ __global__ void k_gs_op(double *__restrict__ u, const int32_t *__restrict__ off,
     if (b - a < 2) continue;
     double acc = u[idx[a]];
     for (int32_t c = a + 1; c < b; ++c) acc = gs_combine<OP>(acc, u[idx[c]]);
+    // op 4 (dsavg, ic.f:1871-1897): the sum times vmult = 1/multiplicity
+    if constexpr (OP == 4) acc = acc * (1.0 / (double)(b - a));
     for (int32_t c = a; c < b; ++c) u[idx[c]] = acc;
   }
 }
@@ -152,6 +154,7 @@ int gs_op(const GsMap &gs, double *u, int op, hipStream_t s) {
     case 1: hipLaunchKernelGGL(k_gs_op<1>, dim3(grid), dim3(256), 0, s, u, gs.off, gs.idx, ng); break;
     case 2: hipLaunchKernelGGL(k_gs_op<2>, dim3(grid), dim3(256), 0, s, u, gs.off, gs.idx, ng); break;
     case 3: hipLaunchKernelGGL(k_gs_op<3>, dim3(grid), dim3(256), 0, s, u, gs.off, gs.idx, ng); break;
+    case 4: hipLaunchKernelGGL(k_gs_op<4>, dim3(grid), dim3(256), 0, s, u, gs.off, gs.idx, ng); break;
     default: set_error("gs_op: unknown op %d", op); return NKH_ERR_ARG;
   }
   NK_HIP(hipGetLastError());

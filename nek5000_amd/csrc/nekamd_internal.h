@@ -93,6 +93,8 @@ struct CgState {
   int param18;
 };
 
+struct Mg;  // multigrid hierarchy (hsmg.hip)
+
 struct Mesh {
   int np = 0;        // GLL points per direction (lx1)
   int64_t nel = 0;
@@ -113,6 +115,7 @@ struct Mesh {
   size_t ws_doubles = 0;
   CgState *st = nullptr;       // device
   double *red = nullptr;       // device, partial sums
+  Mg *mg = nullptr;            // H1 multigrid hierarchy (nkh_mg_setup)
 };
 
 // CTIMER-style accumulators (seconds, calls), see nkh_timers_*.
@@ -157,6 +160,7 @@ int launch_axhelm_cg(const Mesh &m, double *w, double *p, const double *r, const
 // ---- gs.hip ----
 int gs_build(GsMap &gs, const int64_t *glo_num_host, int64_t n, int np, int64_t nel);
 void gs_free(GsMap &gs);
+// op: 0 sum, 1 product, 2 min, 3 max, 4 sum times 1/multiplicity (dsavg)
 int gs_op(const GsMap &gs, double *u, int op, hipStream_t s);
 // CG assembly: w <- mask * dssum(w) using the flags of gs_prepare; skipped
 // when st->done.
@@ -176,5 +180,17 @@ int classify_coefs(Mesh &m, const double *h1, const double *h2, Coef *c1, Coef *
                    hipStream_t s);
 int run_setprec(Mesh &m, double *d, Coef h1, Coef h2, hipStream_t s);
 int mul_inplace(double *a, const double *b, int64_t n, hipStream_t s);
+
+// ---- hsmg.hip ----
+// fbc: 6 codes per element in symmetric face order (0 interior/periodic,
+// 1 Dirichlet, 2 Neumann); vertex: 8 ids per element; x/y/z: fine GLL
+// coordinates.  All host arrays.
+int mg_setup(Mesh &m, const int *fbc, const int64_t *vertex, const double *x, const double *y,
+             const double *z);
+void mg_free(Mesh &m);
+int mg_solve(Mesh &m, double *z, double *rhs, hipStream_t s);
+int mg_info(const Mesh &m, int *lmax, int *nh, int *ncrs, double *crs_avg_iters);
+int run_flex_cg(Mesh &m, double *res, Coef h1, Coef h2, const double *wt, const double *mask,
+                int maxit, const nkh_params &prm, nkh_result *out, double *rnorm_history);
 
 }  // namespace nekamd

@@ -200,6 +200,17 @@ class BoxMesh:
         f[:, 5] = ez == self.nelz - 1
         return f
 
+    # symmetric face order (r-, r+, s-, s+, t-, t+) -> preprocessor face
+    # (connect1.f:1009 facind numbering: s-, r+, s+, r-, t-, t+)
+    SYM_TO_PREP = (3, 1, 0, 2, 4, 5)
+
+    def fast_bc(self, code=1):
+        """get_fast_bc codes (fast3d.f:802-877) per element face in symmetric
+        order: `code` on domain-boundary faces (1 = 'O', Dirichlet for the
+        pressure), 0 elsewhere.  Shape (nel, 6) int32."""
+        prep = self.dirichlet_faces()
+        return np.ascontiguousarray(prep[:, list(self.SYM_TO_PREP)].astype(np.int32) * code)
+
     def face_mask(self):
         """bdry.f:317- BCMASK before its dsop: 1 everywhere, 0 on the nodes of
         Dirichlet faces (facev, connect1.f:1058).  Shape (nel, lz1, ly1, lx1)."""

@@ -8,6 +8,9 @@ Names and argument meanings follow the reference routines:
   * ``Field.hmholtz(name, u, rhs, h1, h2, mask, mult, tli, maxit, isd)``
                                                       -- hmholtz.f:2
   * ``Field.cggo(x, f, h1, h2, mask, mult, tin, maxit)`` -- hmholtz.f:611
+  * ``Field.h1mg_setup(fbc, vertex, x, y, z)``        -- hsmg.f:2234
+  * ``Field.h1mg_solve(z, rhs)``                      -- hsmg.f:1855
+  * ``Field.hmh_flex_cg(res, h1, h2, wt, maxit)``     -- hmholtz.f:2164
 
 Arrays are torch tensors on the GPU (float64, Nek layout (nel,lz1,ly1,lx1)
 C-order = (lx1,ly1,lz1,nel) column-major).  torch is only the allocator and
@@ -58,9 +61,11 @@ def _ptr(t):
     return ctypes.c_void_p(t.data_ptr())
 
 
-def default_params(istep=1, param18=0, param22=1e-7, restol=1e-7, precision=0):
-    """Nek defaults (reader_par.f:26-27,76-79): param(22)=1e-7, restol=param(22)."""
-    return NkhParams(istep, param18, param22, restol, precision, 0)
+def default_params(istep=1, param18=0, param22=1e-7, restol=1e-7, precision=0, param42=0,
+                   param21=0.0):
+    """Nek defaults (reader_par.f:26-27,76-79): param(22)=1e-7, restol=param(22);
+    param(42)=0 and param(21)=0 unless the case file sets them."""
+    return NkhParams(istep, param18, param22, restol, precision, param42, param21)
 
 
 class Field:
@@ -158,10 +163,13 @@ class Field:
     def hmholtz(self, name, u, rhs, h1, h2, mask, mult, tli, maxit, isd=1, params=None,
                 history=False):
         """hmholtz.f:2: rhs is dssum'd and masked in place; u gets the solution.
-        Returns a dict with niter, rbn0, rbn2, tol (and history if asked)."""
+        Returns a dict with niter, rbn0, rbn2, tol (and history if asked: per
+        Jacobi-PCG iteration (rbn2, rtz1, rho, alpha, beta), or for 'PRES'
+        with param42=2 the flexible-CG residual norm per iteration)."""
         prm = params if params is not None else default_params()
         res = NkhResult()
-        hist = np.zeros(5 * 900) if history else None
+        pres = name[:4] == "PRES"
+        hist = np.zeros(max(int(maxit), 1) if pres else 5 * 900) if history else None
         _sync_stream(self.lib)
         check(self.lib.nkd_hmholtz(self.handle, name.encode(), _ptr(u), _ptr(rhs), _ptr(h1),
                                    _ptr(h2), _ptr(mask), _ptr(mult), float(tli), int(maxit),
@@ -171,7 +179,8 @@ class Field:
         out = dict(niter=res.niter, converged=res.converged, rbn0=res.rbn0, rbn2=res.rbn2,
                    tol=res.tol)
         if history:
-            out["history"] = hist.reshape(900, 5)[: max(res.niter + 1, 1)].copy()
+            out["history"] = (hist[: res.niter].copy() if pres else
+                              hist.reshape(900, 5)[: max(res.niter + 1, 1)].copy())
         return out
 
     def cggo(self, x, f, h1, h2, mask, mult, tin, maxit, params=None, history=False):
@@ -189,6 +198,52 @@ class Field:
         if history:
             out["history"] = hist.reshape(900, 5)[: max(res.niter + 1, 1)].copy()
         return out
+
+    # ---- H1 multigrid (PnPn pressure) ------------------------------------
+    def h1mg_setup(self, fbc, vertex, x, y, z):
+        """hsmg.f:2234 h1mg_setup.  fbc: (nel, 6) int codes in symmetric face
+        order (0 interior/periodic, 1 Dirichlet, 2 Neumann); vertex: (nel, 8)
+        int64; x, y, z: GLL coordinates (host arrays)."""
+        arrs = [np.ascontiguousarray(fbc, dtype=np.int32).reshape(-1),
+                np.ascontiguousarray(vertex, dtype=np.int64).reshape(-1)]
+        arrs += [np.ascontiguousarray(a, dtype=np.float64).reshape(-1) for a in (x, y, z)]
+        assert arrs[0].size == 6 * self.nel and arrs[1].size == 8 * self.nel
+        assert all(a.size == self.n for a in arrs[2:])
+        _sync_stream(self.lib)
+        check(self.lib.nkh_mg_setup(self.handle, *[a.ctypes.data for a in arrs]), "nkh_mg_setup")
+
+    def h1mg_info(self):
+        lmax = ctypes.c_int()
+        nh = np.zeros(4, dtype=np.int32)
+        ncrs = ctypes.c_int()
+        avg = ctypes.c_double()
+        check(self.lib.nkh_mg_info(self.handle, ctypes.byref(lmax), nh.ctypes.data,
+                                   ctypes.byref(ncrs), ctypes.byref(avg)), "nkh_mg_info")
+        return dict(lmax=lmax.value, nh=[int(v) for v in nh[: lmax.value]], ncrs=ncrs.value,
+                    crs_avg_iters=avg.value)
+
+    def h1mg_solve(self, z, rhs):
+        """hsmg.f:1855 h1mg_solve: z = M rhs (rhs masked in place)."""
+        _sync_stream(self.lib)
+        check(self.lib.nkd_mg_solve(self.handle, _ptr(z), _ptr(rhs)), "nkd_mg_solve")
+
+    def hmh_flex_cg(self, res, h1, h2, wt, maxit, pmask=None, params=None, history=False):
+        """hmholtz.f:2164 hmh_flex_cg: res = rhs in, solution out."""
+        prm = params if params is not None else default_params()
+        out = NkhResult()
+        hist = np.zeros(max(int(maxit), 1)) if history else None
+        mask = self.mask if pmask is None else pmask
+        _sync_stream(self.lib)
+        check(self.lib.nkd_hmh_flex_cg(self.handle, _ptr(res), _ptr(h1), _ptr(h2), _ptr(wt),
+                                       _ptr(mask), int(maxit), ctypes.byref(prm),
+                                       ctypes.byref(out),
+                                       ctypes.c_void_p(hist.ctypes.data) if history else None),
+              "nkd_hmh_flex_cg")
+        d = dict(niter=out.niter, converged=out.converged, div0=out.rbn0, rnorm=out.rbn2,
+                 tolpss=out.tol)
+        if history:
+            d["history"] = hist[: out.niter].copy()
+        return d
 
     def destroy(self):
         if self.handle is not None and self.handle >= 0:
@@ -210,10 +265,11 @@ def _copy_raw_to_tensor(ptr, t):
         raise _lib.NekAmdError(f"hipMemcpy failed: {rc}")
 
 
-def box_field(box: BoxMesh, device=0):
+def box_field(box: BoxMesh, device=0, multigrid=False):
     """Set up a Field for a BoxMesh the way nek_init does (drive1.f:2-179):
     numbering (setvert3d), geometry (geodat1, on the GPU), gs map, binvm1,
-    Dirichlet mask (bcmask: facev then dsop '*'), vmult = 1/dssum(1)."""
+    Dirichlet mask (bcmask: facev then dsop '*'), vmult = 1/dssum(1); with
+    multigrid=True also the h1mg hierarchy (Dirichlet faces as 'O')."""
     f = Field(box.lx1, box.nel, device)
     glo, ngv = box.glo_num()
     f.ngv = ngv
@@ -227,4 +283,6 @@ def box_field(box: BoxMesh, device=0):
     mult = 1.0 / mult
     f.mask = mask
     f.mult = mult
+    if multigrid:
+        f.h1mg_setup(box.fast_bc(), box.vertex, x, y, z)
     return f

@@ -1,0 +1,167 @@
+"""GPU parity of the H1 multigrid (hsmg.f h1mg_setup / h1mg_solve) and the
+flexible CG around it (hmholtz.f:2164 hmh_flex_cg), through the C ABI,
+against the oracle on the same seeded inputs.
+
+Tolerances (FP64): the hierarchy's integer facts (levels, points per level,
+coarse unknowns) exactly; one V-cycle z = M r within 1e-10 relative L2 (the
+FDM eigenvectors come from the same Cholesky + Jacobi algorithm compiled by
+a different compiler, and the coarse problem is solved by CG to a 1e-13
+relative residual instead of a direct factor); flexible-CG iteration counts
+within +-1, residual-norm histories within 1e-6 relative over the common
+iterations, solutions within 1e-9 relative L2.
+"""
+import sys
+
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from problems import Problem, box, mask_from_fbc, mg_for
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def host(t):
+    return t.detach().cpu().numpy()
+
+
+def l2rel(a, b):
+    return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300))
+
+
+MG_CASES = [
+    dict(nel=3, lx1=8, deform=0.1),
+    dict(nel=(3, 2, 4), lx1=5, jitter=0.05, seed=4),
+    dict(nel=3, lx1=4, deform=0.05),
+    dict(nel=(4, 3, 3), lx1=7, periodic_x=True, jitter=0.04, seed=7),
+    dict(nel=(2, 3, 3), lx1=10, deform=0.1),
+    dict(nel=(3, 3, 2), lx1=6, neumann_top=True),
+]
+
+
+def build(gpu, case):
+    case = dict(case)
+    neumann_top = case.pop("neumann_top", False)
+    b = box(**case)
+    fbc = b.fast_bc()
+    if neumann_top:
+        top = (np.arange(b.nel) // (b.nelx * b.nely)) == b.nelz - 1
+        fbc[top, 5] = 2
+    say(f"[hsmg setup] {case} neumann_top={neumann_top}")
+    P = Problem(b, fbc=fbc if neumann_top else None)
+    mg = mg_for(P)
+    F = gpu.box_field(b)
+    if neumann_top:
+        m = F.to_dev(mask_from_fbc(fbc, b.lx1))
+        F.dsop(m, "*")
+        F.mask = m
+    assert np.array_equal(host(F.mask), P.mask)
+    x, y, z = b.xyz()
+    F.h1mg_setup(fbc, b.vertex, x, y, z)
+    return P, mg, F
+
+
+@pytest.mark.parametrize("case", MG_CASES)
+def test_hierarchy_facts(gpu, case):
+    P, mg, F = build(gpu, case)
+    info = F.h1mg_info()
+    assert info["lmax"] == mg.lmax
+    assert info["nh"] == mg.nh
+    assert info["ncrs"] == O.lib().nko_mg_ncrs(mg.h)
+    F.destroy()
+
+
+@pytest.mark.parametrize("case", MG_CASES)
+def test_h1mg_solve(gpu, case):
+    P, mg, F = build(gpu, case)
+    rng = np.random.default_rng(11)
+    r = P.field.dssum(P.bm1 * rng.standard_normal(P.shape))
+    zo = mg.solve(r)
+    rd = F.to_dev(r)
+    zd = F.zeros()
+    say(f"[h1mg_solve] {case}")
+    F.h1mg_solve(zd, rd)
+    zg = host(zd)
+    # the input is masked in place at the fine level (hsmg.f h1mg_mask)
+    np.testing.assert_array_equal(host(rd), r * mg.mask(mg.lmax))
+    e = l2rel(zg, zo)
+    say(f"[h1mg_solve] rel L2 = {e:.3e}")
+    assert e < 1e-10
+    # z is continuous (dsavg) and zero where the fine mask is
+    zz = F.zeros()
+    zz.copy_(zd)
+    F.dsop(zz, "M")
+    assert np.max(np.abs(host(zz) - zg)) <= 1e-14 * np.max(np.abs(zg))
+    F.destroy()
+
+
+@pytest.mark.parametrize("case", MG_CASES)
+@pytest.mark.parametrize("h2v", [0.0, 10.0])
+def test_flex_cg(gpu, case, h2v):
+    P, mg, F = build(gpu, case)
+    h1 = np.ones(P.shape)
+    h2 = np.full(P.shape, h2v)
+    b = P.bm1 * P.uniform_field(21)
+    rhs = P.field.dssum(b) * P.mask
+    uo, ito, histo, div0 = mg.flex_cg(rhs, h1, h2, 200, -1e-10)
+    say(f"[flex_cg] {case} h2={h2v}: oracle iterations {ito}")
+    # through hmholtz('PRES', ...) with param(42)=2 (hmholtz.f:646-653)
+    u = F.zeros()
+    rd = F.to_dev(b)
+    prm = gpu.default_params(param42=2, param21=-1e-10)
+    out = F.hmholtz("PRES", u, rd, F.to_dev(h1), F.to_dev(h2), F.mask, F.mult, -1e-10, 200,
+                    params=prm, history=True)
+    say(f"[flex_cg] gpu iterations {out['niter']} rnorm {out['rbn2']:.3e}")
+    np.testing.assert_allclose(host(rd), rhs, rtol=0, atol=1e-14 * np.max(np.abs(rhs)))
+    assert abs(out["niter"] - ito) <= 1
+    k = min(out["niter"], ito)
+    np.testing.assert_allclose(out["history"][:k], histo[:k], rtol=1e-6)
+    assert abs(out["rbn0"] - div0) <= 1e-12 * div0
+    e = l2rel(host(u), uo)
+    say(f"[flex_cg] solution rel L2 = {e:.3e}")
+    assert e < 1e-9
+    F.destroy()
+
+
+def test_flex_cg_entry_and_tolerances(gpu):
+    """nkd_hmh_flex_cg directly: istep=0 forces tolps=1e-4 (absolute), a
+    positive param21 caps the chktcg1 tolerance, maxit bounds the count."""
+    P, mg, F = build(gpu, dict(nel=3, lx1=6, deform=0.1))
+    h1 = np.ones(P.shape)
+    h2 = np.zeros(P.shape)
+    rhs = P.field.dssum(P.bm1 * P.uniform_field(3)) * P.mask
+    for param21, istep, maxit in ((0.0, 0, 100), (1e-6, 1, 100), (-1e-12, 1, 3)):
+        uo, ito, histo, div0 = mg.flex_cg(rhs, h1, h2, maxit, param21, istep)
+        res = F.to_dev(rhs)
+        prm = gpu.default_params(istep=istep, param42=2, param21=param21)
+        out = F.hmh_flex_cg(res, F.to_dev(h1), F.to_dev(h2), F.mult, maxit, params=prm,
+                            history=True)
+        say(f"[flex_cg entry] param21={param21} istep={istep}: {out['niter']} vs {ito}")
+        assert abs(out["niter"] - ito) <= 1
+        if maxit == 3:
+            assert out["niter"] == 3 and not out["converged"]
+        else:
+            assert out["converged"]
+            assert out["rnorm"] < out["tolpss"]
+        assert l2rel(host(res), uo) < 1e-8
+    F.destroy()
+
+
+def test_pres_other_solvers_unsupported(gpu):
+    b = box(2, 6)
+    F = gpu.box_field(b)
+    u = F.zeros()
+    r = F.zeros() + 1.0
+    one = F.zeros() + 1.0
+    with pytest.raises(gpu._lib.NekAmdError):
+        F.hmholtz("PRES", u, r, one, F.zeros(), F.mask, F.mult, -1e-8, 10,
+                  params=gpu.default_params(param42=0))
+    with pytest.raises(gpu._lib.NekAmdError):  # no hierarchy yet
+        F.hmholtz("PRES", u, r, one, F.zeros(), F.mask, F.mult, -1e-8, 10,
+                  params=gpu.default_params(param42=2))
+    F.destroy()
