@@ -95,6 +95,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--maxit", type=int, default=900,
                     help="iteration cap per solve (900 = the reference's maxcg)")
+    ap.add_argument("--solver", choices=("hsmg", "jacobi"), default="hsmg",
+                    help="hsmg: hmholtz('PRES') -> hmh_flex_cg with h1mg (param42=2); "
+                         "jacobi: hmholtz -> cggo")
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
@@ -114,6 +117,13 @@ def main():
     log(f"[bench] mesh E={box.nel} lx1={box.lx1} n={box.n}")
     F = solver.box_field(box)
     log(f"[bench] field set up in {time.time() - t0:.1f} s")
+    hsmg = args.solver == "hsmg"
+    if hsmg:
+        t1 = time.time()
+        x, y, z = box.xyz()
+        F.h1mg_setup(box.fast_bc(), box.vertex, x, y, z)
+        del x, y, z
+        log(f"[bench] h1mg set up in {time.time() - t1:.1f} s: {F.h1mg_info()}")
     f = make_rhs(box, F.glo_num, F.ngv)
     bm1, _ = F.mass()
     rhs0 = bm1 * torch.as_tensor(f, device="cuda")
@@ -123,9 +133,12 @@ def main():
     free_dof = float(torch.sum(F.mask * F.mult))
     log(f"[bench] free unique DOF = {free_dof:.0f}")
 
+    prm = solver.default_params(param42=2, param21=-args.tol) if hsmg else None
+
     def solve(r):
         u = torch.zeros_like(rhs0)
-        return F.hmholtz("TEMP", u, r, h1, h2, F.mask, F.mult, -args.tol, args.maxit)
+        return F.hmholtz("PRES" if hsmg else "TEMP", u, r, h1, h2, F.mask, F.mult, -args.tol,
+                         args.maxit, params=prm)
 
     # plain copy of 2 GiB by torch: the achievable-bandwidth reference for
     # the roofline numbers (and the calibration kernel of the PMC byte counts)
@@ -166,11 +179,18 @@ def main():
     t_ax = sec.value / max(calls.value, 1)
     lib.nkh_timers_read(b"dsum", ctypes.byref(sec), ctypes.byref(calls))
     t_gs = sec.value / max(calls.value, 1)
-    bpn = fused_axhelm_bytes_per_node(args.lx1)
+    if hsmg:
+        # plain axhelm of ax (gmres.f:284): reads u, g1..g6, bm1; writes au
+        bpn = 8.0 * 9
+        ax_kernel = f"k_axhelm<{args.lx1},0,..> (axhelm of ax, h2 uniform)"
+    else:
+        bpn = fused_axhelm_bytes_per_node(args.lx1)
+        ax_kernel = f"k_axhelm<{args.lx1},1,false,1> (CG-fused axhelm)"
     ax_bytes = bpn * box.n
     achieved = ax_bytes / t_ax / 1e9
     total_iters = sum(iters)
     value = free_dof * total_iters / elapsed / 1e9
+    mg_info = F.h1mg_info() if hsmg else None
 
     cpu = None
     if not args.no_cpu:
@@ -194,8 +214,13 @@ def main():
         "config": {
             "workload": (f"hmholtz E={args.nel}^3={box.nel} N={args.lx1 - 1} deformed box "
                          f"(0.1h sin map), h1=1 h2=10, f~U(-1,1), Dirichlet, "
-                         f"Jacobi-PCG FP64 to {args.tol:g} relative"),
-            "nel": box.nel, "lx1": args.lx1, "preconditioner": "jacobi", "precision": "fp64",
+                         + (f"flexible CG + h1mg V-cycle (PRES, param42=2) FP64 to "
+                            f"{args.tol:g} relative" if hsmg else
+                            f"Jacobi-PCG FP64 to {args.tol:g} relative")),
+            "nel": box.nel, "lx1": args.lx1,
+            "preconditioner": "h1mg" if hsmg else "jacobi", "precision": "fp64",
+            "multigrid": mg_info,
+            "seconds_per_solve": elapsed / args.steps,
             "free_dof": free_dof, "iterations_per_solve": iters,
             "axhelm_ms": t_ax * 1e3, "gs_ms": t_gs * 1e3,
             "ms_per_iteration": elapsed / total_iters * 1e3,
@@ -204,7 +229,7 @@ def main():
         "roofline": {
             "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-            "kernel": "k_axhelm<8,1,false,1> (CG-fused axhelm)",
+            "kernel": ax_kernel,
             "bytes_per_launch": ax_bytes,
         },
         "cpu_baseline": cpu,

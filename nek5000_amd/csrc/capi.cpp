@@ -824,6 +824,7 @@ int nkd_mg_solve(int id, double *z, double *rhs) {
   Mesh *m = get_mesh(id);
   NK_TRY(ready(m, id, true));
   NK_CHECK(z && rhs, NKH_ERR_ARG, "nkd_mg_solve: null array");
+  NK_TRY(mg_mask_fine(*m, rhs, g_stream));  // h1mg_mask of the fine level, in place
   return mg_solve(*m, z, rhs, g_stream);
 }
 

@@ -330,7 +330,7 @@ __global__ void __launch_bounds__(256)
     const int i = q % np, j = (q / np) % np, k = q / (np * np);
     if (i > 0 && i < np - 1 && j > 0 && j < np - 1 && k > 0 && k < np - 1)
       v = fmax(v, fmax(fabs(mask[l] - 1.0), fabs(mult[l] - 1.0)));
-    if (mask[l] == 0.0) v = fmax(v, fabs(f[l]));
+    if (f && mask[l] == 0.0) v = fmax(v, fabs(f[l]));
   }
   for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_down(v, o, 64));
   if ((threadIdx.x & 63) == 0) sb[threadIdx.x >> 6] = v;
@@ -420,6 +420,28 @@ int run_setprec(Mesh &m, double *d, Coef h1, Coef h2, hipStream_t s) {
   return NKH_OK;
 }
 
+// The conditions under which a CG denominator (mask * QQ^T w, p)_mult may be
+// taken as sum_l p_l w_l before the assembly: ids zero exactly at element
+// interiors, mask 1 and mult 1 there, f (when given) zero where masked, and
+// per group a 0/1 mask and mult summing to 1.  Leaves the group flags of
+// gs_prepare in m.gs.
+int fused_conditions(Mesh &m, const double *mask, const double *mult, const double *f, bool *ok,
+                     hipStream_t s) {
+  *ok = false;
+  if (!m.gs.interior_zero) return NKH_OK;
+  NK_TRY(ensure_small(m));
+  hipLaunchKernelGGL(k_fused_checks, dim3(kRedBlocks), dim3(256), 0, s, mask, mult, f, m.n, m.np,
+                     m.red + kRedScratch);
+  hipLaunchKernelGGL(k_reduce_final, dim3(1), dim3(256), 0, s, m.red + kRedScratch, kRedBlocks,
+                     2, m.red + kRedOut);
+  NK_HIP(hipGetLastError());
+  double dev;
+  NK_HIP(hipMemcpyAsync(&dev, m.red + kRedOut, sizeof(double), hipMemcpyDeviceToHost, s));
+  NK_HIP(hipStreamSynchronize(s));
+  if (dev != 0.0) return NKH_OK;
+  return gs_prepare(m.gs, mask, mult, m.red + kRedScratch, ok, s);
+}
+
 int run_chktcg1(Mesh &m, double *tol, const double *res, Coef h1, Coef h2,
                 const double *mask, const double *mult) {
   hipStream_t s = cur_stream();
@@ -491,23 +513,8 @@ int run_cggo(Mesh &m, double *x, const double *f, Coef h1, Coef h2, const double
   // conditions hold: ids zero exactly at element interiors, mask 1 and mult 1
   // there, f zero where masked, and per group a 0/1 mask and mult summing
   // to 1.  Checked once per solve.
-  bool fused = m.gs.interior_zero;
-  if (fused) {
-    hipLaunchKernelGGL(k_fused_checks, dim3(kRedBlocks), dim3(256), 0, s, mask, mult, f, n,
-                       m.np, m.red + kRedScratch);
-    hipLaunchKernelGGL(k_reduce_final, dim3(1), dim3(256), 0, s, m.red + kRedScratch,
-                       kRedBlocks, 2, m.red + kRedOut);
-    NK_HIP(hipGetLastError());
-    double dev;
-    NK_HIP(hipMemcpyAsync(&dev, m.red + kRedOut, sizeof(double), hipMemcpyDeviceToHost, s));
-    NK_HIP(hipStreamSynchronize(s));
-    fused = (dev == 0.0);
-  }
-  if (fused) {
-    bool ok = false;
-    NK_TRY(gs_prepare(m.gs, mask, mult, m.red + kRedScratch, &ok, s));
-    fused = ok;
-  }
+  bool fused = false;
+  NK_TRY(fused_conditions(m, mask, mult, f, &fused, s));
   const int ax_grid_parts = axhelm_cg_grid(m);
   auto al16 = [](const void *q) { return (reinterpret_cast<uintptr_t>(q) & 15) == 0; };
   const bool vec_ok = al16(x) && al16(r) && al16(p) && al16(w) && al16(dm) && al16(bm);

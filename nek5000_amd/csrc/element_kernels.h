@@ -48,7 +48,9 @@ __device__ __forceinline__ double block_sum(double v, double *sbuf) {
 // MODE 0: au = A u.  MODE 1 (CG): p = d*r + beta*p is formed on the fly and
 // stored, w = A_L p (unassembled), and sum_l p_l w_l -- the CG denominator
 // rho = (Q Q^T w, p)_mult for a continuous, masked p -- is written to one
-// slot per block when rho_part is given.
+// slot per block when rho_part is given.  MODE 2 (flexible CG): the same
+// with p = beta*p + z (add2s1, hmholtz.f:2243), z passed as r, beta read
+// from *betap.
 // ---------------------------------------------------------------------------
 template <int NP, int MODE, bool H1F, int H2M>
 __global__ void __launch_bounds__(NP * NP * epb<NP>())
@@ -58,7 +60,8 @@ __global__ void __launch_bounds__(NP * NP * epb<NP>())
              double h2s, const double *__restrict__ bm1, int64_t nel,
              double *__restrict__ pout, const double *__restrict__ r,
              const double *__restrict__ dinv, const double *__restrict__ mult,
-             const CgState *__restrict__ st, double *__restrict__ rho_part) {
+             const CgState *__restrict__ st, double *__restrict__ rho_part,
+             const double *__restrict__ betap) {
   constexpr int NP2 = NP * NP, NP3 = NP2 * NP, EPB = epb<NP>();
   __shared__ double s_D[NP][NP + 1];
   __shared__ double s_u[EPB][NP3];
@@ -71,6 +74,7 @@ __global__ void __launch_bounds__(NP * NP * epb<NP>())
     if (st->done) return;
     beta = st->beta;
   }
+  if constexpr (MODE == 2) beta = *betap;
   const int tid = threadIdx.x;
   const int le = tid / NP2;
   const int ij = tid - le * NP2;
@@ -91,6 +95,9 @@ __global__ void __launch_bounds__(NP * NP * epb<NP>())
         const int64_t l = base + k * NP2;
         if constexpr (MODE == 1) {
           v = beta * uin[l] + dinv[l] * r[l];  // add2s1: p = beta*p + z, z = r*d
+          pout[l] = v;
+        } else if constexpr (MODE == 2) {
+          v = beta * uin[l] + r[l];
           pout[l] = v;
         } else {
           v = uin[l];
@@ -150,11 +157,11 @@ __global__ void __launch_bounds__(NP * NP * epb<NP>())
         if constexpr (H2M == 1) a = a + (h2s * bm1[l]) * r_u[k];
         if constexpr (H2M == 2) a = a + (h2f[l] * bm1[l]) * r_u[k];
         au[l] = a;
-        if constexpr (MODE == 1) rho += a * r_u[k];  // p . (A_L p), element-local
+        if constexpr (MODE >= 1) rho += a * r_u[k];  // p . (A_L p), element-local
       }
     }
   }
-  if constexpr (MODE == 1) {
+  if constexpr (MODE >= 1) {
     if (rho_part != nullptr) {
       const double t = block_sum(rho, s_red);
       if (threadIdx.x == 0) rho_part[blockIdx.x] = t;
@@ -166,13 +173,13 @@ template <int NP, int MODE>
 static int axhelm_dispatch(const Mesh &m, double *au, const double *u, Coef h1, Coef h2,
                            double *pout, const double *r, const double *dinv,
                            const double *mult, const CgState *st, double *rho_part,
-                           hipStream_t s) {
+                           hipStream_t s, const double *betap = nullptr) {
   DMat<NP> Dm;
   for (int t = 0; t < NP * NP; ++t) Dm.d[t] = m.D[t];
   constexpr int EPB = epb<NP>();
   const int64_t ngroups = (m.nel + EPB - 1) / EPB;
   int grid;
-  if (MODE == 1) grid = axhelm_cg_grid(m);
+  if (MODE >= 1) grid = axhelm_cg_grid(m);
   else grid = (int)((ngroups < (int64_t)1 << 20) ? ngroups : ((int64_t)1 << 20));
   if (grid < 1) grid = 1;
   const dim3 block(NP * NP * EPB);
@@ -182,7 +189,7 @@ static int axhelm_dispatch(const Mesh &m, double *au, const double *u, Coef h1, 
 #define NK_LAUNCH(H1F_, H2M_)                                                            \
   hipLaunchKernelGGL((k_axhelm<NP, MODE, H1F_, H2M_>), dim3(grid), block, 0, s, Dm, m.Dd, \
                      au, u, m.g, m.n, h1.f, h1.s, h2.f, h2.s, m.bm1, m.nel, pout, r, dinv, \
-                     mult, st, rho_part)
+                     mult, st, rho_part, betap)
   if (h1f) {
     if (h2m == 0) NK_LAUNCH(true, 0);
     else if (h2m == 1) NK_LAUNCH(true, 1);

@@ -171,7 +171,7 @@ __global__ void __launch_bounds__(256)
     k_gs_assemble(double *__restrict__ w, const int32_t *__restrict__ off,
                   const int32_t *__restrict__ idx, const uint8_t *__restrict__ flag,
                   int64_t ngroups, int64_t chunk, const CgState *__restrict__ st) {
-  if (st->done) return;
+  if (st && st->done) return;
   const int nb = gridDim.x, b = blockIdx.x;
   const int64_t lb = (int64_t)(b % 8) * (nb / 8) + b / 8;
   const int64_t s0 = lb * chunk;

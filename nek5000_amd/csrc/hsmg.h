@@ -1,0 +1,78 @@
+// Internal declarations of the H1 multigrid (hsmg_setup.hip builds the
+// hierarchy, hsmg.hip applies it and runs the flexible CG).
+#pragma once
+#include <vector>
+
+#include "coarse.h"
+#include "nekamd_internal.h"
+
+namespace nekamd {
+
+// Scalars of one hmh_flex_cg solve, resident on the device.
+struct FcgState {
+  double rho0, rho1, rho2, beta, den, alpha, rnorm, div0, tolpss;
+  int iter, done;
+};
+
+// One multigrid level.  Fields are (nh,nh,nh,nel) element-major.  On the
+// Schwarz levels (l >= 2) the overlapping subdomain of an element is its
+// (nh+2)^3 extension; its ghost faces are exchanged through two gather maps
+// built from the extended numbering (hsmg.f:2361-2394):
+//   gin[(e*6+f)*nh^2 + a + nh*b]  = index (in a level field) of the node of
+//        the face neighbour that lies one layer inside it, behind ghost node
+//        (f,a,b) of element e; -1 where the face has no neighbour;
+//   gout[...] = the neighbour's own ghost slot that coincides with (f,a,b).
+// Faces f in symmetric order r-, r+, s-, s+, t-, t+; (a,b) run over the two
+// other directions, lower direction first.
+struct MgLevel {
+  int nx = 0, nh = 0;
+  int64_t n = 0;  // nh^3 * nel
+  std::vector<double> ah, bh, zh;
+  double *J = nullptr;   // interpolation to level l+1: (nh[l+1] x nh[l]) col-major
+  double *Jt = nullptr;  // its transpose
+  GsMap gs;
+  bool own_gs = false;
+  double *mask = nullptr;  // 0/1
+  double *rwt = nullptr;   // restriction weights
+  double *mswt = nullptr;  // mask * Schwarz weight (l >= 2)
+  double *fs = nullptr;    // per element S_r, S_s, S_t ((nh+2)^2 each)
+  double *fl = nullptr;    // per element lam_r, lam_s, lam_t (nh+2 each), eps
+  int32_t *gin = nullptr, *gout = nullptr;
+  double *wint = nullptr;  // local solutions: interior (nh^3 per element)
+  // local solutions of the neighbours at this element's ghost slots (6 nh^2
+  // per element; 0 where a face has no neighbour)
+  double *wgh = nullptr;
+  double *e = nullptr, *r = nullptr;
+};
+
+struct Mg {
+  int lmax = 0;
+  int64_t nel = 0;
+  MgLevel lev[4];
+  int *fbc = nullptr;  // device, 6 per element
+  hipStream_t sb = nullptr;  // second stream: coarse levels beside the fine one
+  hipEvent_t ev_fork = nullptr, ev_join = nullptr;
+  // coarse problem (navier8.f:83-233): unknowns = free vertex ids
+  int ncrs = 0;
+  int32_t *slot = nullptr;                   // per level-1 node: coarse unknown or -1
+  int32_t *goff = nullptr, *gidx = nullptr;  // unknown -> level-1 nodes (CSR)
+  double *cb = nullptr, *cx = nullptr;       // coarse rhs and solution
+  CoarseSolver crs;
+  // flexible CG work (5 fine fields) and scalars
+  double *fr = nullptr, *fr1 = nullptr, *fp = nullptr, *fz = nullptr, *fw = nullptr;
+  double *fpart = nullptr;
+  FcgState *fst = nullptr;
+  int *fflag = nullptr;
+};
+
+int mg_grid(int64_t n);
+int mg_dalloc(double **p, int64_t n);
+
+// Schwarz solve on level l of r (masked): writes the level's e (not yet
+// summed) from the local FDM solutions; wint/wgh as scratch.
+int mg_schwarz_local(Mg &mg, int l, const double *r, double *e, hipStream_t s);
+// The overlap sum alone: e = interior of wint + the neighbours' solutions
+// held in wgh, on the first interior layer.
+int mg_overlap_out(Mg &mg, int l, double *e, hipStream_t s);
+
+}  // namespace nekamd

@@ -157,13 +157,18 @@ int launch_axhelm_cg(const Mesh &m, double *w, double *p, const double *r, const
                      Coef h1, Coef h2, const double *mult, const CgState *st,
                      double *rho_part, hipStream_t s);
 
+// axhelm fused into the flexible-CG iteration: p = beta*p + z (beta read
+// from *betap), w = A p, rho partials of p . w over all local nodes.
+int launch_axhelm_flex(const Mesh &m, double *w, double *p, const double *z, Coef h1, Coef h2,
+                       const double *betap, double *rho_part, hipStream_t s);
+
 // ---- gs.hip ----
 int gs_build(GsMap &gs, const int64_t *glo_num_host, int64_t n, int np, int64_t nel);
 void gs_free(GsMap &gs);
 // op: 0 sum, 1 product, 2 min, 3 max, 4 sum times 1/multiplicity (dsavg)
 int gs_op(const GsMap &gs, double *u, int op, hipStream_t s);
 // CG assembly: w <- mask * dssum(w) using the flags of gs_prepare; skipped
-// when st->done.
+// when st is given and st->done.
 int gs_assemble(const GsMap &gs, double *w, const CgState *st, hipStream_t s);
 // Flags per group from mask/mult; *ok when every group has a 0/1 mask equal
 // at all members and mult summing to 1 (red: kRedBlocks scratch).
@@ -179,6 +184,8 @@ int run_chktcg1(Mesh &m, double *tol, const double *res, Coef h1, Coef h2,
 int classify_coefs(Mesh &m, const double *h1, const double *h2, Coef *c1, Coef *c2,
                    hipStream_t s);
 int run_setprec(Mesh &m, double *d, Coef h1, Coef h2, hipStream_t s);
+int fused_conditions(Mesh &m, const double *mask, const double *mult, const double *f, bool *ok,
+                     hipStream_t s);
 int mul_inplace(double *a, const double *b, int64_t n, hipStream_t s);
 
 // ---- hsmg.hip ----
@@ -188,7 +195,9 @@ int mul_inplace(double *a, const double *b, int64_t n, hipStream_t s);
 int mg_setup(Mesh &m, const int *fbc, const int64_t *vertex, const double *x, const double *y,
              const double *z);
 void mg_free(Mesh &m);
-int mg_solve(Mesh &m, double *z, double *rhs, hipStream_t s);
+// z = M rhs for rhs already masked with the fine multigrid mask
+int mg_solve(Mesh &m, double *z, const double *rhs, hipStream_t s);
+int mg_mask_fine(Mesh &m, double *r, hipStream_t s);
 int mg_info(const Mesh &m, int *lmax, int *nh, int *ncrs, double *crs_avg_iters);
 int run_flex_cg(Mesh &m, double *res, Coef h1, Coef h2, const double *wt, const double *mask,
                 int maxit, const nkh_params &prm, nkh_result *out, double *rnorm_history);

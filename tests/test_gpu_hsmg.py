@@ -5,8 +5,9 @@ against the oracle on the same seeded inputs.
 Tolerances (FP64): the hierarchy's integer facts (levels, points per level,
 coarse unknowns) exactly; one V-cycle z = M r within 1e-10 relative L2 (the
 FDM eigenvectors come from the same Cholesky + Jacobi algorithm compiled by
-a different compiler, and the coarse problem is solved by CG to a 1e-13
-relative residual instead of a direct factor); flexible-CG iteration counts
+a different compiler, and above 600 unknowns the coarse problem is solved by
+AMG-preconditioned CG to a 1e-13 relative residual instead of a direct
+factor); flexible-CG iteration counts
 within +-1, residual-norm histories within 1e-6 relative over the common
 iterations, solutions within 1e-9 relative L2.
 """
@@ -41,6 +42,9 @@ MG_CASES = [
     dict(nel=(4, 3, 3), lx1=7, periodic_x=True, jitter=0.04, seed=7),
     dict(nel=(2, 3, 3), lx1=10, deform=0.1),
     dict(nel=(3, 3, 2), lx1=6, neumann_top=True),
+    # more than 600 coarse unknowns: the coarse problem goes through the
+    # AMG-preconditioned CG instead of the dense inverse
+    dict(nel=(12, 11, 10), lx1=4, deform=0.05),
 ]
 
 
@@ -164,4 +168,32 @@ def test_pres_other_solvers_unsupported(gpu):
     with pytest.raises(gpu._lib.NekAmdError):  # no hierarchy yet
         F.hmholtz("PRES", u, r, one, F.zeros(), F.mask, F.mult, -1e-8, 10,
                   params=gpu.default_params(param42=2))
+    F.destroy()
+
+
+def test_flex_cg_pmask_differs_from_mg_mask(gpu):
+    """pmask (all boundary faces Dirichlet) differs from the multigrid's fine
+    mask (top faces Neumann, code 2): r is masked again before every V-cycle
+    and the denominator is taken after the assembly (the unfused path)."""
+    b = box((3, 3, 2), 6, deform=0.1)
+    fbc_mg = b.fast_bc()
+    top = (np.arange(b.nel) // (b.nelx * b.nely)) == b.nelz - 1
+    fbc_mg[top, 5] = 2
+    P = Problem(b)
+    mg = O.MG(P.field, b.vertex, np.ascontiguousarray(fbc_mg, np.int32), P.x, P.y, P.zc, P.w)
+    h1 = np.ones(P.shape)
+    h2 = np.zeros(P.shape)
+    rhs = P.field.dssum(P.bm1 * P.uniform_field(8)) * P.mask
+    uo, ito, histo, div0 = mg.flex_cg(rhs, h1, h2, 200, -1e-10)
+    F = gpu.box_field(b)
+    x, y, z = b.xyz()
+    F.h1mg_setup(fbc_mg, b.vertex, x, y, z)
+    res = F.to_dev(rhs)
+    out = F.hmh_flex_cg(res, F.to_dev(h1), F.to_dev(h2), F.mult, 200,
+                        params=gpu.default_params(param42=2, param21=-1e-10), history=True)
+    say(f"[flex_cg pmask] gpu {out['niter']} oracle {ito}")
+    assert abs(out["niter"] - ito) <= 1
+    k = min(out["niter"], ito)
+    np.testing.assert_allclose(out["history"][:k], histo[:k], rtol=1e-6)
+    assert l2rel(host(res), uo) < 1e-9
     F.destroy()
