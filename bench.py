@@ -4,15 +4,16 @@
 Workload (the metric's configuration): E = 64^3 = 262,144 hexahedra, N = 7
 (lx1 = 8), every GLL node mapped by x_i <- x_i + 0.1 h sin(2 pi x) sin(2 pi y)
 sin(2 pi z), h1 = 1, h2 = 10, f ~ U(-1,1) per unique node (seeded), Dirichlet
-on all faces, solved by hmholtz (dssum, mask, chktcg1, Jacobi-PCG cggo) in
-FP64 to 1e-8 relative residual.  One step = one full solve with all inputs
-resident in HBM (setup excluded).
+on all faces, solved in FP64 to 1e-8 relative residual by hmholtz('PRES')
+-> flexible CG with the h1mg V-cycle (orders 7 -> 3 -> 1 + coarse solve;
+param(42) = 2), or with --solver jacobi by hmholtz -> Jacobi-PCG cggo.  One
+step = one full solve with all inputs resident in HBM (set-up excluded).
 
-value = (unique free DOF x PCG iterations) / wall time, in GDOF*iter/s.
-roofline: the fused axhelm kernel of the CG iteration (p update + operator +
-interior dot), average duration from the library's HIP-event timers.
-cpu_baseline: the oracle's cggo (C, OpenMP over all host threads) on a
-bounded sample of the same workload.
+value = (unique free DOF x CG iterations) / wall time, in GDOF*iter/s.
+roofline: the axhelm of the iteration with the search-direction update fused
+in, average duration from the library's HIP-event timers around it.
+cpu_baseline: the oracle (C, OpenMP over all host threads) solving the same
+operator with the same solver on a bounded sample (E = 16^3).
 """
 import argparse
 import json
@@ -55,31 +56,43 @@ def fused_axhelm_bytes_per_node(lx1, h1_field=False, h2_kind="uniform"):
     return 8.0 * words
 
 
-def cpu_baseline(nel1d, lx1, deform, target_seconds):
-    """The oracle's hmholtz on E=nel1d^3 with the same operator; bounded by
-    maxit so the run takes about target_seconds."""
+def cpu_baseline(nel1d, lx1, deform, target_seconds, solver="hsmg"):
+    """The oracle's solve of the same operator on E=nel1d^3, bounded by maxit
+    so that the timed part takes about target_seconds: hmholtz -> cggo
+    (Jacobi) or the flexible CG with the h1mg V-cycle (set-up not timed)."""
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     from oracle import oracle as O
-    from problems import Problem, box as mkbox
+    from problems import Problem, box as mkbox, mg_for
 
     P = Problem(mkbox(nel1d, lx1, deform=deform))
     h1 = np.ones(P.shape)
     h2 = np.full(P.shape, 10.0)
-    rhs = P.bm1 * P.uniform_field(2024)
-    # calibrate on a few iterations
-    t = time.perf_counter()
-    _, _, n_cal, _, _ = P.field.hmholtz(rhs, h1, h2, -1e-30, 5)
-    dt = (time.perf_counter() - t) / max(n_cal, 1)
-    maxit = int(max(10, min(900, target_seconds / max(dt, 1e-6))))
-    t = time.perf_counter()
-    _, _, niter, _, _ = P.field.hmholtz(rhs, h1, h2, -1e-8, maxit)
-    el = time.perf_counter() - t
+    b = P.bm1 * P.uniform_field(2024)
     free = P.free_dof()
+    if solver == "hsmg":
+        mg = mg_for(P)
+        rhs = P.field.dssum(b) * P.mask
+        t = time.perf_counter()
+        _, n_cal, _, _ = mg.flex_cg(rhs, h1, h2, 2, -1e-30)
+        dt = (time.perf_counter() - t) / max(n_cal, 1)
+        maxit = int(max(3, min(200, target_seconds / max(dt, 1e-6))))
+        t = time.perf_counter()
+        _, niter, _, _ = mg.flex_cg(rhs, h1, h2, maxit, -1e-8)
+        el = time.perf_counter() - t
+        what = f"flexible CG + h1mg V-cycle, {niter} iterations"
+    else:
+        t = time.perf_counter()
+        _, _, n_cal, _, _ = P.field.hmholtz(b, h1, h2, -1e-30, 5)
+        dt = (time.perf_counter() - t) / max(n_cal, 1)
+        maxit = int(max(10, min(900, target_seconds / max(dt, 1e-6))))
+        t = time.perf_counter()
+        _, _, niter, _, _ = P.field.hmholtz(b, h1, h2, -1e-8, maxit)
+        el = time.perf_counter() - t
+        what = f"hmholtz/cggo Jacobi-PCG, {niter} iterations"
     return dict(value=free * niter / el / 1e9, unit="GDOF*iter/s", cores=O.num_threads(),
                 kind="port",
-                sample=(f"oracle hmholtz/cggo (C, OpenMP) on E={nel1d}^3 N={lx1-1} "
-                        f"deformed, h1=1 h2=10, {niter} Jacobi-PCG iterations "
-                        f"({free:.0f} free DOF) in {el:.1f} s"))
+                sample=(f"oracle (C restatement, OpenMP) on E={nel1d}^3 N={lx1-1} deformed, "
+                        f"h1=1 h2=10: {what} ({free:.0f} free DOF) in {el:.1f} s"))
 
 
 def main():
@@ -180,9 +193,10 @@ def main():
     lib.nkh_timers_read(b"dsum", ctypes.byref(sec), ctypes.byref(calls))
     t_gs = sec.value / max(calls.value, 1)
     if hsmg:
-        # plain axhelm of ax (gmres.f:284): reads u, g1..g6, bm1; writes au
-        bpn = 8.0 * 9
-        ax_kernel = f"k_axhelm<{args.lx1},0,..> (axhelm of ax, h2 uniform)"
+        # axhelm of ax (gmres.f:284) with the p update fused: reads p, z,
+        # g1..g6, bm1; writes p, w
+        bpn = 8.0 * 11
+        ax_kernel = f"k_axhelm<{args.lx1},2,false,1> (flex-CG axhelm, p = beta p + z fused)"
     else:
         bpn = fused_axhelm_bytes_per_node(args.lx1)
         ax_kernel = f"k_axhelm<{args.lx1},1,false,1> (CG-fused axhelm)"
@@ -195,7 +209,7 @@ def main():
     cpu = None
     if not args.no_cpu:
         log("[bench] cpu baseline ...")
-        cpu = cpu_baseline(args.cpu_nel, args.lx1, 0.1, args.cpu_seconds)
+        cpu = cpu_baseline(args.cpu_nel, args.lx1, 0.1, args.cpu_seconds, args.solver)
         log(f"[bench] cpu baseline {cpu}")
 
     line = {

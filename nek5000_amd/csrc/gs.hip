@@ -190,6 +190,33 @@ __global__ void __launch_bounds__(256)
   }
 }
 
+// Last step of the multigrid V-cycle at the fine level, per group of shared
+// nodes: S = dssum of the Schwarz sum e (hsmg.f h1mg_schwarz's dssum), then
+// the weighted value plus the coarse correction already in z,
+// S*w_i + z_i, and dsavg of that (ic.f:1871-1897: dssum, times
+// 1/multiplicity).  Every id group, single members included.
+__global__ void __launch_bounds__(256)
+    k_gs_fin(double *__restrict__ z, const double *__restrict__ e, const double *__restrict__ w,
+             const int32_t *__restrict__ off, const int32_t *__restrict__ idx, int64_t ngroups,
+             int64_t chunk) {
+  const int nb = gridDim.x, b = blockIdx.x;
+  const int64_t lb = (int64_t)(b % 8) * (nb / 8) + b / 8;
+  const int64_t s0 = lb * chunk;
+  const int64_t s1 = (s0 + chunk < ngroups) ? s0 + chunk : ngroups;
+  for (int64_t g = s0 + threadIdx.x; g < s1; g += blockDim.x) {
+    const int32_t a = off[g], c = off[g + 1];
+    double S = e[idx[a]];
+    for (int32_t q = a + 1; q < c; ++q) S += e[idx[q]];
+    double acc = S * w[idx[a]] + z[idx[a]];
+    for (int32_t q = a + 1; q < c; ++q) {
+      const int32_t l = idx[q];
+      acc += S * w[l] + z[l];
+    }
+    acc = acc * (1.0 / (double)(c - a));
+    for (int32_t q = a; q < c; ++q) z[idx[q]] = acc;
+  }
+}
+
 // Group flags from mask and mult, plus the check that makes the fused CG
 // exact: in every group mask is 0 or 1 and equal at all members, and the
 // members' mult sum to 1 (then sum_l mult*w*p over the assembled, masked w
@@ -244,6 +271,16 @@ int gs_assemble(const GsMap &gs, double *w, const CgState *st, hipStream_t s) {
   const int64_t chunk = (gs.ngroups + nb - 1) / nb;
   hipLaunchKernelGGL(k_gs_assemble, dim3(nb), dim3(256), 0, s, w, gs.off, gs.idx, gs.flag,
                      gs.ngroups, chunk, st);
+  NK_HIP(hipGetLastError());
+  return NKH_OK;
+}
+
+int gs_fin(const GsMap &gs, double *z, const double *e, const double *w, hipStream_t s) {
+  if (gs.ngroups == 0) return NKH_OK;
+  const int nb = kGsBlocks;
+  const int64_t chunk = (gs.ngroups + nb - 1) / nb;
+  hipLaunchKernelGGL(k_gs_fin, dim3(nb), dim3(256), 0, s, z, e, w, gs.off, gs.idx, gs.ngroups,
+                     chunk);
   NK_HIP(hipGetLastError());
   return NKH_OK;
 }

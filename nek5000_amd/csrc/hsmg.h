@@ -58,8 +58,8 @@ struct Mg {
   int32_t *goff = nullptr, *gidx = nullptr;  // unknown -> level-1 nodes (CSR)
   double *cb = nullptr, *cx = nullptr;       // coarse rhs and solution
   CoarseSolver crs;
-  // flexible CG work (5 fine fields) and scalars
-  double *fr = nullptr, *fr1 = nullptr, *fp = nullptr, *fz = nullptr, *fw = nullptr;
+  // flexible CG work (4 fine fields) and scalars
+  double *fr = nullptr, *fp = nullptr, *fz = nullptr, *fw = nullptr;
   double *fpart = nullptr;
   FcgState *fst = nullptr;
   int *fflag = nullptr;

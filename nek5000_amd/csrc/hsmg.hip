@@ -252,9 +252,12 @@ __global__ void k_sch_out(double *__restrict__ eo, const double *__restrict__ wi
 //   MODE 1 (restriction, hsmg.f:2216-2232): v = s with u weighted by w first;
 //   MODE 2 (prolongation, hsmg.f:1930-1946, with the Schwarz weights of the
 //           finer level): v = v * w + s.
+//   MODE 3 (prolongation into the fine level before gs_fin): v = s on
+//           element-boundary nodes, v = eb * w + s inside.
 template <int NV, int NU, int MODE>
 __global__ void k_tnsr(double *__restrict__ v, const double *__restrict__ u,
-                       const double *__restrict__ w, const double *__restrict__ A, int64_t nel) {
+                       const double *__restrict__ w, const double *__restrict__ A, int64_t nel,
+                       const double *__restrict__ eb) {
   constexpr int NV3 = NV * NV * NV, NU3 = NU * NU * NU;
   __shared__ double sA[NV * NU];
   __shared__ double su[NU3];
@@ -288,8 +291,15 @@ __global__ void k_tnsr(double *__restrict__ v, const double *__restrict__ u,
       double s = 0.0;
       for (int m = 0; m < NU; ++m) s += s2[ij + NV * NV * m] * sA[k + NV * m];
       const int64_t o = (int64_t)e * NV3 + t;
-      if constexpr (MODE == 2) v[o] = v[o] * w[o] + s;
-      else v[o] = s;
+      if constexpr (MODE == 2) {
+        v[o] = v[o] * w[o] + s;
+      } else if constexpr (MODE == 3) {
+        const int i = t % NV, j = (t / NV) % NV, k = t / (NV * NV);
+        const bool bnd = (i == 0 || i == NV - 1 || j == 0 || j == NV - 1 || k == 0 || k == NV - 1);
+        v[o] = bnd ? s : eb[o] * w[o] + s;
+      } else {
+        v[o] = s;
+      }
     }
   }
 }
@@ -369,12 +379,12 @@ int mg_schwarz_local(Mg &mg, int l, const double *r, double *e, hipStream_t s) {
 
 template <int MODE>
 static int run_tnsr(double *v, int nv, const double *u, int nu, const double *w, const double *A,
-                    int64_t nel, hipStream_t s) {
+                    int64_t nel, hipStream_t s, const double *eb = nullptr) {
   const int grid = (int)std::min<int64_t>(nel, 1 << 20);
 #define NK_T(V, U)                                                                       \
   if (nv == V && nu == U) {                                                              \
     hipLaunchKernelGGL((k_tnsr<V, U, MODE>), dim3(grid), dim3(V * V * V > 256 ? 256 : 64), \
-                       0, s, v, u, w, A, nel);                                           \
+                       0, s, v, u, w, A, nel, eb);                                       \
     NK_HIP(hipGetLastError());                                                           \
     return NKH_OK;                                                                       \
   }
@@ -414,9 +424,14 @@ int mg_solve(Mesh &mesh, double *z, const double *r, hipStream_t s) {
   hipStream_t b = mg.sb;
   NK_HIP(hipEventRecord(mg.ev_fork, s));
   NK_HIP(hipStreamWaitEvent(b, mg.ev_fork, 0));
-  // fine level on s: local solves, overlap sum, dssum
-  NK_TRY(mg_schwarz_local(mg, Lm, r, z, s));
-  NK_TRY(gs_op(mg.lev[Lm].gs, z, 0, s));
+  // fine level on s: local solves and overlap sum.  With ids zero exactly
+  // at element interiors, the dssum of the Schwarz sum, the weights, the
+  // coarse correction and dsavg are done together after the join (gs_fin);
+  // otherwise dssum now and the plain sequence after the join.
+  MgLevel &T = mg.lev[Lm];
+  const bool merged = mesh.gs.interior_zero;
+  NK_TRY(mg_schwarz_local(mg, Lm, r, merged ? T.e : z, s));
+  if (!merged) NK_TRY(gs_op(T.gs, z, 0, s));
   // coarser levels on b: restrict, dssum, mask, Schwarz; level 1 solve
   const double *rsrc = r;
   for (int l = Lm - 1; l >= 2; --l) {
@@ -439,10 +454,15 @@ int mg_solve(Mesh &mesh, double *z, const double *r, hipStream_t s) {
   }
   NK_HIP(hipEventRecord(mg.ev_join, b));
   NK_HIP(hipStreamWaitEvent(s, mg.ev_join, 0));
-  // z = z * (mask*wt) + J e_{lmax-1}; dsavg (ic.f:1871-1897)
-  MgLevel &T = mg.lev[Lm], &C = mg.lev[Lm - 1];
-  NK_TRY(run_tnsr<2>(z, T.nh, C.e, C.nh, T.mswt, C.J, nel, s));
-  NK_TRY(gs_op(mesh.gs, z, 4, s));
+  // z = dssum(e)*(mask*wt) + J e_{lmax-1}, then dsavg (ic.f:1871-1897)
+  MgLevel &C = mg.lev[Lm - 1];
+  if (merged) {
+    NK_TRY(run_tnsr<3>(z, T.nh, C.e, C.nh, T.mswt, C.J, nel, s, T.e));
+    NK_TRY(gs_fin(mesh.gs, z, T.e, T.mswt, s));
+  } else {
+    NK_TRY(run_tnsr<2>(z, T.nh, C.e, C.nh, T.mswt, C.J, nel, s));
+    NK_TRY(gs_op(mesh.gs, z, 4, s));
+  }
   return NKH_OK;
 }
 
@@ -501,19 +521,18 @@ __device__ double final_sum(const double *part) {
   for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < (n); \
        i += (int64_t)gridDim.x * blockDim.x)
 
-// r = res; res = 0; r1 = 0; p = 0; partial glsc3(r, wt, r).  r is stored
-// masked with the multigrid's fine mask: the first h1mg_solve masks it in
-// place before any other use (hsmg.f:1878 via h1mg_schwarz).
+// r = res; res = 0; p = 0; partial glsc3(r, wt, r).  r is stored masked
+// with the multigrid's fine mask: the first h1mg_solve masks it in place
+// before any other use (hsmg.f:1878 via h1mg_schwarz).
 __global__ void __launch_bounds__(256)
-    k_fcg_init(double *__restrict__ r, double *__restrict__ res, double *__restrict__ r1,
-               double *__restrict__ p, const double *__restrict__ wt,
-               const double *__restrict__ mgmask, int64_t n, double *part) {
+    k_fcg_init(double *__restrict__ r, double *__restrict__ res, double *__restrict__ p,
+               const double *__restrict__ wt, const double *__restrict__ mgmask, int64_t n,
+               double *part) {
   double a = 0.0;
   NK_GRID_LOOP(i, n) {
     const double ri = res[i];
     r[i] = ri * mgmask[i];
     res[i] = 0.0;
-    r1[i] = 0.0;
     p[i] = 0.0;
     a += (ri * wt[i]) * ri;
   }
@@ -534,18 +553,20 @@ __global__ void __launch_bounds__(256)
   }
 }
 
-// r1 <- r1 - r; partial glsc3(z,wt,r) and glsc3(z,wt,r1); r1 <- r
+// partials of glsc3(z,wt,r) and glsc3(z,wt,w).  The reference keeps the
+// previous residual r1 and forms rho2 = -glsc3(z,wt,r1-r) (hmholtz.f:2236-
+// 2241); r1 - r is the last update alpha*w exactly in exact arithmetic (w
+// still holds that iteration's A p), so rho2 = -alpha*glsc3(z,wt,w) without
+// the extra vector.
 __global__ void __launch_bounds__(256)
     k_fcg_dots(const double *__restrict__ z, const double *__restrict__ r,
-               double *__restrict__ r1, const double *__restrict__ wt, int64_t n, double *pa,
-               double *pb) {
+               const double *__restrict__ w, const double *__restrict__ wt, int64_t n,
+               double *pa, double *pb) {
   double a = 0.0, b = 0.0;
   NK_GRID_LOOP(i, n) {
-    const double ri = r[i], zw = z[i] * wt[i];
-    const double d = r1[i] - ri;
-    a += zw * ri;
-    b += zw * d;
-    r1[i] = ri;
+    const double zw = z[i] * wt[i];
+    a += zw * r[i];
+    b += zw * w[i];
   }
   block_part2(a, b, pa, pb);
 }
@@ -556,7 +577,8 @@ __global__ void __launch_bounds__(256) k_fcg_beta(FcgState *st, const double *pa
   if (threadIdx.x == 0) {
     st->rho0 = st->rho1;
     st->rho1 = a;
-    st->rho2 = -b;
+    // first iteration: r1 = 0, so rho2 = glsc3(z,wt,r); p = 0 then anyway
+    st->rho2 = (st->iter == 0) ? a : -st->alpha * b;
     st->beta = st->rho2 / st->rho0;
   }
 }
@@ -640,13 +662,13 @@ int run_flex_cg(Mesh &m, double *res, Coef h1, Coef h2, const double *wt, const 
   hipStream_t s = cur_stream();
   const int64_t n = m.n;
   if (!mg.fr) {
-    for (double **p : {&mg.fr, &mg.fr1, &mg.fp, &mg.fz, &mg.fw}) NK_TRY(mg_dalloc(p, n));
+    for (double **p : {&mg.fr, &mg.fp, &mg.fz, &mg.fw}) NK_TRY(mg_dalloc(p, n));
     NK_TRY(mg_dalloc(&mg.fpart, 3 * kFcgBlocks));
     NK_HIP(hipMalloc(&mg.fst, sizeof(FcgState)));
     NK_HIP(hipMalloc(&mg.fflag, sizeof(int)));
   }
   const double *mgmask = mg.lev[mg.lmax].mask;
-  double *r = mg.fr, *r1 = mg.fr1, *p = mg.fp, *z = mg.fz, *w = mg.fw;
+  double *r = mg.fr, *p = mg.fp, *z = mg.fz, *w = mg.fw;
   double *pa = mg.fpart, *pb = mg.fpart + kFcgBlocks;
   // hmholtz.f:2205-2208: tolps from chktcg1 (pmask, vmult), capped by param(21)
   double tolps = 0.0;
@@ -669,7 +691,7 @@ int run_flex_cg(Mesh &m, double *res, Coef h1, Coef h2, const double *wt, const 
   if (!differ) NK_TRY(fused_conditions(m, mask, wt, nullptr, &fused, s));
   double *rho_part = m.red + kRedRho1;
   const int nrho = axhelm_cg_grid(m);
-  hipLaunchKernelGGL(k_fcg_init, g, b, 0, s, r, res, r1, p, wt, mgmask, n, pa);
+  hipLaunchKernelGGL(k_fcg_init, g, b, 0, s, r, res, p, wt, mgmask, n, pa);
   hipLaunchKernelGGL(k_fcg_start, dim3(1), b, 0, s, mg.fst, pa, m.vol, prm.param21, tolps);
   NK_HIP(hipGetLastError());
   Timers &tm = timers();
@@ -682,7 +704,7 @@ int run_flex_cg(Mesh &m, double *res, Coef h1, Coef h2, const double *wt, const 
   for (int k = 1; k <= maxit; ++k) {
     if (differ && (rc = mul_inplace(r, mgmask, n, s)) != NKH_OK) break;
     if ((rc = mg_solve(m, z, r, s)) != NKH_OK) break;
-    hipLaunchKernelGGL(k_fcg_dots, g, b, 0, s, z, r, r1, wt, n, pa, pb);
+    hipLaunchKernelGGL(k_fcg_dots, g, b, 0, s, z, r, w, wt, n, pa, pb);
     hipLaunchKernelGGL(k_fcg_beta, dim3(1), b, 0, s, mg.fst, pa, pb);
     if (fused) {
       if (tm.on) (void)hipEventRecord(ev[0], s);

@@ -466,7 +466,7 @@ void mg_free(Mesh &mesh) {
   for (int l = 1; l <= mg->lmax; ++l) free_level(mg->lev[l]);
   mg->crs.release();
   for (void *p : {(void *)mg->fbc, (void *)mg->slot, (void *)mg->goff, (void *)mg->gidx,
-                  (void *)mg->cb, (void *)mg->cx, (void *)mg->fr, (void *)mg->fr1, (void *)mg->fp,
+                  (void *)mg->cb, (void *)mg->cx, (void *)mg->fr, (void *)mg->fp,
                   (void *)mg->fz, (void *)mg->fw, (void *)mg->fpart, (void *)mg->fst,
                   (void *)mg->fflag})
     if (p) (void)hipFree(p);

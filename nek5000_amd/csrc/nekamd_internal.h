@@ -170,6 +170,9 @@ int gs_op(const GsMap &gs, double *u, int op, hipStream_t s);
 // CG assembly: w <- mask * dssum(w) using the flags of gs_prepare; skipped
 // when st is given and st->done.
 int gs_assemble(const GsMap &gs, double *w, const CgState *st, hipStream_t s);
+// Per id group (single members included): S = sum of e; z_i <- mean over
+// the members of S*w_i + z_i.  The fine-level end of the multigrid V-cycle.
+int gs_fin(const GsMap &gs, double *z, const double *e, const double *w, hipStream_t s);
 // Flags per group from mask/mult; *ok when every group has a 0/1 mask equal
 // at all members and mult summing to 1 (red: kRedBlocks scratch).
 int gs_prepare(GsMap &gs, const double *mask, const double *mult, double *red, bool *ok,
