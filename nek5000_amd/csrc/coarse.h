@@ -27,7 +27,9 @@ struct CsrDev {
 
 struct AmgLevel {
   int n = 0;
-  EllDev A;              // level operator
+  bool wide = false;     // operator applied from CSR, one wavefront per row
+  EllDev A;              // level operator (thread per row)
+  CsrDev Aw;             // level operator (wavefront per row), when wide
   EllDev P;              // prolongation to this level from the next (n x n_next)
   CsrDev R;              // restriction P^T (n_next x n), one wavefront per row
   double *wd = nullptr;  // omega / diag(A)

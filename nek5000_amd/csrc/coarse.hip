@@ -367,6 +367,46 @@ __global__ void __launch_bounds__(256)
   if (part) part_store(a, part);
 }
 
+// the same two smoothing steps with one wavefront per row of a CSR operator
+// (levels whose rows are long and few)
+__global__ void __launch_bounds__(256)
+    k_amg_pre_w(const int32_t *__restrict__ rp, const int32_t *__restrict__ col,
+                const double *__restrict__ val, int n, const double *__restrict__ wd,
+                const double *__restrict__ b, double *__restrict__ x, double *__restrict__ r,
+                const CrsPcg *st) {
+  if (st->done) return;
+  const int lane = threadIdx.x & 63;
+  for (int i = (blockIdx.x * blockDim.x + threadIdx.x) >> 6; i < n;
+       i += (gridDim.x * blockDim.x) >> 6) {
+    double s = 0.0;
+    for (int q = rp[i] + lane; q < rp[i + 1]; q += 64) {
+      const int j = col[q];
+      s += val[q] * (wd[j] * b[j]);
+    }
+    for (int o = 32; o > 0; o >>= 1) s += __shfl_down(s, o, 64);
+    if (lane == 0) {
+      x[i] = wd[i] * b[i];
+      r[i] = b[i] - s;
+    }
+  }
+}
+
+__global__ void __launch_bounds__(256)
+    k_amg_post_w(const int32_t *__restrict__ rp, const int32_t *__restrict__ col,
+                 const double *__restrict__ val, int n, const double *__restrict__ wd,
+                 const double *__restrict__ b, const double *__restrict__ x,
+                 double *__restrict__ y, const CrsPcg *st) {
+  if (st->done) return;
+  const int lane = threadIdx.x & 63;
+  for (int i = (blockIdx.x * blockDim.x + threadIdx.x) >> 6; i < n;
+       i += (gridDim.x * blockDim.x) >> 6) {
+    double s = 0.0;
+    for (int q = rp[i] + lane; q < rp[i + 1]; q += 64) s += val[q] * x[col[q]];
+    for (int o = 32; o > 0; o >>= 1) s += __shfl_down(s, o, 64);
+    if (lane == 0) y[i] = x[i] + wd[i] * (b[i] - s);
+  }
+}
+
 // v = M u (ADD: v += M u)
 template <bool ADD>
 __global__ void __launch_bounds__(256)
@@ -576,7 +616,10 @@ int CoarseSolver::setup(const std::vector<int32_t> &rowptr, const std::vector<in
     AmgLevel &L = lev[k];
     const Csr &Ak = As[k];
     L.n = Ak.n;
-    NK_TRY(to_ell(Ak, L.A));
+    // rows long and few (Galerkin levels): one wavefront per row
+    L.wide = k > 0 && Ak.ci.size() > (size_t)24 * Ak.n;
+    if (L.wide) NK_TRY(to_csr(Ak, L.Aw));
+    else NK_TRY(to_ell(Ak, L.A));
     NK_TRY(to_ell(Ps[k], L.P));
     NK_TRY(to_csr(transpose(Ps[k]), L.R));
     std::vector<double> d(Ak.n);
@@ -603,6 +646,7 @@ int CoarseSolver::setup(const std::vector<int32_t> &rowptr, const std::vector<in
 void CoarseSolver::release() {
   for (AmgLevel &L : lev) {
     free_ell(L.A);
+    free_csr(L.Aw);
     free_ell(L.P);
     free_csr(L.R);
     for (double *q : {L.wd, L.b, L.x, L.y, L.r})
@@ -634,8 +678,12 @@ int CoarseSolver::solve(double *xout, const double *b, hipStream_t s) {
     const double *bk = rhs;
     for (int k = 0; k < nlevels_amg; ++k) {
       AmgLevel &L = lev[k];
-      hipLaunchKernelGGL(k_amg_pre, dim3(nblocks(L.n)), B, 0, s, L.A.col, L.A.val, L.A.width,
-                         L.n, L.wd, bk, L.x, L.r, st);
+      if (L.wide)
+        hipLaunchKernelGGL(k_amg_pre_w, dim3(wblocks(L.n)), B, 0, s, L.Aw.rp, L.Aw.col, L.Aw.val,
+                           L.n, L.wd, bk, L.x, L.r, st);
+      else
+        hipLaunchKernelGGL(k_amg_pre, dim3(nblocks(L.n)), B, 0, s, L.A.col, L.A.val, L.A.width,
+                           L.n, L.wd, bk, L.x, L.r, st);
       double *bnext = (k + 1 < nlevels_amg) ? lev[k + 1].b : bd;
       const int nnext = (k + 1 < nlevels_amg) ? lev[k + 1].n : ndense;
       hipLaunchKernelGGL(k_csr_apply, dim3(wblocks(nnext)), B, 0, s, L.R.rp, L.R.col, L.R.val,
@@ -652,8 +700,12 @@ int CoarseSolver::solve(double *xout, const double *b, hipStream_t s) {
                          xc, L.x, st);
       const double *bk2 = (k == 0) ? rhs : L.b;
       double *yk = (k == 0) ? out : L.y;
-      hipLaunchKernelGGL(k_amg_post, dim3(nbk), B, 0, s, L.A.col, L.A.val, L.A.width, L.n, L.wd,
-                         bk2, L.x, yk, st, (k == 0) ? part_out : nullptr);
+      if (L.wide)
+        hipLaunchKernelGGL(k_amg_post_w, dim3(wblocks(L.n)), B, 0, s, L.Aw.rp, L.Aw.col,
+                           L.Aw.val, L.n, L.wd, bk2, L.x, yk, st);
+      else
+        hipLaunchKernelGGL(k_amg_post, dim3(nbk), B, 0, s, L.A.col, L.A.val, L.A.width, L.n,
+                           L.wd, bk2, L.x, yk, st, (k == 0) ? part_out : nullptr);
       xc = yk;
     }
   };

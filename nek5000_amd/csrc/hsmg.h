@@ -51,7 +51,9 @@ struct Mg {
   MgLevel lev[4];
   int *fbc = nullptr;  // device, 6 per element
   hipStream_t sb = nullptr;  // second stream: coarse levels beside the fine one
+  hipStream_t sc = nullptr;  // third: intermediate-level Schwarz beside the coarse solve
   hipEvent_t ev_fork = nullptr, ev_join = nullptr;
+  hipEvent_t ev_lr[4] = {}, ev_le[4] = {};  // per level: r ready, e ready
   // coarse problem (navier8.f:83-233): unknowns = free vertex ids
   int ncrs = 0;
   int32_t *slot = nullptr;                   // per level-1 node: coarse unknown or -1

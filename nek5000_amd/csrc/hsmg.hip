@@ -20,6 +20,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <type_traits>
 #include <vector>
 
 #include "hsmg.h"
@@ -30,12 +31,16 @@ namespace {
 
 // odd LDS row pitch for the (n+2)^3 block, so that pencils along r are
 // read without bank conflicts
+// Launch shape of the local FDM solves: each thread owns PPT pencils of NL
+// points per contraction (so each S value read from LDS feeds PPT FMAs); P
+// is an odd LDS row pitch so that pencils along r are read without bank
+// conflicts.
 template <int NL>
 struct Fdm {
   static constexpr int P = NL | 1;
-  static constexpr int kThreads = ((NL * NL + 63) / 64) * 64;
-  // waves per SIMD the register budget is sized for
-  static constexpr int kWaves = NL <= 7 ? 8 : (NL <= 10 ? 6 : (NL <= 14 ? 4 : 3));
+  static constexpr int PPT = NL >= 8 ? 2 : 1;
+  static constexpr int kActive = (NL * NL + PPT - 1) / PPT;
+  static constexpr int kThreads = ((kActive + 63) / 64) * 64;
 };
 
 // Local Schwarz solves on level with nh = NL-2 points (hsmg.f:897-929
@@ -45,12 +50,12 @@ struct Fdm {
 // and corners 0 -- what hsmg.f:455-467 leaves in the extended array.
 // D(i,j,k) = 1/(lr_i + ls_j + lt_k) where that sum exceeds eps, else 0
 // (hsmg.f:753-770), formed from the per-element eigenvalues.  One element
-// per block; thread = one pencil of NL points along the direction being
-// contracted, held in registers.  The element's S and eigenvalues are staged
-// in LDS with the input (one coalesced load phase) and read from there as
-// wave-uniform operands.
-// Output: the interior of w, and each ghost face value of w sent to the
-// neighbour's slot it coincides with (gout), where the overlap sum reads it.
+// per block.  A contraction along one direction gives each thread whole
+// pencils along that direction in registers; the element's S and
+// eigenvalues are staged in LDS with the input (one coalesced load phase)
+// and read there as wave-uniform operands.  Output: the interior of w, and
+// each ghost face value of w sent to the neighbour's slot it coincides with
+// (gout), where the overlap sum reads it.
 template <int NL>
 __global__ void __launch_bounds__(Fdm<NL>::kThreads)
     k_sch_fdm(double *__restrict__ wint, double *__restrict__ wgh, const double *__restrict__ r,
@@ -58,12 +63,11 @@ __global__ void __launch_bounds__(Fdm<NL>::kThreads)
               const double *__restrict__ S, const double *__restrict__ lam, int64_t nel) {
   constexpr int N = NL - 2, N2 = N * N, N3 = N2 * N, M2 = NL * NL, NLAM = 3 * NL + 1;
   constexpr int P = Fdm<NL>::P, PL = P * NL, T = Fdm<NL>::kThreads;
+  constexpr int PPT = Fdm<NL>::PPT, NA = Fdm<NL>::kActive;
   __shared__ double u[PL * NL];
   __shared__ double sS[3 * M2];
   __shared__ double sl[NLAM];
   const int tid = threadIdx.x;
-  const bool act = tid < M2;
-  const int pa = tid % NL, pb = tid / NL;
   // ghost face slot (f,a,b) -> LDS position of the extended array
   auto gpos = [](int q) {
     const int f = q / N2, a = q % N, b = (q / N) % N;
@@ -78,6 +82,54 @@ __global__ void __launch_bounds__(Fdm<NL>::kThreads)
     }
     return i + P * j + PL * k;
   };
+  // pencils of this thread: p = tid + q*NA (q < PPT), valid when p < M2;
+  // (pa, pb) = the two coordinates that fix the pencil
+  int pa[PPT], pb[PPT];
+  bool pv[PPT];
+#pragma unroll
+  for (int q = 0; q < PPT; ++q) {
+    const int p = tid + q * NA;
+    pv[q] = (tid < NA) && (p < M2);
+    pa[q] = pv[q] ? p % NL : 0;
+    pb[q] = pv[q] ? p / NL : 0;
+  }
+  // contraction of every pencil along direction D (0: r, 1: s, 2: t) with
+  // the matrix Sm: y_i = sum_m Sm(m,i) x_m when TR, else Sm(i,m) x_m
+  auto contract = [&](auto dir_tag, auto tr_tag, const double *Sm) {
+    constexpr int D = decltype(dir_tag)::value;
+    constexpr bool TR = decltype(tr_tag)::value;
+    constexpr int st = D == 0 ? 1 : (D == 1 ? P : PL);
+    double x[PPT][NL];
+    int base[PPT];
+#pragma unroll
+    for (int q = 0; q < PPT; ++q) {
+      base[q] = D == 0 ? P * pa[q] + PL * pb[q] : (D == 1 ? pa[q] + PL * pb[q] : pa[q] + P * pb[q]);
+      if (pv[q]) {
+#pragma unroll
+        for (int m = 0; m < NL; ++m) x[q][m] = u[base[q] + st * m];
+      }
+    }
+#pragma unroll 2
+    for (int i = 0; i < NL; ++i) {
+      double acc[PPT];
+#pragma unroll
+      for (int q = 0; q < PPT; ++q) acc[q] = 0.0;
+#pragma unroll
+      for (int m = 0; m < NL; ++m) {
+        const double sv = TR ? Sm[m + NL * i] : Sm[i + NL * m];
+#pragma unroll
+        for (int q = 0; q < PPT; ++q) acc[q] += sv * x[q][m];
+      }
+#pragma unroll
+      for (int q = 0; q < PPT; ++q)
+        if (pv[q]) u[base[q] + st * i] = acc[q];
+    }
+  };
+  using R_ = std::integral_constant<int, 0>;
+  using S_ = std::integral_constant<int, 1>;
+  using T_ = std::integral_constant<int, 2>;
+  using TR_ = std::true_type;
+  using NT_ = std::false_type;
   // element order: consecutive blocks go to different XCDs (round-robin), so
   // block b takes the elements of chunk b % 8, and face neighbours -- which
   // read each other's data -- are processed by one XCD around the same time
@@ -119,87 +171,29 @@ __global__ void __launch_bounds__(Fdm<NL>::kThreads)
       if (g >= 2) u[i + P * j + PL * k] = 0.0;
     }
     __syncthreads();
-    const double *Sr = sS, *Ss = sS + M2, *St = sS + 2 * M2;
-    const double *le = sl;
-    // each thread owns one pencil per phase: it reads the pencil into
-    // registers and overwrites it in place, so no other thread is involved
-    // until the barrier
-    double x[NL];
-    // r (transposed), pencil (j,k)
-    if (act) {
-#pragma unroll
-      for (int m = 0; m < NL; ++m) x[m] = u[m + P * pa + PL * pb];
-#pragma unroll 2
-      for (int i = 0; i < NL; ++i) {
-        double acc = 0.0;
-#pragma unroll
-        for (int m = 0; m < NL; ++m) acc += Sr[m + NL * i] * x[m];
-        u[i + P * pa + PL * pb] = acc;
-      }
-    }
+    contract(R_(), TR_(), sS);
     __syncthreads();
-    // s (transposed), pencil (i,k)
-    if (act) {
-#pragma unroll
-      for (int m = 0; m < NL; ++m) x[m] = u[pa + P * m + PL * pb];
-#pragma unroll 2
-      for (int j = 0; j < NL; ++j) {
-        double acc = 0.0;
-#pragma unroll
-        for (int m = 0; m < NL; ++m) acc += Ss[m + NL * j] * x[m];
-        u[pa + P * j + PL * pb] = acc;
-      }
-    }
+    contract(S_(), TR_(), sS + M2);
     __syncthreads();
-    // t (transposed) and the diagonal, then t; pencil (i,j)
-    if (act) {
+    contract(T_(), TR_(), sS + 2 * M2);
+    // the diagonal, on the pencils just written by this thread
 #pragma unroll
-      for (int m = 0; m < NL; ++m) x[m] = u[pa + P * pb + PL * m];
-      const double lij = le[pa] + le[NL + pb], eps = le[3 * NL];
-#pragma unroll 2
+    for (int q = 0; q < PPT; ++q) {
+      if (!pv[q]) continue;
+      const double lij = sl[pa[q]] + sl[NL + pb[q]], eps = sl[3 * NL];
+      const int base = pa[q] + P * pb[q];
+#pragma unroll
       for (int k = 0; k < NL; ++k) {
-        double acc = 0.0;
-#pragma unroll
-        for (int m = 0; m < NL; ++m) acc += St[m + NL * k] * x[m];
-        const double diag = lij + le[2 * NL + k];
-        u[pa + P * pb + PL * k] = (diag > eps) ? acc * (1.0 / diag) : 0.0;
-      }
-#pragma unroll
-      for (int m = 0; m < NL; ++m) x[m] = u[pa + P * pb + PL * m];
-#pragma unroll 2
-      for (int k = 0; k < NL; ++k) {
-        double acc = 0.0;
-#pragma unroll
-        for (int m = 0; m < NL; ++m) acc += St[k + NL * m] * x[m];
-        u[pa + P * pb + PL * k] = acc;
+        const double diag = lij + sl[2 * NL + k];
+        const double v = u[base + PL * k];
+        u[base + PL * k] = (diag > eps) ? v * (1.0 / diag) : 0.0;
       }
     }
+    contract(T_(), NT_(), sS + 2 * M2);
     __syncthreads();
-    // s, pencil (i,k)
-    if (act) {
-#pragma unroll
-      for (int m = 0; m < NL; ++m) x[m] = u[pa + P * m + PL * pb];
-#pragma unroll 2
-      for (int j = 0; j < NL; ++j) {
-        double acc = 0.0;
-#pragma unroll
-        for (int m = 0; m < NL; ++m) acc += Ss[j + NL * m] * x[m];
-        u[pa + P * j + PL * pb] = acc;
-      }
-    }
+    contract(S_(), NT_(), sS + M2);
     __syncthreads();
-    // r, pencil (j,k)
-    if (act) {
-#pragma unroll
-      for (int m = 0; m < NL; ++m) x[m] = u[m + P * pa + PL * pb];
-#pragma unroll 2
-      for (int i = 0; i < NL; ++i) {
-        double acc = 0.0;
-#pragma unroll
-        for (int m = 0; m < NL; ++m) acc += Sr[i + NL * m] * x[m];
-        u[i + P * pa + PL * pb] = acc;
-      }
-    }
+    contract(R_(), NT_(), sS);
     __syncthreads();
     double *wi = wint + e * N3;
 #pragma unroll
@@ -432,15 +426,21 @@ int mg_solve(Mesh &mesh, double *z, const double *r, hipStream_t s) {
   const bool merged = mesh.gs.interior_zero;
   NK_TRY(mg_schwarz_local(mg, Lm, r, merged ? T.e : z, s));
   if (!merged) NK_TRY(gs_op(T.gs, z, 0, s));
-  // coarser levels on b: restrict, dssum, mask, Schwarz; level 1 solve
+  // coarser levels on b: restrict, dssum, mask; the Schwarz solve of each
+  // intermediate level on c (it only reads the masked residual), beside the
+  // restriction to level 1 and the coarse solve on b
+  hipStream_t c = mg.sc;
   const double *rsrc = r;
   for (int l = Lm - 1; l >= 2; --l) {
     MgLevel &C = mg.lev[l], &F = mg.lev[l + 1];
     NK_TRY(run_tnsr<1>(C.r, C.nh, rsrc, F.nh, F.rwt, C.Jt, nel, b));
     NK_TRY(gs_op(C.gs, C.r, 0, b));
     NK_TRY(mul_inplace(C.r, C.mask, C.n, b));
-    NK_TRY(mg_schwarz_local(mg, l, C.r, C.e, b));
-    NK_TRY(gs_op(C.gs, C.e, 0, b));
+    NK_HIP(hipEventRecord(mg.ev_lr[l], b));
+    NK_HIP(hipStreamWaitEvent(c, mg.ev_lr[l], 0));
+    NK_TRY(mg_schwarz_local(mg, l, C.r, C.e, c));
+    NK_TRY(gs_op(C.gs, C.e, 0, c));
+    NK_HIP(hipEventRecord(mg.ev_le[l], c));
     rsrc = C.r;
   }
   MgLevel &L1 = mg.lev[1], &L2 = mg.lev[2];
@@ -450,6 +450,7 @@ int mg_solve(Mesh &mesh, double *z, const double *r, hipStream_t s) {
   NK_TRY(mul_inplace(L1.e, L1.mask, L1.n, b));
   for (int l = 2; l < Lm; ++l) {  // e_l = e_l * (mask*wt) + J e_{l-1}
     MgLevel &C = mg.lev[l - 1], &F = mg.lev[l];
+    NK_HIP(hipStreamWaitEvent(b, mg.ev_le[l], 0));
     NK_TRY(run_tnsr<2>(F.e, F.nh, C.e, C.nh, F.mswt, C.J, nel, b));
   }
   NK_HIP(hipEventRecord(mg.ev_join, b));

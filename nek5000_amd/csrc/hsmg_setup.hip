@@ -463,6 +463,7 @@ void mg_free(Mesh &mesh) {
   Mg *mg = mesh.mg;
   if (!mg) return;
   if (mg->sb) (void)hipStreamSynchronize(mg->sb);
+  if (mg->sc) (void)hipStreamSynchronize(mg->sc);
   for (int l = 1; l <= mg->lmax; ++l) free_level(mg->lev[l]);
   mg->crs.release();
   for (void *p : {(void *)mg->fbc, (void *)mg->slot, (void *)mg->goff, (void *)mg->gidx,
@@ -472,7 +473,12 @@ void mg_free(Mesh &mesh) {
     if (p) (void)hipFree(p);
   if (mg->ev_fork) (void)hipEventDestroy(mg->ev_fork);
   if (mg->ev_join) (void)hipEventDestroy(mg->ev_join);
+  for (int l = 0; l < 4; ++l) {
+    if (mg->ev_lr[l]) (void)hipEventDestroy(mg->ev_lr[l]);
+    if (mg->ev_le[l]) (void)hipEventDestroy(mg->ev_le[l]);
+  }
   if (mg->sb) (void)hipStreamDestroy(mg->sb);
+  if (mg->sc) (void)hipStreamDestroy(mg->sc);
   delete mg;
   mesh.mg = nullptr;
 }
@@ -497,6 +503,11 @@ int mg_setup(Mesh &mesh, const int *fbc_h, const int64_t *vertex, const double *
   const int64_t nel = m->nel;
   mg.nel = nel;
   NK_HIP(hipStreamCreateWithFlags(&mg.sb, hipStreamNonBlocking));
+  NK_HIP(hipStreamCreateWithFlags(&mg.sc, hipStreamNonBlocking));
+  for (int l = 0; l < 4; ++l) {
+    NK_HIP(hipEventCreateWithFlags(&mg.ev_lr[l], hipEventDisableTiming));
+    NK_HIP(hipEventCreateWithFlags(&mg.ev_le[l], hipEventDisableTiming));
+  }
   NK_HIP(hipEventCreateWithFlags(&mg.ev_fork, hipEventDisableTiming));
   NK_HIP(hipEventCreateWithFlags(&mg.ev_join, hipEventDisableTiming));
   int nx[4] = {0, 0, 0, 0};
