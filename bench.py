@@ -56,6 +56,21 @@ def fused_axhelm_bytes_per_node(lx1, h1_field=False, h2_kind="uniform"):
     return 8.0 * words
 
 
+def measured_traffic(name):
+    """HBM bytes per launch of the roofline kernel from the committed
+    rocprofv3 FETCH_SIZE/WRITE_SIZE summary of this workload (scaled by the
+    calibration factors stated there), or None."""
+    path = os.path.join(ROOT, "profiles", name)
+    try:
+        with open(path) as fh:
+            for line in fh:
+                if "traffic_bytes_per_launch:" in line:
+                    return float(line.split(":")[1])
+    except OSError:
+        return None
+    return None
+
+
 def cpu_baseline(nel1d, lx1, deform, target_seconds, solver="hsmg"):
     """The oracle's solve of the same operator on E=nel1d^3, bounded by maxit
     so that the timed part takes about target_seconds: hmholtz -> cggo
@@ -108,6 +123,11 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--maxit", type=int, default=900,
                     help="iteration cap per solve (900 = the reference's maxcg)")
+    ap.add_argument("--crs", type=int, default=0,
+                    help="coarse solver of the V-cycle: 0 solved to 1e-13 (the reference's "
+                         "exact XXT), 1 one AMG V-cycle (its param(40)=1)")
+    ap.add_argument("--no-variants", action="store_true",
+                    help="skip the extra short run of the other coarse solver")
     ap.add_argument("--solver", choices=("hsmg", "jacobi"), default="hsmg",
                     help="hsmg: hmholtz('PRES') -> hmh_flex_cg with h1mg (param42=2); "
                          "jacobi: hmholtz -> cggo")
@@ -146,12 +166,16 @@ def main():
     free_dof = float(torch.sum(F.mask * F.mult))
     log(f"[bench] free unique DOF = {free_dof:.0f}")
 
-    prm = solver.default_params(param42=2, param21=-args.tol) if hsmg else None
+    def params_for(crs):
+        return (solver.default_params(param42=2, param21=-args.tol, param40=crs) if hsmg
+                else None)
 
-    def solve(r):
+    prm = params_for(args.crs)
+
+    def solve(r, p=None):
         u = torch.zeros_like(rhs0)
         return F.hmholtz("PRES" if hsmg else "TEMP", u, r, h1, h2, F.mask, F.mult, -args.tol,
-                         args.maxit, params=prm)
+                         args.maxit, params=p if p is not None else prm)
 
     # plain copy of 2 GiB by torch: the achievable-bandwidth reference for
     # the roofline numbers (and the calibration kernel of the PMC byte counts)
@@ -205,6 +229,25 @@ def main():
     total_iters = sum(iters)
     value = free_dof * total_iters / elapsed / 1e9
     mg_info = F.h1mg_info() if hsmg else None
+    variant = None
+    if hsmg and not args.no_variants:
+        # the other coarse solver, two solves after one warm-up (reported
+        # beside the main line, not as its value)
+        other = 1 - args.crs
+        solve(rhs0.clone(), params_for(other))
+        rv = [rhs0.clone() for _ in range(2)]
+        torch.cuda.synchronize()
+        tv = time.perf_counter()
+        it_v = [solve(rv[k], params_for(other))["niter"] for k in range(2)]
+        torch.cuda.synchronize()
+        el_v = time.perf_counter() - tv
+        variant = {"coarse_solver": "one AMG V-cycle" if other == 1 else "solved to 1e-13",
+                   "iterations_per_solve": it_v, "seconds_per_solve": el_v / 2,
+                   "value": free_dof * sum(it_v) / el_v / 1e9}
+        log(f"[bench] variant {variant}")
+    traffic = None
+    if hsmg and args.nel == 64 and args.lx1 == 8:
+        traffic = measured_traffic("axhelm_flexcg_E262144_N7_pmc.md")
 
     cpu = None
     if not args.no_cpu:
@@ -234,6 +277,10 @@ def main():
             "nel": box.nel, "lx1": args.lx1,
             "preconditioner": "h1mg" if hsmg else "jacobi", "precision": "fp64",
             "multigrid": mg_info,
+            "coarse_solver": (None if not hsmg else
+                              ("one AMG V-cycle (param40=1)" if args.crs == 1 else
+                               "AMG-preconditioned CG to 1e-13 (exact, as XXT)")),
+            "variant": variant,
             "seconds_per_solve": elapsed / args.steps,
             "free_dof": free_dof, "iterations_per_solve": iters,
             "axhelm_ms": t_ax * 1e3, "gs_ms": t_gs * 1e3,
@@ -242,7 +289,7 @@ def main():
         },
         "roofline": {
             "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+            "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
             "kernel": ax_kernel,
             "bytes_per_launch": ax_bytes,
         },

@@ -43,6 +43,9 @@ typedef struct {
   int precision;   /* 0: FP64 (the drop-in); 1: FP32 operator inside FP64 refinement */
   int param42;     /* param(42): PRES solver, 2 -> hmh_flex_cg with h1mg (hmholtz.f:646-653) */
   double param21;  /* param(21): PRES tolerance cap (>0) or relative tolerance (<0) (hmholtz.f:2206-2220) */
+  int param40;     /* param(40): coarse solver of h1mg, 0 (XXT) -> solved to 1e-13, 1 (AMG) -> one
+                      AMG V-cycle (navier8.f:209-217, crs_amg.c:129-) */
+  int reserved;
 } nkh_params;
 
 /* What the reference leaves behind after a solve. */
@@ -114,12 +117,12 @@ int nkd_dsop(int mesh, double *u, char op);
 /* hmholtz.f:380-524 SETPREC: inverse of the assembled Jacobi diagonal. */
 int nkd_setprec(int mesh, double *d, const double *h1, const double *h2);
 /* hmholtz.f:2-69 HMHOLTZ: dssum(rhs); rhs*=mask; chktcg1; cggo.  name
-   "PRES" with prm->param42 == 2 takes cggo's branch to hmh_flex_cg
-   (hmholtz.f:646-653; needs nkh_mg_setup); other PRES solvers (GMRES,
-   param42 0) return NKH_ERR_UNSUPPORTED.  rhs is modified in place as in
+   "PRES" takes cggo's branches to hmh_gmres (prm->param42 == 0, the
+   reference's default) or hmh_flex_cg (param42 == 2) (hmholtz.f:639-653;
+   both need nkh_mg_setup); param42 == 1 returns NKH_ERR_UNSUPPORTED.  rhs is modified in place as in
    the reference; u receives the solution.  history (optional) receives per
    Jacobi-PCG iteration (rbn2, rtz1, rho, alpha, beta), 5*min(maxit,900)
-   doubles, or per flexible-CG iteration the residual norm, maxit doubles. */
+   doubles, or for PRES the residual norm per iteration, maxit doubles. */
 int nkd_hmholtz(int mesh, const char *name, double *u, double *rhs, const double *h1,
                 const double *h2, const double *mask, const double *mult, double tli,
                 int maxit, int isd, const nkh_params *prm, nkh_result *res,
@@ -146,6 +149,13 @@ int nkd_mg_solve(int mesh, double *z, double *rhs);
 int nkd_hmh_flex_cg(int mesh, double *res, const double *h1, const double *h2,
                     const double *wt, const double *pmask, int maxit, const nkh_params *prm,
                     nkh_result *res_out, double *rnorm_history);
+/* gmres.f:304-545 HMH_GMRES (right-preconditioned, restarted after 30 =
+   lgmres steps): res holds the rhs on entry and the solution on return;
+   tolerance and arguments as nkd_hmh_flex_cg.  rnorm_history (optional,
+   maxit doubles) receives |gamma_{j+1}|/sqrt(vol) per Arnoldi step. */
+int nkd_hmh_gmres(int mesh, double *res, const double *h1, const double *h2, const double *wt,
+                  const double *pmask, int maxit, const nkh_params *prm, nkh_result *res_out,
+                  double *rnorm_history);
 /* Hierarchy facts: number of levels, points per direction per level (nh[0]
    = level 1), coarse unknowns, mean coarse CG iterations per solve so far. */
 int nkh_mg_info(int mesh, int *lmax, int *nh, int *ncrs, double *crs_avg_iters);

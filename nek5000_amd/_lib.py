@@ -27,6 +27,8 @@ class NkhParams(ctypes.Structure):
         ("precision", c_int),
         ("param42", c_int),
         ("param21", c_double),
+        ("param40", c_int),
+        ("reserved", c_int),
     ]
 
 
@@ -72,6 +74,8 @@ SIGNATURES = {
     "nkd_mg_solve": (c_int, [c_int, c_void_p, c_void_p]),
     "nkd_hmh_flex_cg": (c_int, [c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_int,
                                 c_void_p, c_void_p, c_void_p]),
+    "nkd_hmh_gmres": (c_int, [c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_int,
+                              c_void_p, c_void_p, c_void_p]),
     "nkh_mg_info": (c_int, [c_int, P(c_int), c_void_p, P(c_int), P(c_double)]),
     "nkh_timers_enable": (c_int, [c_int]),
     "nkh_timers_reset": (c_int, []),

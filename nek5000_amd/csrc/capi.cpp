@@ -764,7 +764,7 @@ int nkd_setprec(int id, double *d, const double *h1, const double *h2) {
 
 // Nek's defaults (reader_par.f:26-27,76-79): param(22) = restol = 1e-7,
 // param(42) = 0, param(21) = 0.
-static const nkh_params kDefaultParams = {1, 0, 1.0e-7, 1.0e-7, 0, 0, 0.0};
+static const nkh_params kDefaultParams = {1, 0, 1.0e-7, 1.0e-7, 0, 0, 0.0, 0, 0};
 
 int nkd_cggo(int id, double *x, const double *f, const double *h1, const double *h2,
              const double *mask, const double *mult, double tin, int maxit,
@@ -790,8 +790,8 @@ int nkd_hmholtz(int id, const char *name, double *u, double *rhs, const double *
   const nkh_params &p = prm ? *prm : kDefaultParams;
   NK_CHECK(p.precision == 0, NKH_ERR_UNSUPPORTED, "nkd_hmholtz: precision=%d", p.precision);
   const bool pres = name && std::strncmp(name, "PRES", 4) == 0;
-  NK_CHECK(!pres || p.param42 == 2, NKH_ERR_UNSUPPORTED,
-           "nkd_hmholtz: PRES with param42=%d (only 2, flexible CG with h1mg, is built)",
+  NK_CHECK(!pres || p.param42 == 2 || p.param42 == 0, NKH_ERR_UNSUPPORTED,
+           "nkd_hmholtz: PRES with param42=%d (0 GMRES and 2 flexible CG with h1mg are built)",
            p.param42);
   hipStream_t s = g_stream;
   Coef c1, c2;
@@ -804,6 +804,7 @@ int nkd_hmholtz(int id, const char *name, double *u, double *rhs, const double *
     // flexible-CG branch (hmholtz.f:646-653) does not read: x = f, then
     // hmh_flex_cg(x, h1, h2, mult, maxit)
     NK_HIP(hipMemcpyAsync(u, rhs, sizeof(double) * m->n, hipMemcpyDeviceToDevice, s));
+    if (p.param42 == 0) return run_gmres(*m, u, c1, c2, mult, mask, maxit, p, res, history);
     return run_flex_cg(*m, u, c1, c2, mult, mask, maxit, p, res, history);
   }
   double tol = std::fabs(tli);
@@ -840,6 +841,20 @@ int nkd_hmh_flex_cg(int id, double *res, const double *h1, const double *h2, con
   Coef c1, c2;
   NK_TRY(classify_coefs(*m, h1, h2, &c1, &c2, g_stream));
   return run_flex_cg(*m, res, c1, c2, wt, pmask, maxit, p, out, rnorm_history);
+}
+
+int nkd_hmh_gmres(int id, double *res, const double *h1, const double *h2, const double *wt,
+                  const double *pmask, int maxit, const nkh_params *prm, nkh_result *out,
+                  double *rnorm_history) {
+  Mesh *m = get_mesh(id);
+  NK_TRY(ready(m, id, true));
+  NK_CHECK(res && h1 && h2 && wt && pmask, NKH_ERR_ARG, "nkd_hmh_gmres: null array");
+  NK_CHECK(maxit >= 0, NKH_ERR_ARG, "nkd_hmh_gmres: maxit=%d", maxit);
+  const nkh_params &p = prm ? *prm : kDefaultParams;
+  NK_CHECK(p.precision == 0, NKH_ERR_UNSUPPORTED, "nkd_hmh_gmres: precision=%d", p.precision);
+  Coef c1, c2;
+  NK_TRY(classify_coefs(*m, h1, h2, &c1, &c2, g_stream));
+  return run_gmres(*m, res, c1, c2, wt, pmask, maxit, p, out, rnorm_history);
 }
 
 int nkh_mg_info(int id, int *lmax, int *nh, int *ncrs, double *crs_avg_iters) {

@@ -64,6 +64,9 @@ struct CoarseSolver {
             const std::vector<double> &val, hipStream_t s);
   // x = A^-1 b to the relative residual rtol; host waits on stream s.
   int solve(double *xout, const double *b, hipStream_t s);
+  // x = M b, one AMG V-cycle (no host wait)
+  int vcycle_only(double *xout, const double *b, hipStream_t s);
+  void vcycle(const double *rhs, double *out, double *part_out, hipStream_t s);
   void release();
 };
 

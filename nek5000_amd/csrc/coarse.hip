@@ -485,23 +485,49 @@ __global__ void __launch_bounds__(256)
   }
 }
 
-// rz = partial sum (first V-cycle)
-__global__ void __launch_bounds__(256) k_pcg_rz0(CrsPcg *st, const double *part_rz, int nb) {
-  if (st->done) return;
-  const double rz = part_sum(part_rz, nb);
-  if (threadIdx.x == 0) st->rz = rz;
+__global__ void k_pcg_reset(CrsPcg *st) { st->done = 0; }
+
+// Iteration k (1-based) of the coarse PCG.  Scalars are not carried in a
+// mutable state: every block sums the partial sums it needs in the same
+// fixed order (so all blocks agree bit for bit).  prz[k&1] holds the r.z
+// partials of the V-cycle of iteration k (prz[0]: the first V-cycle);
+// p of iteration k lives in cp[k&1], p of k-1 in cp[(k-1)&1].
+//
+// Exit test on the residual of iteration k-1 (by any block; block 0 records
+// it for the host): rr <= rtol^2 rr0, or k-1 >= maxit.
+__device__ __forceinline__ bool pcg_exit(CrsPcg *st, const double *prr, int nb, int k,
+                                         double *rr_out) {
+  if (k == 1) return st->done != 0;
+  const double rr = part_sum(prr, nb);
+  *rr_out = rr;
+  return rr <= st->tol2 || k - 1 >= st->maxit;
 }
 
-// p' = z + beta p (own row into the other buffer); w = A p' with the
-// neighbours' p' formed on the fly; partial p'.w
+// p = z + beta p_prev (own row, into cp[k&1]); w = A p with the neighbours'
+// p formed on the fly; partial p.w
 __global__ void __launch_bounds__(256)
     k_pcg_spmv(const int32_t *__restrict__ col, const double *__restrict__ val, int width, int n,
                double *__restrict__ w, double *cp0, double *cp1, const double *__restrict__ z,
-               const CrsPcg *st, double *part) {
-  if (st->done) return;
-  const double beta = st->beta;
-  const double *po = st->pb ? cp1 : cp0;
-  double *pn = st->pb ? cp0 : cp1;
+               CrsPcg *st, const double *prz0, const double *prz1, const double *prr, int nb,
+               int k, double *ppw) {
+  double rr = 0.0;
+  const bool ex = pcg_exit(st, prr, nb, k, &rr);
+  if (ex) {
+    if (blockIdx.x == 0 && threadIdx.x == 0 && k > 1 && !st->done) {
+      st->done = 1;
+      st->iter = k - 1;
+      st->rr = rr;
+    }
+    return;
+  }
+  double beta = 0.0;
+  if (k > 1) {
+    const double rz_new = part_sum((k - 1) & 1 ? prz1 : prz0, nb);
+    const double rz_old = part_sum(k & 1 ? prz1 : prz0, nb);
+    beta = rz_new / rz_old;
+  }
+  const double *po = (k - 1) & 1 ? cp1 : cp0;
+  double *pn = k & 1 ? cp1 : cp0;
   double a = 0.0;
   NK_ROWS(i, n) {
     double s = 0.0;
@@ -515,25 +541,19 @@ __global__ void __launch_bounds__(256)
     w[i] = s;
     a += pi * s;
   }
-  part_store(a, part);
+  part_store(a, ppw);
 }
 
-__global__ void __launch_bounds__(256) k_pcg_alpha(CrsPcg *st, const double *part, int nb) {
-  if (st->done) return;
-  const double pAp = part_sum(part, nb);
-  if (threadIdx.x == 0) {
-    st->pAp = pAp;
-    st->alpha = st->rz / pAp;
-  }
-}
-
-// x += alpha p'; r -= alpha w; partial r.r
+// alpha = r.z / p.w; x += alpha p; r -= alpha w; partial r.r
 __global__ void __launch_bounds__(256)
     k_pcg_update(double *x, double *r, const double *w, const double *cp0, const double *cp1,
-                 int n, const CrsPcg *st, double *part) {
+                 int n, const CrsPcg *st, const double *prz0, const double *prz1,
+                 const double *ppw, int nb, int k, double *prr) {
   if (st->done) return;
-  const double al = st->alpha;
-  const double *pn = st->pb ? cp0 : cp1;
+  const double rz = part_sum((k - 1) & 1 ? prz1 : prz0, nb);
+  const double pAp = part_sum(ppw, nb);
+  const double al = rz / pAp;
+  const double *pn = k & 1 ? cp1 : cp0;
   double a = 0.0;
   NK_ROWS(i, n) {
     x[i] += al * pn[i];
@@ -541,22 +561,17 @@ __global__ void __launch_bounds__(256)
     r[i] = ri;
     a += ri * ri;
   }
-  part_store(a, part);
+  part_store(a, prr);
 }
 
-// after the V-cycle: rz_new, beta, exit test on the residual of the update
-__global__ void __launch_bounds__(256)
-    k_pcg_beta(CrsPcg *st, const double *part_rz, const double *part_rr, int nb) {
-  if (st->done) return;
-  const double rz = part_sum(part_rz, nb);
-  const double rr = part_sum(part_rr, nb);
-  if (threadIdx.x == 0) {
-    st->beta = rz / st->rz;
-    st->rz = rz;
+// the exit test alone (after a batch of iterations)
+__global__ void __launch_bounds__(256) k_pcg_check(CrsPcg *st, const double *prr, int nb, int k) {
+  double rr = 0.0;
+  const bool ex = pcg_exit(st, prr, nb, k, &rr);
+  if (ex && threadIdx.x == 0 && k > 1 && !st->done) {
+    st->done = 1;
+    st->iter = k - 1;
     st->rr = rr;
-    st->pb ^= 1;
-    st->iter += 1;
-    if (rr <= st->tol2 || st->iter >= st->maxit) st->done = 1;
   }
 }
 
@@ -661,6 +676,56 @@ void CoarseSolver::release() {
   n = ndense = nlevels_amg = 0;
 }
 
+// z = M rhs: one V(1,1)-cycle; the partial sums of rhs.z land in part_out
+void CoarseSolver::vcycle(const double *rhs, double *out, double *part_out, hipStream_t s) {
+  const dim3 B(256);
+  const double *bk = rhs;
+  for (int k = 0; k < nlevels_amg; ++k) {
+    AmgLevel &L = lev[k];
+    if (L.wide)
+      hipLaunchKernelGGL(k_amg_pre_w, dim3(wblocks(L.n)), B, 0, s, L.Aw.rp, L.Aw.col, L.Aw.val,
+                         L.n, L.wd, bk, L.x, L.r, st);
+    else
+      hipLaunchKernelGGL(k_amg_pre, dim3(nblocks(L.n)), B, 0, s, L.A.col, L.A.val, L.A.width,
+                         L.n, L.wd, bk, L.x, L.r, st);
+    double *bnext = (k + 1 < nlevels_amg) ? lev[k + 1].b : bd;
+    const int nnext = (k + 1 < nlevels_amg) ? lev[k + 1].n : ndense;
+    hipLaunchKernelGGL(k_csr_apply, dim3(wblocks(nnext)), B, 0, s, L.R.rp, L.R.col, L.R.val,
+                       nnext, L.r, bnext, st);
+    bk = bnext;
+  }
+  hipLaunchKernelGGL(k_dense_apply, dim3(wblocks(ndense)), B, 0, s, dinv_dense, ndense, bd, xd,
+                     st);
+  const double *xc = xd;
+  for (int k = nlevels_amg - 1; k >= 0; --k) {
+    AmgLevel &L = lev[k];
+    const int nbk = nblocks(L.n);
+    hipLaunchKernelGGL(k_ell_apply<true>, dim3(nbk), B, 0, s, L.P.col, L.P.val, L.P.width, L.n,
+                       xc, L.x, st);
+    const double *bk2 = (k == 0) ? rhs : L.b;
+    double *yk = (k == 0) ? out : L.y;
+    if (L.wide)
+      hipLaunchKernelGGL(k_amg_post_w, dim3(wblocks(L.n)), B, 0, s, L.Aw.rp, L.Aw.col,
+                         L.Aw.val, L.n, L.wd, bk2, L.x, yk, st);
+    else
+      hipLaunchKernelGGL(k_amg_post, dim3(nbk), B, 0, s, L.A.col, L.A.val, L.A.width, L.n,
+                         L.wd, bk2, L.x, yk, st, (k == 0) ? part_out : nullptr);
+    xc = yk;
+  }
+}
+
+// One V-cycle as the whole coarse solve (the reference's AMG coarse solver,
+// param(40) = 1, applies its hierarchy once: crs_amg.c:129-).
+int CoarseSolver::vcycle_only(double *xout, const double *b, hipStream_t s) {
+  if (n == 0) return NKH_OK;
+  if (nlevels_amg == 0) return solve(xout, b, s);
+  hipLaunchKernelGGL(k_pcg_reset, dim3(1), dim3(1), 0, s, st);
+  vcycle(b, xout, nullptr, s);
+  NK_HIP(hipGetLastError());
+  solves += 1;
+  return NKH_OK;
+}
+
 int CoarseSolver::solve(double *xout, const double *b, hipStream_t s) {
   if (n == 0) return NKH_OK;
   const dim3 B(256);
@@ -672,67 +737,30 @@ int CoarseSolver::solve(double *xout, const double *b, hipStream_t s) {
     return NKH_OK;
   }
   const int nb0 = nblocks(n);
-  double *p_rr = part, *p_rz = part + kCrsBlocks, *p_pw = part + 2 * kCrsBlocks;
-  // z = M rhs: one V(1,1)-cycle; the partial sums of rhs.z land in part_out
-  auto vcycle = [&](const double *rhs, double *out, double *part_out) {
-    const double *bk = rhs;
-    for (int k = 0; k < nlevels_amg; ++k) {
-      AmgLevel &L = lev[k];
-      if (L.wide)
-        hipLaunchKernelGGL(k_amg_pre_w, dim3(wblocks(L.n)), B, 0, s, L.Aw.rp, L.Aw.col, L.Aw.val,
-                           L.n, L.wd, bk, L.x, L.r, st);
-      else
-        hipLaunchKernelGGL(k_amg_pre, dim3(nblocks(L.n)), B, 0, s, L.A.col, L.A.val, L.A.width,
-                           L.n, L.wd, bk, L.x, L.r, st);
-      double *bnext = (k + 1 < nlevels_amg) ? lev[k + 1].b : bd;
-      const int nnext = (k + 1 < nlevels_amg) ? lev[k + 1].n : ndense;
-      hipLaunchKernelGGL(k_csr_apply, dim3(wblocks(nnext)), B, 0, s, L.R.rp, L.R.col, L.R.val,
-                         nnext, L.r, bnext, st);
-      bk = bnext;
-    }
-    hipLaunchKernelGGL(k_dense_apply, dim3(wblocks(ndense)), B, 0, s, dinv_dense, ndense, bd, xd,
-                       st);
-    const double *xc = xd;
-    for (int k = nlevels_amg - 1; k >= 0; --k) {
-      AmgLevel &L = lev[k];
-      const int nbk = nblocks(L.n);
-      hipLaunchKernelGGL(k_ell_apply<true>, dim3(nbk), B, 0, s, L.P.col, L.P.val, L.P.width, L.n,
-                         xc, L.x, st);
-      const double *bk2 = (k == 0) ? rhs : L.b;
-      double *yk = (k == 0) ? out : L.y;
-      if (L.wide)
-        hipLaunchKernelGGL(k_amg_post_w, dim3(wblocks(L.n)), B, 0, s, L.Aw.rp, L.Aw.col,
-                           L.Aw.val, L.n, L.wd, bk2, L.x, yk, st);
-      else
-        hipLaunchKernelGGL(k_amg_post, dim3(nbk), B, 0, s, L.A.col, L.A.val, L.A.width, L.n,
-                           L.wd, bk2, L.x, yk, st, (k == 0) ? part_out : nullptr);
-      xc = yk;
-    }
-  };
+  double *p_rr = part, *p_pw = part + kCrsBlocks;
+  double *p_rz0 = part + 2 * kCrsBlocks, *p_rz1 = part + 3 * kCrsBlocks;
   const EllDev &A0 = lev[0].A;
   hipLaunchKernelGGL(k_pcg_init, dim3(nb0), B, 0, s, x, r, p[0], b, n, p_rr);
   hipLaunchKernelGGL(k_pcg_start, dim3(1), B, 0, s, st, p_rr, nb0, rtol, maxit);
-  vcycle(r, z, p_rz);
-  hipLaunchKernelGGL(k_pcg_rz0, dim3(1), B, 0, s, st, p_rz, nb0);
+  vcycle(r, z, p_rz0, s);
   NK_HIP(hipGetLastError());
   CrsPcg hs{};
-  int done_iters = 0;
-  int chunk = std::max(4, last_iters + 2);
+  int k = 1;
+  int chunk = std::max(4, last_iters + 1);
   while (true) {
-    for (int q = 0; q < chunk; ++q) {
+    for (int q = 0; q < chunk; ++q, ++k) {
       hipLaunchKernelGGL(k_pcg_spmv, dim3(nb0), B, 0, s, A0.col, A0.val, A0.width, n, w, p[0],
-                         p[1], z, st, p_pw);
-      hipLaunchKernelGGL(k_pcg_alpha, dim3(1), B, 0, s, st, p_pw, nb0);
-      hipLaunchKernelGGL(k_pcg_update, dim3(nb0), B, 0, s, x, r, w, p[0], p[1], n, st, p_rr);
-      vcycle(r, z, p_rz);
-      hipLaunchKernelGGL(k_pcg_beta, dim3(1), B, 0, s, st, p_rz, p_rr, nb0);
+                         p[1], z, st, p_rz0, p_rz1, p_rr, nb0, k, p_pw);
+      hipLaunchKernelGGL(k_pcg_update, dim3(nb0), B, 0, s, x, r, w, p[0], p[1], n, st, p_rz0,
+                         p_rz1, p_pw, nb0, k, p_rr);
+      vcycle(r, z, (k & 1) ? p_rz1 : p_rz0, s);
     }
+    hipLaunchKernelGGL(k_pcg_check, dim3(1), B, 0, s, st, p_rr, nb0, k);
     NK_HIP(hipGetLastError());
-    done_iters += chunk;
     NK_HIP(hipMemcpyAsync(&hs, st, sizeof(CrsPcg), hipMemcpyDeviceToHost, s));
     NK_HIP(hipStreamSynchronize(s));
     if (hs.done) break;
-    NK_CHECK(done_iters <= maxit + chunk, NKH_ERR_STATE, "coarse PCG did not stop");
+    NK_CHECK(k <= maxit + chunk + 1, NKH_ERR_STATE, "coarse PCG did not stop");
     chunk = 4;
   }
   last_iters = hs.iter;

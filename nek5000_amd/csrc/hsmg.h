@@ -65,7 +65,12 @@ struct Mg {
   double *fpart = nullptr;
   FcgState *fst = nullptr;
   int *fflag = nullptr;
+  // GMRES work: V (kGmresM+1 fields), Z (kGmresM fields), x, r, w, partials
+  double *gV = nullptr, *gZ = nullptr, *gx = nullptr, *gr = nullptr, *gw = nullptr;
+  double *gpart = nullptr, *gsm = nullptr;
 };
+
+constexpr int kGmresM = 30;  // SIZE lgmres: Krylov dimension before a restart
 
 int mg_grid(int64_t n);
 int mg_dalloc(double **p, int64_t n);

@@ -394,13 +394,16 @@ static int run_tnsr(double *v, int nv, const double *u, int nu, const double *w,
 
 // Coarse solve (hsmg.f:1321-1354 -> crs_solve, fcrs.c:77-85): assemble by
 // unknown, solve, scatter back (Dirichlet vertices 0).
-static int coarse_solve(Mg &mg, double *e1, const double *r1, hipStream_t s) {
+// crs_mode 1 applies one AMG V-cycle instead (the reference's param(40) = 1
+// coarse solver is likewise one pass of its AMG hierarchy).
+static int coarse_solve(Mg &mg, double *e1, const double *r1, int crs_mode, hipStream_t s) {
   const int n = mg.ncrs;
   if (n > 0) {
     hipLaunchKernelGGL(k_crs_gather, dim3(mg_grid(n)), dim3(256), 0, s, mg.cb, r1, mg.goff,
                        mg.gidx, n);
     NK_HIP(hipGetLastError());
-    NK_TRY(mg.crs.solve(mg.cx, mg.cb, s));
+    if (crs_mode == 1) NK_TRY(mg.crs.vcycle_only(mg.cx, mg.cb, s));
+    else NK_TRY(mg.crs.solve(mg.cx, mg.cb, s));
   }
   hipLaunchKernelGGL(k_crs_scatter, dim3(mg_grid(8 * mg.nel)), dim3(256), 0, s, e1, mg.cx,
                      mg.slot, 8 * mg.nel);
@@ -410,7 +413,7 @@ static int coarse_solve(Mg &mg, double *e1, const double *r1, hipStream_t s) {
 
 // hsmg.f:1855-1949 H1MG_SOLVE (additive V-cycle, sigma = 1): z = M r, r
 // already masked with the fine-level mask.
-int mg_solve(Mesh &mesh, double *z, const double *r, hipStream_t s) {
+int mg_solve(Mesh &mesh, double *z, const double *r, hipStream_t s, int crs_mode) {
   NK_CHECK(mesh.mg, NKH_ERR_STATE, "mesh has no multigrid hierarchy (nkh_mg_setup)");
   Mg &mg = *mesh.mg;
   const int Lm = mg.lmax;
@@ -446,7 +449,7 @@ int mg_solve(Mesh &mesh, double *z, const double *r, hipStream_t s) {
   MgLevel &L1 = mg.lev[1], &L2 = mg.lev[2];
   NK_TRY(run_tnsr<1>(L1.r, L1.nh, rsrc, L2.nh, L2.rwt, L1.Jt, nel, b));
   NK_TRY(mul_inplace(L1.r, L1.mask, L1.n, b));
-  NK_TRY(coarse_solve(mg, L1.e, L1.r, b));
+  NK_TRY(coarse_solve(mg, L1.e, L1.r, crs_mode, b));
   NK_TRY(mul_inplace(L1.e, L1.mask, L1.n, b));
   for (int l = 2; l < Lm; ++l) {  // e_l = e_l * (mask*wt) + J e_{l-1}
     MgLevel &C = mg.lev[l - 1], &F = mg.lev[l];
@@ -659,6 +662,8 @@ __global__ void __launch_bounds__(256) k_fcg_rnorm(FcgState *st, const double *p
 int run_flex_cg(Mesh &m, double *res, Coef h1, Coef h2, const double *wt, const double *mask,
                 int maxit, const nkh_params &prm, nkh_result *out, double *rnorm_history) {
   NK_CHECK(m.mg, NKH_ERR_STATE, "hmh_flex_cg needs the multigrid hierarchy (nkh_mg_setup)");
+  NK_CHECK(prm.param40 == 0 || prm.param40 == 1, NKH_ERR_UNSUPPORTED,
+           "hmh_flex_cg: coarse solver param40=%d (0 exact, 1 one AMG V-cycle)", prm.param40);
   Mg &mg = *m.mg;
   hipStream_t s = cur_stream();
   const int64_t n = m.n;
@@ -704,7 +709,7 @@ int run_flex_cg(Mesh &m, double *res, Coef h1, Coef h2, const double *wt, const 
   int rc = NKH_OK;
   for (int k = 1; k <= maxit; ++k) {
     if (differ && (rc = mul_inplace(r, mgmask, n, s)) != NKH_OK) break;
-    if ((rc = mg_solve(m, z, r, s)) != NKH_OK) break;
+    if ((rc = mg_solve(m, z, r, s, prm.param40)) != NKH_OK) break;
     hipLaunchKernelGGL(k_fcg_dots, g, b, 0, s, z, r, w, wt, n, pa, pb);
     hipLaunchKernelGGL(k_fcg_beta, dim3(1), b, 0, s, mg.fst, pa, pb);
     if (fused) {

@@ -469,7 +469,8 @@ void mg_free(Mesh &mesh) {
   for (void *p : {(void *)mg->fbc, (void *)mg->slot, (void *)mg->goff, (void *)mg->gidx,
                   (void *)mg->cb, (void *)mg->cx, (void *)mg->fr, (void *)mg->fp,
                   (void *)mg->fz, (void *)mg->fw, (void *)mg->fpart, (void *)mg->fst,
-                  (void *)mg->fflag})
+                  (void *)mg->fflag, (void *)mg->gV, (void *)mg->gZ, (void *)mg->gx,
+                  (void *)mg->gr, (void *)mg->gw, (void *)mg->gpart, (void *)mg->gsm})
     if (p) (void)hipFree(p);
   if (mg->ev_fork) (void)hipEventDestroy(mg->ev_fork);
   if (mg->ev_join) (void)hipEventDestroy(mg->ev_join);

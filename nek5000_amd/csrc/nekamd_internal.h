@@ -199,10 +199,15 @@ int mg_setup(Mesh &m, const int *fbc, const int64_t *vertex, const double *x, co
              const double *z);
 void mg_free(Mesh &m);
 // z = M rhs for rhs already masked with the fine multigrid mask
-int mg_solve(Mesh &m, double *z, const double *rhs, hipStream_t s);
+// crs_mode: 0 coarse problem solved to a 1e-13 relative residual, 1 one AMG
+// V-cycle
+int mg_solve(Mesh &m, double *z, const double *rhs, hipStream_t s, int crs_mode = 0);
 int mg_mask_fine(Mesh &m, double *r, hipStream_t s);
 int mg_info(const Mesh &m, int *lmax, int *nh, int *ncrs, double *crs_avg_iters);
 int run_flex_cg(Mesh &m, double *res, Coef h1, Coef h2, const double *wt, const double *mask,
                 int maxit, const nkh_params &prm, nkh_result *out, double *rnorm_history);
+// ---- gmres.hip ----
+int run_gmres(Mesh &m, double *res, Coef h1, Coef h2, const double *wt, const double *mask,
+              int maxit, const nkh_params &prm, nkh_result *out, double *rnorm_history);
 
 }  // namespace nekamd

@@ -11,6 +11,7 @@ Names and argument meanings follow the reference routines:
   * ``Field.h1mg_setup(fbc, vertex, x, y, z)``        -- hsmg.f:2234
   * ``Field.h1mg_solve(z, rhs)``                      -- hsmg.f:1855
   * ``Field.hmh_flex_cg(res, h1, h2, wt, maxit)``     -- hmholtz.f:2164
+  * ``Field.hmh_gmres(res, h1, h2, wt, maxit)``       -- gmres.f:304
 
 Arrays are torch tensors on the GPU (float64, Nek layout (nel,lz1,ly1,lx1)
 C-order = (lx1,ly1,lz1,nel) column-major).  torch is only the allocator and
@@ -62,10 +63,10 @@ def _ptr(t):
 
 
 def default_params(istep=1, param18=0, param22=1e-7, restol=1e-7, precision=0, param42=0,
-                   param21=0.0):
+                   param21=0.0, param40=0):
     """Nek defaults (reader_par.f:26-27,76-79): param(22)=1e-7, restol=param(22);
     param(42)=0 and param(21)=0 unless the case file sets them."""
-    return NkhParams(istep, param18, param22, restol, precision, param42, param21)
+    return NkhParams(istep, param18, param22, restol, precision, param42, param21, param40, 0)
 
 
 class Field:
@@ -239,6 +240,24 @@ class Field:
                                        ctypes.byref(out),
                                        ctypes.c_void_p(hist.ctypes.data) if history else None),
               "nkd_hmh_flex_cg")
+        d = dict(niter=out.niter, converged=out.converged, div0=out.rbn0, rnorm=out.rbn2,
+                 tolpss=out.tol)
+        if history:
+            d["history"] = hist[: out.niter].copy()
+        return d
+
+    def hmh_gmres(self, res, h1, h2, wt, maxit, pmask=None, params=None, history=False):
+        """gmres.f:304 hmh_gmres: res = rhs in, solution out."""
+        prm = params if params is not None else default_params()
+        out = NkhResult()
+        hist = np.zeros(max(int(maxit), 1)) if history else None
+        mask = self.mask if pmask is None else pmask
+        _sync_stream(self.lib)
+        check(self.lib.nkd_hmh_gmres(self.handle, _ptr(res), _ptr(h1), _ptr(h2), _ptr(wt),
+                                     _ptr(mask), int(maxit), ctypes.byref(prm),
+                                     ctypes.byref(out),
+                                     ctypes.c_void_p(hist.ctypes.data) if history else None),
+              "nkd_hmh_gmres")
         d = dict(niter=out.niter, converged=out.converged, div0=out.rbn0, rnorm=out.rbn2,
                  tolpss=out.tol)
         if history:

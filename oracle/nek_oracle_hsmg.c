@@ -955,6 +955,115 @@ int nko_flex_cg(const nko_field *f, nko_mg *mg, double *res, const double *h1,
   return iter;
 }
 
+/*
+ * gmres.f:304-545 HMH_GMRES with h1mg_solve (ifmgrid), restart m = lgmres.
+ * ml = mu = 1 (uzawa_gmres_split, gmres.f:252-271), so no scaling.  res:
+ * rhs in, solution out.  Tolerance as nko_flex_cg.  Returns the iteration
+ * count; rnorm_hist[k] = |gamma_{j+1}| * norm_fac after Arnoldi step k+1.
+ */
+int nko_gmres(const nko_field *f, nko_mg *mg, double *res, const double *h1, const double *h2,
+              int maxit, int m, double param21, int istep, double *rnorm_hist,
+              double *div0_out) {
+  int64_t n = (int64_t)f->n * f->n * f->n * f->nel;
+  const double *wt = f->mult;
+  double *x = calloc((size_t)n, sizeof(double)), *r = malloc(sizeof(double) * n),
+         *w = malloc(sizeof(double) * n);
+  double *v = malloc(sizeof(double) * n * (m + 1)), *z = malloc(sizeof(double) * n * m);
+  double *H = calloc((size_t)m * m, sizeof(double)), *gam = calloc((size_t)m + 1, sizeof(double)),
+         *c = calloc((size_t)m, sizeof(double)), *sn = calloc((size_t)m, sizeof(double));
+#define HH(i, j) H[(i) + m * (j)]
+  const double norm_fac = 1.0 / sqrt(f->vol);
+  double tolps = nko_chktcg1(f, 0.0, res, h1, h2);
+  if (param21 > 0 && tolps > fabs(param21)) tolps = fabs(param21);
+  if (istep == 0) tolps = 1.e-4;
+  double tolpss = tolps, div0 = 0.0, rnorm = 0.0;
+  int iter = 0, iconv = 0, j = 0;
+  while (!iconv) {
+    if (iter == 0) {
+      memcpy(r, res, sizeof(double) * n);
+    } else {
+      ax(f, w, x, h1, h2);
+      for (int64_t i = 0; i < n; i++) r[i] = res[i] - w[i];
+    }
+    gam[0] = sqrt(glsc3(r, r, wt, n));
+    if (iter == 0) {
+      div0 = gam[0] * norm_fac;
+      if (param21 < 0) tolpss = fabs(param21) * div0;
+    }
+    rnorm = 0.0;
+    if (gam[0] == 0.0) break;
+    double temp = 1.0 / gam[0];
+    for (int64_t i = 0; i < n; i++) v[i] = r[i] * temp;
+    int restart = 0;
+    for (j = 1; j <= m; j++) {
+      iter = iter + 1;
+      double *vj = v + n * (j - 1), *zj = z + n * (j - 1);
+      memcpy(w, vj, sizeof(double) * n);
+      nko_mg_solve(mg, zj, w);
+      ax(f, w, zj, h1, h2);
+      for (int i = 1; i <= j; i++) { /* classical Gram-Schmidt, one pass */
+        double sum = 0.0;
+        const double *vi = v + n * (i - 1);
+        for (int64_t q = 0; q < n; q++) sum = sum + w[q] * vi[q] * wt[q];
+        HH(i - 1, j - 1) = sum;
+      }
+      for (int i = 1; i <= j; i++) {
+        const double *vi = v + n * (i - 1);
+        const double hij = HH(i - 1, j - 1);
+        for (int64_t q = 0; q < n; q++) w[q] = w[q] + (-hij) * vi[q];
+      }
+      for (int i = 1; i <= j - 1; i++) { /* previous Givens rotations */
+        temp = HH(i - 1, j - 1);
+        HH(i - 1, j - 1) = c[i - 1] * temp + sn[i - 1] * HH(i, j - 1);
+        HH(i, j - 1) = -sn[i - 1] * temp + c[i - 1] * HH(i, j - 1);
+      }
+      double alpha = sqrt(glsc3(w, w, wt, n));
+      rnorm = 0.0;
+      if (alpha == 0.0) {
+        iconv = 1;
+        break;
+      }
+      double l = sqrt(HH(j - 1, j - 1) * HH(j - 1, j - 1) + alpha * alpha);
+      temp = 1.0 / l;
+      c[j - 1] = HH(j - 1, j - 1) * temp;
+      sn[j - 1] = alpha * temp;
+      HH(j - 1, j - 1) = l;
+      gam[j] = -sn[j - 1] * gam[j - 1];
+      gam[j - 1] = c[j - 1] * gam[j - 1];
+      rnorm = fabs(gam[j]) * norm_fac;
+      if (rnorm_hist) rnorm_hist[iter - 1] = rnorm;
+      if (iter + 1 > maxit || rnorm < tolpss) {
+        iconv = 1;
+        break;
+      }
+      if (j == m) {
+        restart = 1;
+        break;
+      }
+      temp = 1.0 / alpha;
+      double *vn = v + n * j;
+      for (int64_t q = 0; q < n; q++) vn[q] = w[q] * temp;
+    }
+    (void)restart;
+    if (j > m) j = m;
+    /* back substitution c = H^-1 gamma, then x += sum c_i z_i */
+    for (int k = j; k >= 1; k--) {
+      temp = gam[k - 1];
+      for (int i = j; i >= k + 1; i--) temp = temp - HH(k - 1, i - 1) * c[i - 1];
+      c[k - 1] = temp / HH(k - 1, k - 1);
+    }
+    for (int i = 1; i <= j; i++) {
+      const double *zi = z + n * (i - 1);
+      for (int64_t q = 0; q < n; q++) x[q] = x[q] + c[i - 1] * zi[q];
+    }
+  }
+#undef HH
+  memcpy(res, x, sizeof(double) * n);
+  if (div0_out) *div0_out = div0;
+  free(x); free(r); free(w); free(v); free(z); free(H); free(gam); free(c); free(sn);
+  return iter;
+}
+
 /* accessors for tests */
 void nko_mg_apply_fdm(nko_mg *mg, int l, double *e_ext, double *r_ext) {
   do_fast(e_ext, r_ext, mg->fs[l], mg->fd[l], mg->nh[l] + 2, mg->nel);

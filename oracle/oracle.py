@@ -94,6 +94,9 @@ def lib():
         L.nko_flex_cg.argtypes = [ctypes.POINTER(_Field), vp, vp, vp, vp, c_int, c_double,
                                   c_int, vp, ctypes.POINTER(c_double)]
         L.nko_flex_cg.restype = c_int
+        L.nko_gmres.argtypes = [ctypes.POINTER(_Field), vp, vp, vp, vp, c_int, c_int, c_double,
+                                c_int, vp, ctypes.POINTER(c_double)]
+        L.nko_gmres.restype = c_int
         for nm in ("nko_mg_fdm_s", "nko_mg_fdm_d", "nko_mg_mask", "nko_mg_swt", "nko_mg_rwt",
                    "nko_mg_jh"):
             getattr(L, nm).argtypes = [vp, c_int]
@@ -341,6 +344,16 @@ class MG:
         it = lib().nko_flex_cg(ctypes.byref(self.field._s), self.h, _p(res), _p(_c(h1)),
                                _p(_c(h2)), int(maxit), float(param21), int(istep),
                                _p(hist), ctypes.byref(div0))
+        return res, it, hist[:it], div0.value
+
+    def gmres(self, rhs, h1, h2, maxit, param21, m=30, istep=1):
+        """gmres.f:304 hmh_gmres with this V-cycle, restart m."""
+        res = _c(rhs).copy()
+        hist = np.zeros(maxit)
+        div0 = c_double(0)
+        it = lib().nko_gmres(ctypes.byref(self.field._s), self.h, _p(res), _p(_c(h1)),
+                             _p(_c(h2)), int(maxit), int(m), float(param21), int(istep),
+                             _p(hist), ctypes.byref(div0))
         return res, it, hist[:it], div0.value
 
 

@@ -162,9 +162,9 @@ def test_pres_other_solvers_unsupported(gpu):
     u = F.zeros()
     r = F.zeros() + 1.0
     one = F.zeros() + 1.0
-    with pytest.raises(gpu._lib.NekAmdError):
+    with pytest.raises(gpu._lib.NekAmdError):  # param(42) = 1: FDM-Jacobi PCG
         F.hmholtz("PRES", u, r, one, F.zeros(), F.mask, F.mult, -1e-8, 10,
-                  params=gpu.default_params(param42=0))
+                  params=gpu.default_params(param42=1))
     with pytest.raises(gpu._lib.NekAmdError):  # no hierarchy yet
         F.hmholtz("PRES", u, r, one, F.zeros(), F.mask, F.mult, -1e-8, 10,
                   params=gpu.default_params(param42=2))
@@ -195,5 +195,67 @@ def test_flex_cg_pmask_differs_from_mg_mask(gpu):
     assert abs(out["niter"] - ito) <= 1
     k = min(out["niter"], ito)
     np.testing.assert_allclose(out["history"][:k], histo[:k], rtol=1e-6)
+    assert l2rel(host(res), uo) < 1e-9
+    F.destroy()
+
+
+def test_flex_cg_amg_coarse_vcycle(gpu):
+    """param(40) = 1: the coarse problem gets one AMG V-cycle instead of a
+    solve to 1e-13 (the reference's AMG coarse-solver option).  The flexible
+    CG still converges to the same solution (oracle, exact coarse solve);
+    the iteration count may change slightly."""
+    P, mg, F = build(gpu, dict(nel=(12, 11, 10), lx1=4, deform=0.05))
+    assert F.h1mg_info()["ncrs"] > 600  # the AMG path, not the dense inverse
+    h1 = np.ones(P.shape)
+    h2 = np.zeros(P.shape)
+    rhs = P.field.dssum(P.bm1 * P.uniform_field(31)) * P.mask
+    uo, ito, histo, div0 = mg.flex_cg(rhs, h1, h2, 200, -1e-10)
+    res = F.to_dev(rhs)
+    out = F.hmh_flex_cg(res, F.to_dev(h1), F.to_dev(h2), F.mult, 200,
+                        params=gpu.default_params(param42=2, param21=-1e-10, param40=1))
+    say(f"[flex_cg param40=1] gpu {out['niter']} oracle (exact coarse) {ito}")
+    assert out["converged"]
+    assert out["niter"] <= ito + 5
+    assert l2rel(host(res), uo) < 1e-7
+    F.destroy()
+
+
+@pytest.mark.parametrize("case", [MG_CASES[0], MG_CASES[3], MG_CASES[6]])
+def test_gmres(gpu, case):
+    """hmholtz('PRES') with param(42) = 0 -> hmh_gmres (gmres.f:304-545)."""
+    P, mg, F = build(gpu, case)
+    h1 = np.ones(P.shape)
+    h2 = np.full(P.shape, 3.0)
+    b = P.bm1 * P.uniform_field(41)
+    rhs = P.field.dssum(b) * P.mask
+    uo, ito, histo, div0 = mg.gmres(rhs, h1, h2, 200, -1e-11, m=30)
+    u = F.zeros()
+    out = F.hmholtz("PRES", u, F.to_dev(b), F.to_dev(h1), F.to_dev(h2), F.mask, F.mult,
+                    -1e-11, 200, params=gpu.default_params(param42=0, param21=-1e-11),
+                    history=True)
+    say(f"[gmres] {case}: gpu {out['niter']} oracle {ito}")
+    assert abs(out["niter"] - ito) <= 1
+    k = min(out["niter"], ito)
+    np.testing.assert_allclose(out["history"][:k], histo[:k], rtol=1e-6)
+    assert abs(out["rbn0"] - div0) <= 1e-12 * div0
+    assert l2rel(host(u), uo) < 1e-9
+    F.destroy()
+
+
+def test_gmres_restart(gpu):
+    """More than 30 Arnoldi steps: the restart (true residual, new basis)."""
+    P, mg, F = build(gpu, dict(nel=4, lx1=8, deform=0.3))
+    h1 = np.ones(P.shape)
+    h2 = np.zeros(P.shape)
+    rhs = P.field.dssum(P.bm1 * P.uniform_field(42)) * P.mask
+    uo, ito, histo, div0 = mg.gmres(rhs, h1, h2, 200, -1e-11, m=30)
+    res = F.to_dev(rhs)
+    out = F.hmh_gmres(res, F.to_dev(h1), F.to_dev(h2), F.mult, 200,
+                      params=gpu.default_params(param42=0, param21=-1e-11), history=True)
+    say(f"[gmres restart] gpu {out['niter']} oracle {ito}")
+    assert ito > 30
+    assert abs(out["niter"] - ito) <= 1
+    k = min(out["niter"], ito)
+    np.testing.assert_allclose(out["history"][:k], histo[:k], rtol=1e-5)
     assert l2rel(host(res), uo) < 1e-9
     F.destroy()

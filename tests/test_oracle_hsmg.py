@@ -128,3 +128,27 @@ def test_flex_cg_vs_jacobi():
     uj, _, nj, _, _ = P.field.hmholtz(b, h1, h2, -1e-13, 900)
     assert np.linalg.norm(u - uj) / np.linalg.norm(uj) < 1e-9
     assert it * 4 < nj
+
+
+@pytest.mark.parametrize("m", [30, 4])
+def test_gmres_oracle(m):
+    """hmh_gmres (right-preconditioned, restarted): the residual norms it
+    reports are the true ones (recomputed from its solution), they do not
+    increase within a restart cycle, and the solution agrees with the
+    flexible CG's."""
+    P = Problem(box((3, 4, 3), 7, deform=0.1))
+    mg = mg_for(P)
+    h1 = np.ones(P.shape)
+    h2 = np.full(P.shape, 2.0)
+    rhs = P.field.dssum(P.bm1 * P.uniform_field(9)) * P.mask
+    u, it, hist, div0 = mg.gmres(rhs, h1, h2, 300, -1e-11, m=m)
+    assert it < 200
+    res = rhs - P.mask * P.field.dssum(P.field.axhelm(u, h1, h2))
+    rn = np.sqrt(np.sum(res * res * P.mult) / P.field.vol)
+    assert rn < 1.0001e-11 * div0 * 1.001 + 1e-14 * div0
+    np.testing.assert_allclose(rn, hist[-1], rtol=1e-4)
+    for c0 in range(0, it, m):
+        seg = hist[c0:c0 + m]
+        assert np.all(np.diff(seg) <= 1e-12 * seg[0])
+    uf, _, _, _ = mg.flex_cg(rhs, h1, h2, 300, -1e-11)
+    assert np.linalg.norm(u - uf) / np.linalg.norm(uf) < 1e-8
