@@ -166,9 +166,9 @@ def main():
     free_dof = float(torch.sum(F.mask * F.mult))
     log(f"[bench] free unique DOF = {free_dof:.0f}")
 
-    def params_for(crs):
-        return (solver.default_params(param42=2, param21=-args.tol, param40=crs) if hsmg
-                else None)
+    def params_for(crs, precision=0):
+        return (solver.default_params(param42=2, param21=-args.tol, param40=crs,
+                                      precision=precision) if hsmg else None)
 
     prm = params_for(args.crs)
 
@@ -229,22 +229,30 @@ def main():
     total_iters = sum(iters)
     value = free_dof * total_iters / elapsed / 1e9
     mg_info = F.h1mg_info() if hsmg else None
-    variant = None
+    variants = []
     if hsmg and not args.no_variants:
-        # the other coarse solver, two solves after one warm-up (reported
-        # beside the main line, not as its value)
+        # beside the main line (not its value): the other coarse solver, and
+        # the mixed-precision smoother; one warm-up and two timed solves each
+        def measure(label, p):
+            solve(rhs0.clone(), p)
+            rv = [rhs0.clone() for _ in range(2)]
+            torch.cuda.synchronize()
+            tv = time.perf_counter()
+            it_v = [solve(rv[k], p)["niter"] for k in range(2)]
+            torch.cuda.synchronize()
+            el_v = time.perf_counter() - tv
+            v = {"variant": label, "iterations_per_solve": it_v, "seconds_per_solve": el_v / 2,
+                 "value": free_dof * sum(it_v) / el_v / 1e9}
+            log(f"[bench] {v}")
+            variants.append(v)
+
         other = 1 - args.crs
-        solve(rhs0.clone(), params_for(other))
-        rv = [rhs0.clone() for _ in range(2)]
-        torch.cuda.synchronize()
-        tv = time.perf_counter()
-        it_v = [solve(rv[k], params_for(other))["niter"] for k in range(2)]
-        torch.cuda.synchronize()
-        el_v = time.perf_counter() - tv
-        variant = {"coarse_solver": "one AMG V-cycle" if other == 1 else "solved to 1e-13",
-                   "iterations_per_solve": it_v, "seconds_per_solve": el_v / 2,
-                   "value": free_dof * sum(it_v) / el_v / 1e9}
-        log(f"[bench] variant {variant}")
+        measure("coarse: " + ("one AMG V-cycle (param40=1)" if other == 1 else
+                              "AMG-PCG to 1e-13"), params_for(other))
+        measure("mixed: FP32 fine-level smoother, FP64 Krylov and coarse",
+                params_for(args.crs, 1))
+        measure("mixed + one-V-cycle coarse", params_for(1, 1))
+
     traffic = None
     if hsmg and args.nel == 64 and args.lx1 == 8:
         traffic = measured_traffic("axhelm_flexcg_E262144_N7_pmc.md")
@@ -280,7 +288,7 @@ def main():
             "coarse_solver": (None if not hsmg else
                               ("one AMG V-cycle (param40=1)" if args.crs == 1 else
                                "AMG-preconditioned CG to 1e-13 (exact, as XXT)")),
-            "variant": variant,
+            "variants": variants,
             "seconds_per_solve": elapsed / args.steps,
             "free_dof": free_dof, "iterations_per_solve": iters,
             "axhelm_ms": t_ax * 1e3, "gs_ms": t_gs * 1e3,

@@ -40,12 +40,20 @@ typedef struct {
   int param18;     /* param(18): 1 -> plain residual norm, else B^-1-weighted (hmholtz.f:755-759) */
   double param22;  /* param(22): chktcg1 gate (hmholtz.f:54) and relative tol when <0 (:764) */
   double restol;   /* restol(ifield): overrides tin when nonzero and tin>0 (hmholtz.f:677) */
-  int precision;   /* 0: FP64 (the drop-in); 1: FP32 operator inside FP64 refinement */
+  int precision;   /* 0: FP64 (the drop-in); 1: multigrid smoother (fine-level local FDM
+                      solves and overlap sum) in FP32 inside the FP64 Krylov iteration */
   int param42;     /* param(42): PRES solver, 2 -> hmh_flex_cg with h1mg (hmholtz.f:646-653) */
   double param21;  /* param(21): PRES tolerance cap (>0) or relative tolerance (<0) (hmholtz.f:2206-2220) */
   int param40;     /* param(40): coarse solver of h1mg, 0 (XXT) -> solved to 1e-13, 1 (AMG) -> one
                       AMG V-cycle (navier8.f:209-217, crs_amg.c:129-) */
-  int reserved;
+  /* residual projection switches read by hsolve (navier4.f:589-606) */
+  int param93;     /* param(93): number of projection vectors; 0 turns projection off */
+  int param94;     /* param(94): first step of projection for Helmholtz fields (0: off) */
+  int param95;     /* param(95): first step of projection for the pressure (0: off) */
+  int ifprojfld;   /* ifprojfld(ifield) of the field being solved */
+  int ifield;      /* /TSTEP/ ifield; fields above ldimt_proj+1 are not projected */
+  int ldimt_proj;  /* SIZE ldimt_proj */
+  int ifmvbd;      /* moving mesh: the projection space is rebuilt every solve (navier4.f:818) */
 } nkh_params;
 
 /* What the reference leaves behind after a solve. */
@@ -156,6 +164,43 @@ int nkd_hmh_flex_cg(int mesh, double *res, const double *h1, const double *h2,
 int nkd_hmh_gmres(int mesh, double *res, const double *h1, const double *h2, const double *wt,
                   const double *pmask, int maxit, const nkh_params *prm, nkh_result *res_out,
                   double *rnorm_history);
+/* ---- residual projection (hsolve) -------------------------------------- */
+
+/* navier4.f:1179-1206 PROJ_GET_IVAR / hsolve's approx, napprox arrays: a
+   projection space of mmx = (mxprev-4)/2 previous solutions and their images
+   A x on the device, for fields of `mesh` (mxprev = SIZE mxprev, 20 by
+   default; up to 68).  Starts empty with h1old = h2old = 0, as Nek's
+   zero-initialised commons. */
+int nkh_proj_create(int mesh, int mxprev, int *proj);
+int nkh_proj_destroy(int proj);
+/* napprox(2) (vectors held) and napprox(1) (= mmx) */
+int nkh_proj_info(int proj, int *m, int *mmx);
+/* Device pointer to column k of X (which 0), of B = A X (which 1), or to
+   xbar (which 2, k ignored): read access for checks. */
+int nkh_proj_column(int proj, int k, int which, const double **ptr);
+/* navier4.f:636-735 PROJECT1 (+ iproj_chk :809, project1_a :737): b <- b
+   minus its projection on the space, xbar kept for project2; rebuilds the
+   space with the current h1/h2 when they changed (or ifmvbd).  msk = the
+   field's mask, w = its vmult weight.  norms (optional, 2 doubles) receive
+   the weighted norms of b before and after; *ireset (optional) 1 when the
+   space was rebuilt.  Nothing happens while the space is empty. */
+int nkd_project1(int proj, double *b, const double *h1, const double *h2, const double *msk,
+                 const double *w, int ifmvbd, double *norms, int *ireset);
+/* navier4.f:1123-1177 PROJECT2 (+ project2_a, proj_ortho :868-975): x <- x +
+   xbar, then x joins the space (the oldest information leaves it once mmx
+   vectors are held). */
+int nkd_project2(int proj, double *x, const double *h1, const double *h2, const double *msk,
+                 const double *w);
+/* navier4.f:562-634 HSOLVE: hmholtz, or, when the switches in prm select
+   projection (not ifprojfld / param93 / param94 / param95 / istep as in the
+   reference), r*vmk, dssum, project1, hmhzpf (navier4.f:513-560: chktcg1
+   then cggo with its PRES branches), project2.  proj < 0 means no space; it
+   is an error when the switches select projection.  r is modified in place,
+   u receives the solution.  res/history as nkd_hmholtz. */
+int nkd_hsolve(int mesh, const char *name, double *u, double *r, const double *h1,
+               const double *h2, const double *vmk, const double *vml, double tol, int maxit,
+               int isd, int proj, const nkh_params *prm, nkh_result *res, double *history);
+
 /* Hierarchy facts: number of levels, points per direction per level (nh[0]
    = level 1), coarse unknowns, mean coarse CG iterations per solve so far. */
 int nkh_mg_info(int mesh, int *lmax, int *nh, int *ncrs, double *crs_avg_iters);
@@ -185,6 +230,9 @@ int nkh_hmholtz(int mesh, const char *name, double *u, double *rhs, const double
                 const double *h2, const double *mask, const double *mult, double tli,
                 int maxit, int isd, const nkh_params *prm, nkh_result *res,
                 double *history);
+int nkh_hsolve(int mesh, const char *name, double *u, double *r, const double *h1,
+               const double *h2, const double *vmk, const double *vml, double tol, int maxit,
+               int isd, int proj, const nkh_params *prm, nkh_result *res, double *history);
 
 #ifdef __cplusplus
 }

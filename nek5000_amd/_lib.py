@@ -28,7 +28,13 @@ class NkhParams(ctypes.Structure):
         ("param42", c_int),
         ("param21", c_double),
         ("param40", c_int),
-        ("reserved", c_int),
+        ("param93", c_int),
+        ("param94", c_int),
+        ("param95", c_int),
+        ("ifprojfld", c_int),
+        ("ifield", c_int),
+        ("ldimt_proj", c_int),
+        ("ifmvbd", c_int),
     ]
 
 
@@ -77,6 +83,16 @@ SIGNATURES = {
     "nkd_hmh_gmres": (c_int, [c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_int,
                               c_void_p, c_void_p, c_void_p]),
     "nkh_mg_info": (c_int, [c_int, P(c_int), c_void_p, P(c_int), P(c_double)]),
+    "nkh_proj_create": (c_int, [c_int, c_int, P(c_int)]),
+    "nkh_proj_destroy": (c_int, [c_int]),
+    "nkh_proj_info": (c_int, [c_int, P(c_int), P(c_int)]),
+    "nkh_proj_column": (c_int, [c_int, c_int, c_int, P(c_void_p)]),
+    "nkd_project1": (c_int, [c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_int,
+                             c_void_p, P(c_int)]),
+    "nkd_project2": (c_int, [c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]),
+    "nkd_hsolve": (c_int, [c_int, c_char_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
+                           c_void_p, c_double, c_int, c_int, c_int, P(NkhParams), P(NkhResult),
+                           c_void_p]),
     "nkh_timers_enable": (c_int, [c_int]),
     "nkh_timers_reset": (c_int, []),
     "nkh_timers_read": (c_int, [c_char_p, P(c_double), P(c_int64)]),
@@ -85,6 +101,9 @@ SIGNATURES = {
     "nkh_hmholtz": (c_int, [c_int, c_char_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
                             c_void_p, c_double, c_int, c_int, P(NkhParams), P(NkhResult),
                             c_void_p]),
+    "nkh_hsolve": (c_int, [c_int, c_char_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
+                           c_void_p, c_double, c_int, c_int, c_int, P(NkhParams), P(NkhResult),
+                           c_void_p]),
 }
 
 _lib = None

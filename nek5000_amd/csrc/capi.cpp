@@ -3,6 +3,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cctype>
 #include <cmath>
 #include <cstdarg>
 #include <cstring>
@@ -762,9 +763,34 @@ int nkd_setprec(int id, double *d, const double *h1, const double *h2) {
   return run_setprec(*m, d, c1, c2, g_stream);
 }
 
-// Nek's defaults (reader_par.f:26-27,76-79): param(22) = restol = 1e-7,
-// param(42) = 0, param(21) = 0.
-static const nkh_params kDefaultParams = {1, 0, 1.0e-7, 1.0e-7, 0, 0, 0.0, 0, 0};
+// Nek's defaults (reader_par.f:26-27,76-79,86-89,159-162): param(22) =
+// restol = 1e-7, param(42) = 0, param(21) = 0, param(93) = mxprev = 20,
+// param(94) = 5, param(95) = 0, ifprojfld = .false.
+static const nkh_params kDefaultParams = {1, 0, 1.0e-7, 1.0e-7, 0, 0, 0.0, 0,
+                                          20, 5, 0, 0, 1, 1, 0};
+
+// cggo from its head (hmholtz.f:611-653): the PRES branches copy f into x and
+// run hmh_gmres (param42 = 0) or hmh_flex_cg (param42 = 2) in place; other
+// fields take the Jacobi PCG.
+static int cggo_branches(Mesh &m, bool pres, double *x, const double *f, Coef c1, Coef c2,
+                         const double *mask, const double *mult, double tin, int maxit,
+                         const nkh_params &p, nkh_result *res, double *history, hipStream_t s) {
+  if (pres) {
+    NK_HIP(hipMemcpyAsync(x, f, sizeof(double) * m.n, hipMemcpyDeviceToDevice, s));
+    if (p.param42 == 0) return run_gmres(m, x, c1, c2, mult, mask, maxit, p, res, history);
+    return run_flex_cg(m, x, c1, c2, mult, mask, maxit, p, res, history);
+  }
+  return run_cggo(m, x, f, c1, c2, mask, mult, tin, maxit, p, res, history);
+}
+
+static int check_solver_params(const nkh_params &p, bool pres, const char *who) {
+  NK_CHECK(p.precision == 0 || (pres && p.precision == 1), NKH_ERR_UNSUPPORTED,
+           "%s: precision=%d (1 only for the multigrid pressure solvers)", who, p.precision);
+  NK_CHECK(!pres || p.param42 == 2 || p.param42 == 0, NKH_ERR_UNSUPPORTED,
+           "%s: PRES with param42=%d (0 GMRES and 2 flexible CG with h1mg are built)", who,
+           p.param42);
+  return NKH_OK;
+}
 
 int nkd_cggo(int id, double *x, const double *f, const double *h1, const double *h2,
              const double *mask, const double *mult, double tin, int maxit,
@@ -788,29 +814,84 @@ int nkd_hmholtz(int id, const char *name, double *u, double *rhs, const double *
   NK_CHECK(u && rhs && h1 && h2 && mask && mult, NKH_ERR_ARG, "nkd_hmholtz: null array");
   NK_CHECK(isd == 1 || isd == 2, NKH_ERR_ARG, "nkd_hmholtz: isd=%d", isd);
   const nkh_params &p = prm ? *prm : kDefaultParams;
-  NK_CHECK(p.precision == 0, NKH_ERR_UNSUPPORTED, "nkd_hmholtz: precision=%d", p.precision);
   const bool pres = name && std::strncmp(name, "PRES", 4) == 0;
-  NK_CHECK(!pres || p.param42 == 2 || p.param42 == 0, NKH_ERR_UNSUPPORTED,
-           "nkd_hmholtz: PRES with param42=%d (0 GMRES and 2 flexible CG with h1mg are built)",
-           p.param42);
+  NK_TRY(check_solver_params(p, pres, "nkd_hmholtz"));
   hipStream_t s = g_stream;
   Coef c1, c2;
   NK_TRY(classify_coefs(*m, h1, h2, &c1, &c2, s));
   // hmholtz.f:50-51: dssum(rhs); rhs *= mask
   NK_TRY(gs_op(m->gs, rhs, 0, s));
   NK_TRY(mul_inplace(rhs, mask, m->n, s));
-  if (pres) {
-    // hmholtz.f:52-55's chktcg1 only sets cggo's tolerance, which the
-    // flexible-CG branch (hmholtz.f:646-653) does not read: x = f, then
-    // hmh_flex_cg(x, h1, h2, mult, maxit)
-    NK_HIP(hipMemcpyAsync(u, rhs, sizeof(double) * m->n, hipMemcpyDeviceToDevice, s));
-    if (p.param42 == 0) return run_gmres(*m, u, c1, c2, mult, mask, maxit, p, res, history);
-    return run_flex_cg(*m, u, c1, c2, mult, mask, maxit, p, res, history);
-  }
+  // hmholtz.f:52-55's chktcg1 only sets cggo's tolerance, which the PRES
+  // branches (hmholtz.f:639-653) do not read
+  if (pres) return cggo_branches(*m, true, u, rhs, c1, c2, mask, mult, 0.0, maxit, p, res, history, s);
   double tol = std::fabs(tli);
   if (p.param22 == 0.0 || p.istep <= 10) NK_TRY(run_chktcg1(*m, &tol, rhs, c1, c2, mask, mult));
   if (tli < 0) tol = tli;
   return run_cggo(*m, u, rhs, c1, c2, mask, mult, tol, maxit, p, res, history);
+}
+
+int nkd_hsolve(int id, const char *name, double *u, double *r, const double *h1,
+               const double *h2, const double *vmk, const double *vml, double tol, int maxit,
+               int isd, int proj, const nkh_params *prm, nkh_result *res, double *history) {
+  Mesh *m = get_mesh(id);
+  NK_TRY(ready(m, id, true));
+  NK_CHECK(u && r && h1 && h2 && vmk && vml, NKH_ERR_ARG, "nkd_hsolve: null array");
+  const nkh_params &p = prm ? *prm : kDefaultParams;
+  // navier4.f:585-606: capit(name), then the switches
+  char cname[5] = {0, 0, 0, 0, 0};
+  for (int i = 0; i < 4 && name && name[i]; ++i)
+    cname[i] = (char)std::toupper((unsigned char)name[i]);
+  bool ifstdh = !p.ifprojfld;
+  int p945 = p.param94;
+  if (std::strncmp(cname, "PRES", 4) == 0) {
+    ifstdh = false;
+    p945 = p.param95;
+  }
+  if (p.ifield > p.ldimt_proj + 1) ifstdh = true;
+  if (p.param93 == 0) ifstdh = true;
+  if (p945 == 0) ifstdh = true;
+  if (p.istep < p945) ifstdh = true;
+  if (ifstdh) return nkd_hmholtz(id, name, u, r, h1, h2, vmk, vml, tol, maxit, isd, prm, res, history);
+  // cggo's PRES test compares the name as given (hmholtz.f:639)
+  const bool pres = name && std::strncmp(name, "PRES", 4) == 0;
+  NK_CHECK(proj >= 0, NKH_ERR_ARG,
+           "nkd_hsolve: the switches select projection but no projection space was given");
+  NK_CHECK(isd == 1 || isd == 2, NKH_ERR_ARG, "nkd_hsolve: isd=%d", isd);
+  NK_TRY(check_solver_params(p, pres, "nkd_hsolve"));
+  hipStream_t s = g_stream;
+  // navier4.f:615-616: r *= vmk; dssum(r)
+  NK_TRY(mul_inplace(r, vmk, m->n, s));
+  NK_TRY(gs_op(m->gs, r, 0, s));
+  NK_TRY(proj_project1(proj, r, h1, h2, vmk, vml, p.ifmvbd, nullptr, nullptr));
+  Coef c1, c2;
+  NK_TRY(classify_coefs(*m, h1, h2, &c1, &c2, s));
+  // hmhzpf (navier4.f:513-560): tol = tli or |param22|, chktcg1, cggo
+  double t = tol;
+  if (p.param22 != 0.0) t = std::fabs(p.param22);
+  NK_TRY(run_chktcg1(*m, &t, r, c1, c2, vmk, vml));
+  NK_TRY(cggo_branches(*m, pres, u, r, c1, c2, vmk, vml, t, maxit, p, res, history, s));
+  return proj_project2(proj, u, h1, h2, vmk, vml);
+}
+
+int nkh_proj_create(int mesh, int mxprev, int *proj) {
+  NK_CHECK(g_init, NKH_ERR_STATE, "nkh_proj_create: library not initialised");
+  return proj_create(mesh, mxprev, proj);
+}
+int nkh_proj_destroy(int proj) { return proj_destroy(proj); }
+int nkh_proj_info(int proj, int *m, int *mmx) { return proj_info(proj, m, mmx); }
+int nkh_proj_column(int proj, int k, int which, const double **ptr) {
+  return proj_column(proj, k, which, ptr);
+}
+int nkd_project1(int proj, double *b, const double *h1, const double *h2, const double *msk,
+                 const double *w, int ifmvbd, double *norms, int *ireset) {
+  NK_CHECK(g_init, NKH_ERR_STATE, "nkd_project1: library not initialised");
+  return proj_project1(proj, b, h1, h2, msk, w, ifmvbd, norms, ireset);
+}
+int nkd_project2(int proj, double *x, const double *h1, const double *h2, const double *msk,
+                 const double *w) {
+  NK_CHECK(g_init, NKH_ERR_STATE, "nkd_project2: library not initialised");
+  return proj_project2(proj, x, h1, h2, msk, w);
 }
 
 int nkh_mg_setup(int id, const int *fbc, const int64_t *vertex, const double *xm1,
@@ -837,7 +918,8 @@ int nkd_hmh_flex_cg(int id, double *res, const double *h1, const double *h2, con
   NK_CHECK(res && h1 && h2 && wt && pmask, NKH_ERR_ARG, "nkd_hmh_flex_cg: null array");
   NK_CHECK(maxit >= 0, NKH_ERR_ARG, "nkd_hmh_flex_cg: maxit=%d", maxit);
   const nkh_params &p = prm ? *prm : kDefaultParams;
-  NK_CHECK(p.precision == 0, NKH_ERR_UNSUPPORTED, "nkd_hmh_flex_cg: precision=%d", p.precision);
+  NK_CHECK(p.precision == 0 || p.precision == 1, NKH_ERR_UNSUPPORTED,
+           "nkd_hmh_flex_cg: precision=%d", p.precision);
   Coef c1, c2;
   NK_TRY(classify_coefs(*m, h1, h2, &c1, &c2, g_stream));
   return run_flex_cg(*m, res, c1, c2, wt, pmask, maxit, p, out, rnorm_history);
@@ -851,7 +933,8 @@ int nkd_hmh_gmres(int id, double *res, const double *h1, const double *h2, const
   NK_CHECK(res && h1 && h2 && wt && pmask, NKH_ERR_ARG, "nkd_hmh_gmres: null array");
   NK_CHECK(maxit >= 0, NKH_ERR_ARG, "nkd_hmh_gmres: maxit=%d", maxit);
   const nkh_params &p = prm ? *prm : kDefaultParams;
-  NK_CHECK(p.precision == 0, NKH_ERR_UNSUPPORTED, "nkd_hmh_gmres: precision=%d", p.precision);
+  NK_CHECK(p.precision == 0 || p.precision == 1, NKH_ERR_UNSUPPORTED,
+           "nkd_hmh_gmres: precision=%d", p.precision);
   Coef c1, c2;
   NK_TRY(classify_coefs(*m, h1, h2, &c1, &c2, g_stream));
   return run_gmres(*m, res, c1, c2, wt, pmask, maxit, p, out, rnorm_history);
@@ -957,6 +1040,30 @@ int nkh_hmholtz(int id, const char *name, double *u, double *rhs, const double *
   NK_TRY(nkd_hmholtz(id, name, du, dr, d1, d2, dk, dm, tli, maxit, isd, prm, res, history));
   NK_HIP(hipMemcpyAsync(u, du, b, hipMemcpyDeviceToHost, s));
   NK_HIP(hipMemcpyAsync(rhs, dr, b, hipMemcpyDeviceToHost, s));
+  NK_HIP(hipStreamSynchronize(s));
+  return NKH_OK;
+}
+
+int nkh_hsolve(int id, const char *name, double *u, double *r, const double *h1,
+               const double *h2, const double *vmk, const double *vml, double tol, int maxit,
+               int isd, int proj, const nkh_params *prm, nkh_result *res, double *history) {
+  Mesh *m = get_mesh(id);
+  NK_TRY(ready(m, id, true));
+  const int64_t n = m->n;
+  const size_t b = sizeof(double) * n;
+  DevBuf d;
+  NK_HIP(hipMalloc(&d.p, 6 * b));
+  hipStream_t s = g_stream;
+  double *du = d.p, *dr = d.p + n, *d1 = d.p + 2 * n, *d2 = d.p + 3 * n, *dk = d.p + 4 * n,
+         *dm = d.p + 5 * n;
+  NK_HIP(hipMemcpyAsync(dr, r, b, hipMemcpyHostToDevice, s));
+  NK_HIP(hipMemcpyAsync(d1, h1, b, hipMemcpyHostToDevice, s));
+  NK_HIP(hipMemcpyAsync(d2, h2, b, hipMemcpyHostToDevice, s));
+  NK_HIP(hipMemcpyAsync(dk, vmk, b, hipMemcpyHostToDevice, s));
+  NK_HIP(hipMemcpyAsync(dm, vml, b, hipMemcpyHostToDevice, s));
+  NK_TRY(nkd_hsolve(id, name, du, dr, d1, d2, dk, dm, tol, maxit, isd, proj, prm, res, history));
+  NK_HIP(hipMemcpyAsync(u, du, b, hipMemcpyDeviceToHost, s));
+  NK_HIP(hipMemcpyAsync(r, dr, b, hipMemcpyDeviceToHost, s));
   NK_HIP(hipStreamSynchronize(s));
   return NKH_OK;
 }

@@ -187,7 +187,7 @@ int run_gmres(Mesh &m, double *res, Coef h1, Coef h2, const double *wt, const do
       iter += 1;
       double *vj = V + (int64_t)(j - 1) * n, *zj = Z + (int64_t)(j - 1) * n;
       hipLaunchKernelGGL(k_gm_copy_mask, dim3(mg_grid(n)), B, 0, s, w, vj, mgmask, n);
-      NK_TRY(mg_solve(m, zj, w, s, prm.param40));
+      NK_TRY(mg_solve(m, zj, w, s, prm.param40, prm.precision));
       NK_TRY(launch_axhelm(m, w, zj, h1, h2, s));
       NK_TRY(gs_op(m.gs, w, 0, s));
       // classical Gram-Schmidt, one pass (gmres.f:464-474)

@@ -195,8 +195,9 @@ __global__ void __launch_bounds__(256)
 // the weighted value plus the coarse correction already in z,
 // S*w_i + z_i, and dsavg of that (ic.f:1871-1897: dssum, times
 // 1/multiplicity).  Every id group, single members included.
+template <typename E>
 __global__ void __launch_bounds__(256)
-    k_gs_fin(double *__restrict__ z, const double *__restrict__ e, const double *__restrict__ w,
+    k_gs_fin(double *__restrict__ z, const E *__restrict__ e, const double *__restrict__ w,
              const int32_t *__restrict__ off, const int32_t *__restrict__ idx, int64_t ngroups,
              int64_t chunk) {
   const int nb = gridDim.x, b = blockIdx.x;
@@ -205,8 +206,9 @@ __global__ void __launch_bounds__(256)
   const int64_t s1 = (s0 + chunk < ngroups) ? s0 + chunk : ngroups;
   for (int64_t g = s0 + threadIdx.x; g < s1; g += blockDim.x) {
     const int32_t a = off[g], c = off[g + 1];
-    double S = e[idx[a]];
-    for (int32_t q = a + 1; q < c; ++q) S += e[idx[q]];
+    E Se = e[idx[a]];
+    for (int32_t q = a + 1; q < c; ++q) Se += e[idx[q]];
+    const double S = double(Se);
     double acc = S * w[idx[a]] + z[idx[a]];
     for (int32_t q = a + 1; q < c; ++q) {
       const int32_t l = idx[q];
@@ -275,14 +277,23 @@ int gs_assemble(const GsMap &gs, double *w, const CgState *st, hipStream_t s) {
   return NKH_OK;
 }
 
-int gs_fin(const GsMap &gs, double *z, const double *e, const double *w, hipStream_t s) {
+template <typename E>
+static int gs_fin_t(const GsMap &gs, double *z, const E *e, const double *w, hipStream_t s) {
   if (gs.ngroups == 0) return NKH_OK;
   const int nb = kGsBlocks;
   const int64_t chunk = (gs.ngroups + nb - 1) / nb;
-  hipLaunchKernelGGL(k_gs_fin, dim3(nb), dim3(256), 0, s, z, e, w, gs.off, gs.idx, gs.ngroups,
-                     chunk);
+  hipLaunchKernelGGL(k_gs_fin<E>, dim3(nb), dim3(256), 0, s, z, e, w, gs.off, gs.idx,
+                     gs.ngroups, chunk);
   NK_HIP(hipGetLastError());
   return NKH_OK;
+}
+
+int gs_fin(const GsMap &gs, double *z, const double *e, const double *w, hipStream_t s) {
+  return gs_fin_t(gs, z, e, w, s);
+}
+
+int gs_fin(const GsMap &gs, double *z, const float *e, const double *w, hipStream_t s) {
+  return gs_fin_t(gs, z, e, w, s);
 }
 
 }  // namespace nekamd

@@ -43,6 +43,8 @@ struct MgLevel {
   // per element; 0 where a face has no neighbour)
   double *wgh = nullptr;
   double *e = nullptr, *r = nullptr;
+  // single-precision copies for the mixed-precision smoother (fine level)
+  float *fs32 = nullptr, *fl32 = nullptr, *wint32 = nullptr, *wgh32 = nullptr, *e32 = nullptr;
 };
 
 struct Mg {
@@ -73,6 +75,8 @@ struct Mg {
 constexpr int kGmresM = 30;  // SIZE lgmres: Krylov dimension before a restart
 
 int mg_grid(int64_t n);
+// allocate and fill the single-precision smoother data of the fine level
+int mg_prepare_fp32(Mg &mg, hipStream_t s);
 int mg_dalloc(double **p, int64_t n);
 
 // Schwarz solve on level l of r (masked): writes the level's e (not yet

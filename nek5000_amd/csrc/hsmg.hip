@@ -56,17 +56,17 @@ struct Fdm {
 // and read there as wave-uniform operands.  Output: the interior of w, and
 // each ghost face value of w sent to the neighbour's slot it coincides with
 // (gout), where the overlap sum reads it.
-template <int NL>
+template <int NL, typename R>
 __global__ void __launch_bounds__(Fdm<NL>::kThreads)
-    k_sch_fdm(double *__restrict__ wint, double *__restrict__ wgh, const double *__restrict__ r,
+    k_sch_fdm(R *__restrict__ wint, R *__restrict__ wgh, const double *__restrict__ r,
               const int32_t *__restrict__ gin, const int32_t *__restrict__ gout,
-              const double *__restrict__ S, const double *__restrict__ lam, int64_t nel) {
+              const R *__restrict__ S, const R *__restrict__ lam, int64_t nel) {
   constexpr int N = NL - 2, N2 = N * N, N3 = N2 * N, M2 = NL * NL, NLAM = 3 * NL + 1;
   constexpr int P = Fdm<NL>::P, PL = P * NL, T = Fdm<NL>::kThreads;
   constexpr int PPT = Fdm<NL>::PPT, NA = Fdm<NL>::kActive;
-  __shared__ double u[PL * NL];
-  __shared__ double sS[3 * M2];
-  __shared__ double sl[NLAM];
+  __shared__ R u[PL * NL];
+  __shared__ R sS[3 * M2];
+  __shared__ R sl[NLAM];
   const int tid = threadIdx.x;
   // ghost face slot (f,a,b) -> LDS position of the extended array
   auto gpos = [](int q) {
@@ -95,11 +95,11 @@ __global__ void __launch_bounds__(Fdm<NL>::kThreads)
   }
   // contraction of every pencil along direction D (0: r, 1: s, 2: t) with
   // the matrix Sm: y_i = sum_m Sm(m,i) x_m when TR, else Sm(i,m) x_m
-  auto contract = [&](auto dir_tag, auto tr_tag, const double *Sm) {
+  auto contract = [&](auto dir_tag, auto tr_tag, const R *Sm) {
     constexpr int D = decltype(dir_tag)::value;
     constexpr bool TR = decltype(tr_tag)::value;
     constexpr int st = D == 0 ? 1 : (D == 1 ? P : PL);
-    double x[PPT][NL];
+    R x[PPT][NL];
     int base[PPT];
 #pragma unroll
     for (int q = 0; q < PPT; ++q) {
@@ -111,12 +111,12 @@ __global__ void __launch_bounds__(Fdm<NL>::kThreads)
     }
 #pragma unroll 2
     for (int i = 0; i < NL; ++i) {
-      double acc[PPT];
+      R acc[PPT];
 #pragma unroll
-      for (int q = 0; q < PPT; ++q) acc[q] = 0.0;
+      for (int q = 0; q < PPT; ++q) acc[q] = R(0);
 #pragma unroll
       for (int m = 0; m < NL; ++m) {
-        const double sv = TR ? Sm[m + NL * i] : Sm[i + NL * m];
+        const R sv = TR ? Sm[m + NL * i] : Sm[i + NL * m];
 #pragma unroll
         for (int q = 0; q < PPT; ++q) acc[q] += sv * x[q][m];
       }
@@ -142,7 +142,7 @@ __global__ void __launch_bounds__(Fdm<NL>::kThreads)
     __syncthreads();
     const double *re = r + e * N3;
     const int32_t *ge = gin + e * 6 * N2;
-    const double *Se = S + e * 3 * M2;
+    const R *Se = S + e * 3 * M2;
 #pragma unroll
     for (int it = 0; it < (3 * M2 + T - 1) / T; ++it) {
       const int q = tid + it * T;
@@ -154,7 +154,7 @@ __global__ void __launch_bounds__(Fdm<NL>::kThreads)
       const int q = tid + it * T;
       if (q < N3) {
         const int i = q % N, j = (q / N) % N, k = q / N2;
-        u[(i + 1) + P * (j + 1) + PL * (k + 1)] = re[q];
+        u[(i + 1) + P * (j + 1) + PL * (k + 1)] = R(re[q]);
       }
     }
 #pragma unroll
@@ -162,13 +162,13 @@ __global__ void __launch_bounds__(Fdm<NL>::kThreads)
       const int q = tid + it * T;
       if (q < 6 * N2) {
         const int32_t sidx = ge[q];
-        u[gpos(q)] = (sidx >= 0) ? r[sidx] : 0.0;
+        u[gpos(q)] = (sidx >= 0) ? R(r[sidx]) : R(0);
       }
     }
     for (int q = tid; q < M2 * NL; q += T) {  // ghost edges and corners
       const int i = q % NL, j = (q / NL) % NL, k = q / M2;
       const int g = (i == 0 || i == NL - 1) + (j == 0 || j == NL - 1) + (k == 0 || k == NL - 1);
-      if (g >= 2) u[i + P * j + PL * k] = 0.0;
+      if (g >= 2) u[i + P * j + PL * k] = R(0);
     }
     __syncthreads();
     contract(R_(), TR_(), sS);
@@ -180,13 +180,13 @@ __global__ void __launch_bounds__(Fdm<NL>::kThreads)
 #pragma unroll
     for (int q = 0; q < PPT; ++q) {
       if (!pv[q]) continue;
-      const double lij = sl[pa[q]] + sl[NL + pb[q]], eps = sl[3 * NL];
+      const R lij = sl[pa[q]] + sl[NL + pb[q]], eps = sl[3 * NL];
       const int base = pa[q] + P * pb[q];
 #pragma unroll
       for (int k = 0; k < NL; ++k) {
-        const double diag = lij + sl[2 * NL + k];
-        const double v = u[base + PL * k];
-        u[base + PL * k] = (diag > eps) ? v * (1.0 / diag) : 0.0;
+        const R diag = lij + sl[2 * NL + k];
+        const R v = u[base + PL * k];
+        u[base + PL * k] = (diag > eps) ? v * (R(1) / diag) : R(0);
       }
     }
     contract(T_(), NT_(), sS + 2 * M2);
@@ -195,7 +195,7 @@ __global__ void __launch_bounds__(Fdm<NL>::kThreads)
     __syncthreads();
     contract(R_(), NT_(), sS);
     __syncthreads();
-    double *wi = wint + e * N3;
+    R *wi = wint + e * N3;
 #pragma unroll
     for (int it = 0; it < (N3 + T - 1) / T; ++it) {
       const int q = tid + it * T;
@@ -221,16 +221,17 @@ __global__ void __launch_bounds__(Fdm<NL>::kThreads)
 // coincides with it (delivered to this element's ghost slots by k_sch_fdm;
 // 0 where there is no neighbour), face by face in the order r-, r+, s-, s+,
 // t-, t+.
-__global__ void k_sch_out(double *__restrict__ eo, const double *__restrict__ wint,
-                          const double *__restrict__ wgh, int n, int64_t nel) {
+template <typename R>
+__global__ void k_sch_out(R *__restrict__ eo, const R *__restrict__ wint,
+                          const R *__restrict__ wgh, int n, int64_t nel) {
   const int64_t n2 = (int64_t)n * n, n3 = n2 * n;
   for (int64_t l = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; l < n3 * nel;
        l += (int64_t)gridDim.x * blockDim.x) {
     const int64_t e = l / n3;
     const int q = (int)(l - e * n3);
     const int i = q % n, j = (q / n) % n, k = q / (int)n2;
-    const double *g = wgh + e * 6 * n2;
-    double v = wint[l];
+    const R *g = wgh + e * 6 * n2;
+    R v = wint[l];
     if (i == 1) v += g[0 * n2 + j + n * k];
     if (i == n - 2) v += g[1 * n2 + j + n * k];
     if (j == 1) v += g[2 * n2 + i + n * k];
@@ -248,10 +249,10 @@ __global__ void k_sch_out(double *__restrict__ eo, const double *__restrict__ wi
 //           finer level): v = v * w + s.
 //   MODE 3 (prolongation into the fine level before gs_fin): v = s on
 //           element-boundary nodes, v = eb * w + s inside.
-template <int NV, int NU, int MODE>
+template <int NV, int NU, int MODE, typename EB>
 __global__ void k_tnsr(double *__restrict__ v, const double *__restrict__ u,
                        const double *__restrict__ w, const double *__restrict__ A, int64_t nel,
-                       const double *__restrict__ eb) {
+                       const EB *__restrict__ eb) {
   constexpr int NV3 = NV * NV * NV, NU3 = NU * NU * NU;
   __shared__ double sA[NV * NU];
   __shared__ double su[NU3];
@@ -290,7 +291,7 @@ __global__ void k_tnsr(double *__restrict__ v, const double *__restrict__ u,
       } else if constexpr (MODE == 3) {
         const int i = t % NV, j = (t / NV) % NV, k = t / (NV * NV);
         const bool bnd = (i == 0 || i == NV - 1 || j == 0 || j == NV - 1 || k == 0 || k == NV - 1);
-        v[o] = bnd ? s : eb[o] * w[o] + s;
+        v[o] = bnd ? s : double(eb[o]) * w[o] + s;
       } else {
         v[o] = s;
       }
@@ -329,55 +330,65 @@ __global__ void k_neq(const double *a, const double *b, int64_t n, int *flag) {
     if (a[i] != b[i]) atomicOr(flag, 1);
 }
 
-template <int NL>
-void launch_sch_fdm(MgLevel &L, const double *r, int64_t nel, hipStream_t s) {
+template <int NL, typename R>
+void launch_sch_fdm(R *wint, R *wgh, const R *S, const R *lam, const MgLevel &L, const double *r,
+                    int64_t nel, hipStream_t s) {
   const int grid = (int)std::min<int64_t>((nel + 7) / 8 * 8, 1 << 20);
-  hipLaunchKernelGGL(k_sch_fdm<NL>, dim3(grid), dim3(Fdm<NL>::kThreads), 0, s, L.wint, L.wgh,
-                     r, L.gin, L.gout, L.fs, L.fl, nel);
+  hipLaunchKernelGGL((k_sch_fdm<NL, R>), dim3(grid), dim3(Fdm<NL>::kThreads), 0, s, wint, wgh, r,
+                     L.gin, L.gout, S, lam, nel);
 }
 
 }  // namespace
 
 int mg_overlap_out(Mg &mg, int l, double *e, hipStream_t s) {
   MgLevel &L = mg.lev[l];
-  hipLaunchKernelGGL(k_sch_out, dim3(mg_grid(L.n)), dim3(256), 0, s, e, L.wint, L.wgh, L.nh,
+  hipLaunchKernelGGL(k_sch_out<double>, dim3(mg_grid(L.n)), dim3(256), 0, s, e, L.wint, L.wgh,
+                     L.nh, mg.nel);
+  NK_HIP(hipGetLastError());
+  return NKH_OK;
+}
+
+// Local solves + overlap sum in precision R: double uses the level's own
+// arrays, float the single-precision copies (fine level, mixed precision).
+template <typename R>
+static int schwarz_local(Mg &mg, int l, const double *r, R *e, hipStream_t s) {
+  MgLevel &L = mg.lev[l];
+  R *wint, *wgh;
+  const R *S, *lam;
+  if constexpr (std::is_same<R, double>::value) {
+    wint = L.wint; wgh = L.wgh; S = L.fs; lam = L.fl;
+  } else {
+    NK_CHECK(L.fs32, NKH_ERR_STATE, "single-precision smoother data not prepared");
+    wint = L.wint32; wgh = L.wgh32; S = L.fs32; lam = L.fl32;
+  }
+  switch (L.nh + 2) {
+#define NK_C(NL) \
+  case NL: launch_sch_fdm<NL, R>(wint, wgh, S, lam, L, r, mg.nel, s); break;
+    NK_C(5) NK_C(6) NK_C(7) NK_C(8) NK_C(9) NK_C(10) NK_C(11) NK_C(12) NK_C(13) NK_C(14)
+    NK_C(15) NK_C(16) NK_C(17) NK_C(18)
+#undef NK_C
+    default:
+      set_error("Schwarz block of %d points not instantiated", L.nh + 2);
+      return NKH_ERR_UNSUPPORTED;
+  }
+  NK_HIP(hipGetLastError());
+  hipLaunchKernelGGL(k_sch_out<R>, dim3(mg_grid(L.n)), dim3(256), 0, s, e, wint, wgh, L.nh,
                      mg.nel);
   NK_HIP(hipGetLastError());
   return NKH_OK;
 }
 
 int mg_schwarz_local(Mg &mg, int l, const double *r, double *e, hipStream_t s) {
-  MgLevel &L = mg.lev[l];
-  switch (L.nh + 2) {
-    case 5: launch_sch_fdm<5>(L, r, mg.nel, s); break;
-    case 6: launch_sch_fdm<6>(L, r, mg.nel, s); break;
-    case 7: launch_sch_fdm<7>(L, r, mg.nel, s); break;
-    case 8: launch_sch_fdm<8>(L, r, mg.nel, s); break;
-    case 9: launch_sch_fdm<9>(L, r, mg.nel, s); break;
-    case 10: launch_sch_fdm<10>(L, r, mg.nel, s); break;
-    case 11: launch_sch_fdm<11>(L, r, mg.nel, s); break;
-    case 12: launch_sch_fdm<12>(L, r, mg.nel, s); break;
-    case 13: launch_sch_fdm<13>(L, r, mg.nel, s); break;
-    case 14: launch_sch_fdm<14>(L, r, mg.nel, s); break;
-    case 15: launch_sch_fdm<15>(L, r, mg.nel, s); break;
-    case 16: launch_sch_fdm<16>(L, r, mg.nel, s); break;
-    case 17: launch_sch_fdm<17>(L, r, mg.nel, s); break;
-    case 18: launch_sch_fdm<18>(L, r, mg.nel, s); break;
-    default:
-      set_error("Schwarz block of %d points not instantiated", L.nh + 2);
-      return NKH_ERR_UNSUPPORTED;
-  }
-  NK_HIP(hipGetLastError());
-  return mg_overlap_out(mg, l, e, s);
+  return schwarz_local<double>(mg, l, r, e, s);
 }
 
-template <int MODE>
+template <int MODE, typename EB = double>
 static int run_tnsr(double *v, int nv, const double *u, int nu, const double *w, const double *A,
-                    int64_t nel, hipStream_t s, const double *eb = nullptr) {
+                    int64_t nel, hipStream_t s, const EB *eb = nullptr) {
   const int grid = (int)std::min<int64_t>(nel, 1 << 20);
 #define NK_T(V, U)                                                                       \
   if (nv == V && nu == U) {                                                              \
-    hipLaunchKernelGGL((k_tnsr<V, U, MODE>), dim3(grid), dim3(V * V * V > 256 ? 256 : 64), \
+    hipLaunchKernelGGL((k_tnsr<V, U, MODE, EB>), dim3(grid), dim3(V * V * V > 256 ? 256 : 64), \
                        0, s, v, u, w, A, nel, eb);                                       \
     NK_HIP(hipGetLastError());                                                           \
     return NKH_OK;                                                                       \
@@ -413,7 +424,8 @@ static int coarse_solve(Mg &mg, double *e1, const double *r1, int crs_mode, hipS
 
 // hsmg.f:1855-1949 H1MG_SOLVE (additive V-cycle, sigma = 1): z = M r, r
 // already masked with the fine-level mask.
-int mg_solve(Mesh &mesh, double *z, const double *r, hipStream_t s, int crs_mode) {
+int mg_solve(Mesh &mesh, double *z, const double *r, hipStream_t s, int crs_mode,
+             int precision) {
   NK_CHECK(mesh.mg, NKH_ERR_STATE, "mesh has no multigrid hierarchy (nkh_mg_setup)");
   Mg &mg = *mesh.mg;
   const int Lm = mg.lmax;
@@ -427,7 +439,13 @@ int mg_solve(Mesh &mesh, double *z, const double *r, hipStream_t s, int crs_mode
   // otherwise dssum now and the plain sequence after the join.
   MgLevel &T = mg.lev[Lm];
   const bool merged = mesh.gs.interior_zero;
-  NK_TRY(mg_schwarz_local(mg, Lm, r, merged ? T.e : z, s));
+  const bool fp32 = precision == 1 && merged;
+  if (fp32) {
+    NK_TRY(mg_prepare_fp32(mg, s));
+    NK_TRY(schwarz_local<float>(mg, Lm, r, T.e32, s));
+  } else {
+    NK_TRY(schwarz_local<double>(mg, Lm, r, merged ? T.e : z, s));
+  }
   if (!merged) NK_TRY(gs_op(T.gs, z, 0, s));
   // coarser levels on b: restrict, dssum, mask; the Schwarz solve of each
   // intermediate level on c (it only reads the masked residual), beside the
@@ -460,7 +478,10 @@ int mg_solve(Mesh &mesh, double *z, const double *r, hipStream_t s, int crs_mode
   NK_HIP(hipStreamWaitEvent(s, mg.ev_join, 0));
   // z = dssum(e)*(mask*wt) + J e_{lmax-1}, then dsavg (ic.f:1871-1897)
   MgLevel &C = mg.lev[Lm - 1];
-  if (merged) {
+  if (fp32) {
+    NK_TRY((run_tnsr<3, float>(z, T.nh, C.e, C.nh, T.mswt, C.J, nel, s, T.e32)));
+    NK_TRY(gs_fin(mesh.gs, z, T.e32, T.mswt, s));
+  } else if (merged) {
     NK_TRY(run_tnsr<3>(z, T.nh, C.e, C.nh, T.mswt, C.J, nel, s, T.e));
     NK_TRY(gs_fin(mesh.gs, z, T.e, T.mswt, s));
   } else {
@@ -709,7 +730,7 @@ int run_flex_cg(Mesh &m, double *res, Coef h1, Coef h2, const double *wt, const 
   int rc = NKH_OK;
   for (int k = 1; k <= maxit; ++k) {
     if (differ && (rc = mul_inplace(r, mgmask, n, s)) != NKH_OK) break;
-    if ((rc = mg_solve(m, z, r, s, prm.param40)) != NKH_OK) break;
+    if ((rc = mg_solve(m, z, r, s, prm.param40, prm.precision)) != NKH_OK) break;
     hipLaunchKernelGGL(k_fcg_dots, g, b, 0, s, z, r, w, wt, n, pa, pb);
     hipLaunchKernelGGL(k_fcg_beta, dim3(1), b, 0, s, mg.fst, pa, pb);
     if (fused) {

@@ -451,10 +451,51 @@ int mg_dalloc(double **p, int64_t n) {
   return NKH_OK;
 }
 
+namespace {
+__global__ void k_to_f32(float *o, const double *a, int64_t n) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x)
+    o[i] = (float)a[i];
+}
+__global__ void k_fill_f(float *a, float v, int64_t n) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x)
+    a[i] = v;
+}
+int falloc(float **p, int64_t n) {
+  if (hipMalloc(p, sizeof(float) * std::max<int64_t>(n, 1)) != hipSuccess) {
+    set_error("multigrid allocation of %lld floats failed", (long long)n);
+    return NKH_ERR_NOMEM;
+  }
+  return NKH_OK;
+}
+}  // namespace
+
+int mg_prepare_fp32(Mg &mg, hipStream_t s) {
+  MgLevel &L = mg.lev[mg.lmax];
+  if (L.fs32) return NKH_OK;
+  const int nl = L.nh + 2;
+  const int64_t ns = (int64_t)nl * nl * 3 * mg.nel, nlam = (int64_t)(3 * nl + 1) * mg.nel;
+  const int64_t ngh = (int64_t)6 * L.nh * L.nh * mg.nel;
+  NK_TRY(falloc(&L.fs32, ns));
+  NK_TRY(falloc(&L.fl32, nlam));
+  NK_TRY(falloc(&L.wint32, L.n));
+  NK_TRY(falloc(&L.wgh32, ngh));
+  NK_TRY(falloc(&L.e32, L.n));
+  hipLaunchKernelGGL(k_to_f32, dim3(mg_grid(ns)), dim3(256), 0, s, L.fs32, L.fs, ns);
+  hipLaunchKernelGGL(k_to_f32, dim3(mg_grid(nlam)), dim3(256), 0, s, L.fl32, L.fl, nlam);
+  // ghost slots without a neighbour are never written: 0
+  hipLaunchKernelGGL(k_fill_f, dim3(mg_grid(ngh)), dim3(256), 0, s, L.wgh32, 0.0f, ngh);
+  NK_HIP(hipGetLastError());
+  return NKH_OK;
+}
+
 static void free_level(MgLevel &L) {
   for (double *p : {L.J, L.Jt, L.mask, L.rwt, L.mswt, L.fs, L.fl, L.wint, L.wgh, L.e, L.r})
     if (p) (void)hipFree(p);
   for (int32_t *p : {L.gin, L.gout})
+    if (p) (void)hipFree(p);
+  for (float *p : {L.fs32, L.fl32, L.wint32, L.wgh32, L.e32})
     if (p) (void)hipFree(p);
   if (L.own_gs) gs_free(L.gs);
 }

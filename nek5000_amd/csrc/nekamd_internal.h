@@ -173,6 +173,7 @@ int gs_assemble(const GsMap &gs, double *w, const CgState *st, hipStream_t s);
 // Per id group (single members included): S = sum of e; z_i <- mean over
 // the members of S*w_i + z_i.  The fine-level end of the multigrid V-cycle.
 int gs_fin(const GsMap &gs, double *z, const double *e, const double *w, hipStream_t s);
+int gs_fin(const GsMap &gs, double *z, const float *e, const double *w, hipStream_t s);
 // Flags per group from mask/mult; *ok when every group has a 0/1 mask equal
 // at all members and mult summing to 1 (red: kRedBlocks scratch).
 int gs_prepare(GsMap &gs, const double *mask, const double *mult, double *red, bool *ok,
@@ -201,11 +202,23 @@ void mg_free(Mesh &m);
 // z = M rhs for rhs already masked with the fine multigrid mask
 // crs_mode: 0 coarse problem solved to a 1e-13 relative residual, 1 one AMG
 // V-cycle
-int mg_solve(Mesh &m, double *z, const double *rhs, hipStream_t s, int crs_mode = 0);
+// precision: 0 FP64; 1 the fine-level smoother (local FDM solves and overlap
+// sum) in FP32, everything else FP64
+int mg_solve(Mesh &m, double *z, const double *rhs, hipStream_t s, int crs_mode = 0,
+             int precision = 0);
 int mg_mask_fine(Mesh &m, double *r, hipStream_t s);
 int mg_info(const Mesh &m, int *lmax, int *nh, int *ncrs, double *crs_avg_iters);
 int run_flex_cg(Mesh &m, double *res, Coef h1, Coef h2, const double *wt, const double *mask,
                 int maxit, const nkh_params &prm, nkh_result *out, double *rnorm_history);
+// ---- proj.hip ----
+int proj_create(int mesh, int mxprev, int *id);
+int proj_destroy(int id);
+int proj_info(int id, int *m, int *mmx);
+int proj_column(int id, int k, int which, const double **ptr);
+int proj_project1(int id, double *b, const double *h1, const double *h2, const double *msk,
+                  const double *w, int ifmvbd, double *norms, int *ireset_out);
+int proj_project2(int id, double *x, const double *h1, const double *h2, const double *msk,
+                  const double *w);
 // ---- gmres.hip ----
 int run_gmres(Mesh &m, double *res, Coef h1, Coef h2, const double *wt, const double *mask,
               int maxit, const nkh_params &prm, nkh_result *out, double *rnorm_history);

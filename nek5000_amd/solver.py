@@ -12,6 +12,10 @@ Names and argument meanings follow the reference routines:
   * ``Field.h1mg_solve(z, rhs)``                      -- hsmg.f:1855
   * ``Field.hmh_flex_cg(res, h1, h2, wt, maxit)``     -- hmholtz.f:2164
   * ``Field.hmh_gmres(res, h1, h2, wt, maxit)``       -- gmres.f:304
+  * ``Field.hsolve(name, u, r, h1, h2, vmk, vml, tol, maxit, approx)``
+                                                      -- navier4.f:562
+  * ``Projection(field, mxprev)`` with ``project1`` / ``project2``
+                                                      -- navier4.f:636 / :1123
 
 Arrays are torch tensors on the GPU (float64, Nek layout (nel,lz1,ly1,lx1)
 C-order = (lx1,ly1,lz1,nel) column-major).  torch is only the allocator and
@@ -63,10 +67,14 @@ def _ptr(t):
 
 
 def default_params(istep=1, param18=0, param22=1e-7, restol=1e-7, precision=0, param42=0,
-                   param21=0.0, param40=0):
-    """Nek defaults (reader_par.f:26-27,76-79): param(22)=1e-7, restol=param(22);
-    param(42)=0 and param(21)=0 unless the case file sets them."""
-    return NkhParams(istep, param18, param22, restol, precision, param42, param21, param40, 0)
+                   param21=0.0, param40=0, param93=20, param94=5, param95=0, ifprojfld=0,
+                   ifield=1, ldimt_proj=1, ifmvbd=0):
+    """Nek defaults (reader_par.f:26-27,76-79,86-89,159-162): param(22)=1e-7,
+    restol=param(22); param(42)=0, param(21)=0, param(93)=mxprev=20,
+    param(94)=5, param(95)=0 and ifprojfld=.false. unless the case file sets
+    them."""
+    return NkhParams(istep, param18, param22, restol, precision, param42, param21, param40,
+                     param93, param94, param95, ifprojfld, ifield, ldimt_proj, ifmvbd)
 
 
 class Field:
@@ -264,9 +272,87 @@ class Field:
             d["history"] = hist[: out.niter].copy()
         return d
 
+    # ---- residual projection (navier4.f:562) ------------------------------
+    def hsolve(self, name, u, r, h1, h2, vmk, vml, tol, maxit, approx=None, isd=1, params=None,
+               history=False):
+        """navier4.f:562 hsolve: hmholtz, or projection + hmhzpf when the
+        switches in params select it (approx: a Projection of this field).
+        r is modified in place, u gets the solution."""
+        prm = params if params is not None else default_params()
+        res = NkhResult()
+        pres = name[:4] == "PRES"
+        hist = np.zeros(max(int(maxit), 1) if pres else 5 * 900) if history else None
+        _sync_stream(self.lib)
+        check(self.lib.nkd_hsolve(self.handle, name.encode(), _ptr(u), _ptr(r), _ptr(h1),
+                                  _ptr(h2), _ptr(vmk), _ptr(vml), float(tol), int(maxit), isd,
+                                  approx.handle if approx is not None else -1,
+                                  ctypes.byref(prm), ctypes.byref(res),
+                                  ctypes.c_void_p(hist.ctypes.data) if history else None),
+              "nkd_hsolve")
+        out = dict(niter=res.niter, converged=res.converged, rbn0=res.rbn0, rbn2=res.rbn2,
+                   tol=res.tol)
+        if history:
+            out["history"] = (hist[: res.niter].copy() if pres else
+                              hist.reshape(900, 5)[: max(res.niter + 1, 1)].copy())
+        return out
+
     def destroy(self):
         if self.handle is not None and self.handle >= 0:
             check(self.lib.nkh_mesh_destroy(self.handle), "nkh_mesh_destroy")
+            self.handle = None
+
+
+class Projection:
+    """hsolve's approx/napprox (navier4.f:1179 proj_get_ivar): up to
+    mmx = (mxprev-4)/2 previous solutions of one field and their images, on
+    the GPU."""
+
+    def __init__(self, field, mxprev=20):
+        self.field = field
+        self.lib = field.lib
+        h = ctypes.c_int(-1)
+        check(self.lib.nkh_proj_create(field.handle, int(mxprev), ctypes.byref(h)),
+              "nkh_proj_create")
+        self.handle = h.value
+
+    def info(self):
+        m, mmx = ctypes.c_int(), ctypes.c_int()
+        check(self.lib.nkh_proj_info(self.handle, ctypes.byref(m), ctypes.byref(mmx)),
+              "nkh_proj_info")
+        return m.value, mmx.value
+
+    @property
+    def m(self):
+        return self.info()[0]
+
+    def column(self, k, which=0):
+        """Copy of column k of X (which 0) or B = A X (which 1), or xbar (2)."""
+        ptr = ctypes.c_void_p()
+        check(self.lib.nkh_proj_column(self.handle, int(k), int(which), ctypes.byref(ptr)),
+              "nkh_proj_column")
+        t = self.field.zeros()
+        _copy_raw_to_tensor(ptr.value, t)
+        return t
+
+    def project1(self, b, h1, h2, msk, w, ifmvbd=0):
+        """navier4.f:636 project1 in place on b; returns (ireset, bb4, baf)."""
+        norms = np.zeros(2)
+        ir = ctypes.c_int()
+        _sync_stream(self.lib)
+        check(self.lib.nkd_project1(self.handle, _ptr(b), _ptr(h1), _ptr(h2), _ptr(msk), _ptr(w),
+                                    int(ifmvbd), norms.ctypes.data, ctypes.byref(ir)),
+              "nkd_project1")
+        return ir.value, float(norms[0]), float(norms[1])
+
+    def project2(self, x, h1, h2, msk, w):
+        """navier4.f:1123 project2 in place on x."""
+        _sync_stream(self.lib)
+        check(self.lib.nkd_project2(self.handle, _ptr(x), _ptr(h1), _ptr(h2), _ptr(msk), _ptr(w)),
+              "nkd_project2")
+
+    def destroy(self):
+        if self.handle is not None and self.handle >= 0:
+            check(self.lib.nkh_proj_destroy(self.handle), "nkh_proj_destroy")
             self.handle = None
 
 

@@ -108,6 +108,19 @@ def lib():
         L.nko_mg_coarse_solve.argtypes = [vp, vp, vp]
         L.nko_mg_schwarz.argtypes = [vp, c_int, vp, vp]
         L.nko_mg_apply_fdm.argtypes = [vp, c_int, vp, vp]
+        L.nko_proj_create.argtypes = [c_int64, c_int]
+        L.nko_proj_create.restype = vp
+        L.nko_proj_free.argtypes = [vp]
+        L.nko_proj_m.argtypes = [vp]
+        L.nko_proj_m.restype = c_int
+        for nm in ("nko_proj_x", "nko_proj_b"):
+            getattr(L, nm).argtypes = [vp, c_int]
+            getattr(L, nm).restype = ctypes.POINTER(c_double)
+        L.nko_proj_xbar.argtypes = [vp]
+        L.nko_proj_xbar.restype = ctypes.POINTER(c_double)
+        L.nko_project1.argtypes = [ctypes.POINTER(_Field), vp, vp, vp, vp, vp, vp, c_int, vp]
+        L.nko_project1.restype = c_int
+        L.nko_project2.argtypes = [ctypes.POINTER(_Field), vp, vp, vp, vp, vp, vp]
         L.nko_num_threads.restype = c_int
         L.nko_set_num_threads.argtypes = [c_int]
         _lib = L
@@ -355,6 +368,113 @@ class MG:
                              _p(_c(h2)), int(maxit), int(m), float(param21), int(istep),
                              _p(hist), ctypes.byref(div0))
         return res, it, hist[:it], div0.value
+
+
+class Proj:
+    """navier4.f:562-1325 residual projection space of one field (hsolve's
+    approx/napprox), mmx = (mxprev-4)/2 vectors."""
+
+    def __init__(self, field, mxprev=20):
+        self.field = field
+        self.h = lib().nko_proj_create(field.bm1.size, int(mxprev))
+        if not self.h:
+            raise ValueError("mxprev out of range")
+        self.mmx = (int(mxprev) - 4) // 2
+
+    @property
+    def m(self):
+        return lib().nko_proj_m(self.h)
+
+    def _vec(self, ptr):
+        return np.ctypeslib.as_array(ptr, shape=(self.field.bm1.size,)).copy().reshape(
+            self.field.bm1.shape)
+
+    def x(self, k):
+        return self._vec(lib().nko_proj_x(self.h, k))
+
+    def b(self, k):
+        return self._vec(lib().nko_proj_b(self.h, k))
+
+    def xbar(self):
+        return self._vec(lib().nko_proj_xbar(self.h))
+
+    def project1(self, b, h1, h2, msk, w, ifmvbd=0):
+        """In place on b (contiguous float64); returns (ireset, bb4, baf)."""
+        assert b.dtype == np.float64 and b.flags.c_contiguous
+        norms = np.zeros(2)
+        ir = lib().nko_project1(ctypes.byref(self.field._s), self.h, _p(b), _p(_c(h1)),
+                                _p(_c(h2)), _p(_c(msk)), _p(_c(w)), int(ifmvbd), _p(norms))
+        return ir, norms[0], norms[1]
+
+    def project2(self, x, h1, h2, msk, w):
+        assert x.dtype == np.float64 and x.flags.c_contiguous
+        lib().nko_project2(ctypes.byref(self.field._s), self.h, _p(x), _p(_c(h1)), _p(_c(h2)),
+                           _p(_c(msk)), _p(_c(w)))
+
+    def __del__(self):
+        try:
+            lib().nko_proj_free(self.h)
+        except Exception:
+            pass
+
+
+def hsolve_uses_projection(name, istep, param93, param94, param95, ifprojfld, ifield=1,
+                           ldimt_proj=1):
+    """navier4.f:589-606: whether hsolve projects (not ifstdh)."""
+    ifstdh = not ifprojfld
+    p945 = param94
+    if name.upper()[:4] == "PRES":
+        ifstdh = False
+        p945 = param95
+    if ifield > ldimt_proj + 1 or param93 == 0 or p945 == 0 or istep < p945:
+        ifstdh = True
+    return not ifstdh
+
+
+def hsolve(field, name, r, h1, h2, vmk, vml, tol, maxit, proj, params=None, mg=None,
+           param42=2, param21=0.0, use_projection=True, ifmvbd=0):
+    """navier4.f:562-634 hsolve for one solve: (u, niter, (bb4, baf)).
+
+    With use_projection the projection branch runs: r*vmk, dssum, project1,
+    hmhzpf (chktcg1, then cggo or, for PRES, hmh_gmres/hmh_flex_cg with mg),
+    project2.  Otherwise hmholtz.  field.mask must equal vmk and field.mult
+    vml (cggo reads them from the field)."""
+    prm = params or Params(1, 0.0, 1e-7, 1e-7)
+    pres = name.upper()[:4] == "PRES"
+    if not use_projection:
+        if pres:
+            rr = _c(r).copy()
+            field.gs.op(rr, "+")
+            rr *= vmk
+            return _pres_solve(field, mg, rr, h1, h2, maxit, param42, param21, prm.istep) + (
+                (0.0, 0.0),)
+        u, _, niter, _, _ = field.hmholtz(r, h1, h2, tol, maxit, prm)
+        return u, niter, (0.0, 0.0)
+    rr = _c(r).copy() * vmk
+    field.gs.op(rr, "+")
+    _, bb4, baf = proj.project1(rr, h1, h2, vmk, vml, ifmvbd)
+    # hmhzpf (navier4.f:513-560)
+    t = tol
+    if prm.param22 != 0:
+        t = abs(prm.param22)
+    t = field.chktcg1(t, rr, h1, h2)
+    if pres:
+        u, niter = _pres_solve(field, mg, rr, h1, h2, maxit, param42, param21, prm.istep)
+    else:
+        u, niter, _, _ = field.cggo(rr, h1, h2, t, maxit, prm)
+    u = _c(u).copy()
+    proj.project2(u, h1, h2, vmk, vml)
+    return u, niter, (bb4, baf)
+
+
+def _pres_solve(field, mg, f, h1, h2, maxit, param42, param21, istep):
+    """cggo's PRES branches (hmholtz.f:639-653): x = f, then hmh_gmres or
+    hmh_flex_cg."""
+    if param42 == 0:
+        x, it, _, _ = mg.gmres(f, h1, h2, maxit, param21, istep=istep)
+    else:
+        x, it, _, _ = mg.flex_cg(f, h1, h2, maxit, param21, istep=istep)
+    return x, it
 
 
 def num_threads():

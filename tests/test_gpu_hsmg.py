@@ -259,3 +259,28 @@ def test_gmres_restart(gpu):
     np.testing.assert_allclose(out["history"][:k], histo[:k], rtol=1e-5)
     assert l2rel(host(res), uo) < 1e-9
     F.destroy()
+
+
+@pytest.mark.parametrize("case", [MG_CASES[0], MG_CASES[1], MG_CASES[5]])
+def test_flex_cg_fp32_smoother(gpu, case):
+    """precision = 1: the fine-level Schwarz solves in FP32 inside the FP64
+    flexible CG.  The preconditioner changes at the 1e-7 level, so the
+    iteration count may move by a little; the solution meets the same FP64
+    tolerance and agrees with the oracle's to it."""
+    P, mg, F = build(gpu, case)
+    h1 = np.ones(P.shape)
+    h2 = np.full(P.shape, 1.0)
+    b = P.bm1 * P.uniform_field(51)
+    rhs = P.field.dssum(b) * P.mask
+    uo, ito, histo, div0 = mg.flex_cg(rhs, h1, h2, 200, -1e-10)
+    for solver_p42 in (2, 0):
+        u = F.zeros()
+        out = F.hmholtz("PRES", u, F.to_dev(b), F.to_dev(h1), F.to_dev(h2), F.mask, F.mult,
+                        -1e-10, 200,
+                        params=gpu.default_params(param42=solver_p42, param21=-1e-10,
+                                                  precision=1))
+        say(f"[fp32 smoother] {case} param42={solver_p42}: gpu {out['niter']} oracle {ito}")
+        assert out["converged"]
+        assert out["niter"] <= ito + 3
+        assert l2rel(host(u), uo) < 1e-8
+    F.destroy()
