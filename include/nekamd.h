@@ -122,6 +122,14 @@ int nkd_axhelm(int mesh, double *au, const double *u, const double *h1, const do
 int nkd_dssum(int mesh, double *u);
 /* dssum.f:100-161 DSOP: op '+', '*', 'm' (min), 'M' (max). */
 int nkd_dsop(int mesh, double *u, char op);
+/* Adjoint coefficient gradient (no counterpart in the reference: the
+   derivative of its axhelm form, hmholtz.f:188-225, with local_grad3,
+   navier5.f:428-445).  Given the forward solution u of A(h1,h2) u = f and the
+   adjoint lam of A lam = dJ/du (the same symmetric solve, e.g. nkd_hmholtz),
+   writes per local node g1 = dJ/dh1 = -(D lam)^T G (D u) and, when g2 is not
+   NULL, g2 = dJ/dh2 = -lam bm1 u.  For a continuous coefficient the gradient
+   with respect to a shared node's value is nkd_dssum of these. */
+int nkd_adjoint_coef_grad(int mesh, double *g1, double *g2, const double *u, const double *lam);
 /* hmholtz.f:380-524 SETPREC: inverse of the assembled Jacobi diagonal. */
 int nkd_setprec(int mesh, double *d, const double *h1, const double *h2);
 /* hmholtz.f:2-69 HMHOLTZ: dssum(rhs); rhs*=mask; chktcg1; cggo.  name

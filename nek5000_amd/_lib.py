@@ -71,6 +71,7 @@ SIGNATURES = {
     "nkd_dssum": (c_int, [c_int, c_void_p]),
     "nkd_dsop": (c_int, [c_int, c_void_p, c_char]),
     "nkd_setprec": (c_int, [c_int, c_void_p, c_void_p, c_void_p]),
+    "nkd_adjoint_coef_grad": (c_int, [c_int, c_void_p, c_void_p, c_void_p, c_void_p]),
     "nkd_hmholtz": (c_int, [c_int, c_char_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
                             c_void_p, c_double, c_int, c_int, P(NkhParams), P(NkhResult),
                             c_void_p]),

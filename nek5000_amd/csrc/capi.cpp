@@ -754,6 +754,13 @@ int nkd_dsop(int id, double *u, char op) {
   return gs_op(m->gs, u, k, g_stream);
 }
 
+int nkd_adjoint_coef_grad(int id, double *g1, double *g2, const double *u, const double *lam) {
+  Mesh *m = get_mesh(id);
+  NK_TRY(ready(m, id, false));
+  NK_CHECK(g1 && u && lam, NKH_ERR_ARG, "nkd_adjoint_coef_grad: null array");
+  return launch_adj_grad(*m, g1, g2, u, lam, g_stream);
+}
+
 int nkd_setprec(int id, double *d, const double *h1, const double *h2) {
   Mesh *m = get_mesh(id);
   NK_TRY(ready(m, id, true));

@@ -162,6 +162,11 @@ int launch_axhelm_cg(const Mesh &m, double *w, double *p, const double *r, const
 int launch_axhelm_flex(const Mesh &m, double *w, double *p, const double *z, Coef h1, Coef h2,
                        const double *betap, double *rho_part, hipStream_t s);
 
+// ---- adjoint.hip ----
+// g1 = -(D lam)^T G (D u), g2 = -lam*bm1*u per local node (g2 may be null)
+int launch_adj_grad(const Mesh &m, double *g1, double *g2, const double *u, const double *lam,
+                    hipStream_t s);
+
 // ---- gs.hip ----
 int gs_build(GsMap &gs, const int64_t *glo_num_host, int64_t n, int np, int64_t nel);
 void gs_free(GsMap &gs);

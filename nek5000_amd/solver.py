@@ -273,6 +273,14 @@ class Field:
         return d
 
     # ---- residual projection (navier4.f:562) ------------------------------
+    def adjoint_coef_grad(self, g1, u, lam, g2=None):
+        """dJ/dh1 (and dJ/dh2 into g2) per local node from the forward
+        solution u and the adjoint lam."""
+        _sync_stream(self.lib)
+        check(self.lib.nkd_adjoint_coef_grad(self.handle, _ptr(g1),
+                                             _ptr(g2) if g2 is not None else None, _ptr(u),
+                                             _ptr(lam)), "nkd_adjoint_coef_grad")
+
     def hsolve(self, name, u, r, h1, h2, vmk, vml, tol, maxit, approx=None, isd=1, params=None,
                history=False):
         """navier4.f:562 hsolve: hmholtz, or projection + hmhzpf when the

@@ -883,6 +883,49 @@ void nko_axhelm(const nko_field *f, double *au, const double *u, const double *h
   }
 }
 
+/*
+ * Adjoint coefficient gradient of the Helmholtz bilinear form (no reference
+ * counterpart; built from the reference's own pieces): with a(u,v) =
+ * sum_e (D v)^T (h1 G) (D u) + h2 bm1 u v as axhelm applies it, the nodal
+ * derivative of lam^T A(h1,h2) u with respect to the local coefficient value
+ * at node l is (D lam)_l^T G_l (D u)_l (D = local_grad3, navier5.f:428-445;
+ * G = g1..g6 as axhelm pairs them, hmholtz.f:188-218), and with respect to h2
+ * it is lam_l bm1_l u_l.  Outputs the gradients of J with A u = f and
+ * A lam = dJ/du: g1 = -(D lam)^T G (D u), g2 = -lam bm1 u (g2 may be NULL).
+ * Pinned by sum_l g1_l h1_l = -lam . A_L(h1, 0) u (the form is linear in
+ * h1) and by central finite differences of J.
+ */
+void nko_adj_grad(const nko_field *f, double *g1o, double *g2o, const double *u,
+                  const double *lam) {
+  int n = f->n, nxy = n * n, nxyz = n * n * n;
+#pragma omp parallel
+  {
+    double *buf = malloc(sizeof(double) * 6 * nxyz);
+    double *ur = buf, *us = buf + nxyz, *ut = buf + 2 * nxyz, *lr = buf + 3 * nxyz,
+           *ls = buf + 4 * nxyz, *lt = buf + 5 * nxyz;
+#pragma omp for schedule(static)
+    for (int64_t e = 0; e < f->nel; e++) {
+      size_t o = (size_t)e * nxyz;
+      const double *ue = u + o, *le = lam + o;
+      mxm(f->dxm1, n, ue, n, ur, nxy);
+      for (int iz = 0; iz < n; iz++) mxm(ue + iz * nxy, n, f->dxtm1, n, us + iz * nxy, n);
+      mxm(ue, nxy, f->dxtm1, n, ut, n);
+      mxm(f->dxm1, n, le, n, lr, nxy);
+      for (int iz = 0; iz < n; iz++) mxm(le + iz * nxy, n, f->dxtm1, n, ls + iz * nxy, n);
+      mxm(le, nxy, f->dxtm1, n, lt, n);
+      for (int q = 0; q < nxyz; q++) {
+        size_t l = o + q;
+        double wr = ur[q] * f->g1[l] + us[q] * f->g4[l] + ut[q] * f->g5[l];
+        double ws = ur[q] * f->g4[l] + us[q] * f->g2[l] + ut[q] * f->g6[l];
+        double wt = ur[q] * f->g5[l] + us[q] * f->g6[l] + ut[q] * f->g3[l];
+        g1o[l] = -(lr[q] * wr + ls[q] * ws + lt[q] * wt);
+        if (g2o) g2o[l] = -(le[q] * f->bm1[l] * ue[q]);
+      }
+    }
+    free(buf);
+  }
+}
+
 /* hmholtz.f:380-524 SETPREC (3-D, every element deformed) */
 void nko_setprec(const nko_field *f, double *d, const double *h1, const double *h2) {
   int n = f->n, nxyz = n * n * n;

@@ -108,6 +108,7 @@ def lib():
         L.nko_mg_coarse_solve.argtypes = [vp, vp, vp]
         L.nko_mg_schwarz.argtypes = [vp, c_int, vp, vp]
         L.nko_mg_apply_fdm.argtypes = [vp, c_int, vp, vp]
+        L.nko_adj_grad.argtypes = [ctypes.POINTER(_Field), vp, vp, vp, vp]
         L.nko_proj_create.argtypes = [c_int64, c_int]
         L.nko_proj_create.restype = vp
         L.nko_proj_free.argtypes = [vp]
@@ -245,6 +246,15 @@ class Field:
     def dssum(self, u):
         u = _c(u).copy()
         return self.gs.op(u, "+")
+
+    def adj_grad(self, u, lam):
+        """Local nodal gradients (g1, g2) of J w.r.t. h1 and h2 given the
+        forward solution u and the adjoint lam: g1 = -(D lam)^T G (D u),
+        g2 = -lam bm1 u."""
+        g1 = np.zeros_like(self.bm1)
+        g2 = np.zeros_like(self.bm1)
+        lib().nko_adj_grad(ctypes.byref(self._s), _p(g1), _p(g2), _p(_c(u)), _p(_c(lam)))
+        return g1, g2
 
     def setprec(self, h1, h2):
         d = np.zeros_like(self.bm1)
