@@ -89,6 +89,36 @@ int nkh_dgll(double *d, double *dt, const double *z, int nz);
    vertex(2,2,2,nel) int64 -> glo_num(nx,nx,nx,nel) int64; *ngv = ids used. */
 int nkh_setvert3d(int64_t *glo_num, int64_t *ngv, int nx, int nel, const int64_t *vertex);
 
+/* ---- case files (host) --------------------------------------------------- */
+
+/* reader_re2.f:543-620 READ_RE2_HDR: element counts and dimension of a .re2
+   file (#v001..#v004, either byte order); *nfields = boundary-condition
+   sections present. */
+int nkh_re2_header(const char *path, int *nelgt, int *ndim, int *nelgv, int *nfields);
+/* reader_re2.f:2-541 READ_RE2_DATA into Nek's /INPUT/ arrays: igroup(nel);
+   xc, yc, zc(2^ndim, nel) in the file's (preprocessor) corner order (zc
+   unused in 2-D); curve(5,12,nel) and ccurve(12,nel) (' ' where none);
+   cbc(3 chars,6,nel,nfields) ('E  ' where no record) and bc(5,6,nel,nfields)
+   for the first nfields boundary sections.  Any output may be NULL. */
+int nkh_re2_read(const char *path, int nelgt, int ndim, int nfields, int *group, double *xc,
+                 double *yc, double *zc, double *curve, char *ccurve, char *cbc, double *bc);
+/* Element vertex ids from corner coincidence (what Nek takes from the .ma2
+   map, map2.f:712-942, or its parallel connectivity step): corners closer than
+   tol_rel times the shortest element edge share an id, periodic faces
+   (cbc 'P  ', bc(1..2) = connected element and face) join theirs.  3-D;
+   vertex(2,2,2,nel) in the lexicographic corner order nkh_setvert3d reads;
+   ids 1..*nvert in order of first appearance. */
+int nkh_mesh_vertex_ids(int nel, int ndim, const double *xc, const double *yc, const double *zc,
+                        const char *cbc, const double *bc, double tol_rel, int64_t *vertex,
+                        int64_t *nvert);
+/* genxyz.f:1235-1400 LINQUAD (xyzlin, and xyzquad with gh_face_extend for
+   elements with 'm' midside edges), 3-D: GLL coordinates xm1, ym1, zm1
+   (lx1^3 per element) from the corners and curve data of nkh_re2_read.
+   Other curve types return NKH_ERR_UNSUPPORTED. */
+int nkh_mesh_xyz_from_corners(int lx1, int nel, const double *xc, const double *yc,
+                              const double *zc, const char *ccurve, const double *curve,
+                              double *xm1, double *ym1, double *zm1);
+
 /* ---- meshes ------------------------------------------------------------ */
 
 /* coef.f:1-359 GENWZ (3-D GLL part): creates a mesh of nel elements with

@@ -59,6 +59,11 @@ SIGNATURES = {
     "nkh_zwgll": (c_int, [c_void_p, c_void_p, c_int]),
     "nkh_dgll": (c_int, [c_void_p, c_void_p, c_void_p, c_int]),
     "nkh_setvert3d": (c_int, [c_void_p, P(c_int64), c_int, c_int, c_void_p]),
+    "nkh_re2_header": (c_int, [c_char_p, P(c_int), P(c_int), P(c_int), P(c_int)]),
+    "nkh_re2_read": (c_int, [c_char_p, c_int, c_int, c_int] + [c_void_p] * 8),
+    "nkh_mesh_vertex_ids": (c_int, [c_int, c_int, c_void_p, c_void_p, c_void_p, c_void_p,
+                                    c_void_p, c_double, c_void_p, P(c_int64)]),
+    "nkh_mesh_xyz_from_corners": (c_int, [c_int, c_int] + [c_void_p] * 8),
     "nkh_mesh_create": (c_int, [c_int, c_int, P(c_int)]),
     "nkh_mesh_destroy": (c_int, [c_int]),
     "nkh_mesh_geom_from_xyz": (c_int, [c_int, c_void_p, c_void_p, c_void_p]),

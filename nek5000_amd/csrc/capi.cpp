@@ -881,6 +881,24 @@ int nkd_hsolve(int id, const char *name, double *u, double *r, const double *h1,
   return proj_project2(proj, u, h1, h2, vmk, vml);
 }
 
+int nkh_re2_header(const char *path, int *nelgt, int *ndim, int *nelgv, int *nfields) {
+  return re2_header(path, nelgt, ndim, nelgv, nfields);
+}
+int nkh_re2_read(const char *path, int nelgt, int ndim, int nfields, int *group, double *xc,
+                 double *yc, double *zc, double *curve, char *ccurve, char *cbc, double *bc) {
+  return re2_read(path, nelgt, ndim, nfields, group, xc, yc, zc, curve, ccurve, cbc, bc);
+}
+int nkh_mesh_vertex_ids(int nel, int ndim, const double *xc, const double *yc, const double *zc,
+                        const char *cbc, const double *bc, double tol_rel, int64_t *vertex,
+                        int64_t *nvert) {
+  return mesh_vertex_ids(nel, ndim, xc, yc, zc, cbc, bc, tol_rel, vertex, nvert);
+}
+int nkh_mesh_xyz_from_corners(int lx1, int nel, const double *xc, const double *yc,
+                              const double *zc, const char *ccurve, const double *curve,
+                              double *xm1, double *ym1, double *zm1) {
+  return mesh_xyz_from_corners(lx1, nel, xc, yc, zc, ccurve, curve, xm1, ym1, zm1);
+}
+
 int nkh_proj_create(int mesh, int mxprev, int *proj) {
   NK_CHECK(g_init, NKH_ERR_STATE, "nkh_proj_create: library not initialised");
   return proj_create(mesh, mxprev, proj);

@@ -215,6 +215,17 @@ int mg_mask_fine(Mesh &m, double *r, hipStream_t s);
 int mg_info(const Mesh &m, int *lmax, int *nh, int *ncrs, double *crs_avg_iters);
 int run_flex_cg(Mesh &m, double *res, Coef h1, Coef h2, const double *wt, const double *mask,
                 int maxit, const nkh_params &prm, nkh_result *out, double *rnorm_history);
+// ---- mesh_io.cpp ----
+int re2_header(const char *path, int *nelgt, int *ndim, int *nelgv, int *nfields);
+int re2_read(const char *path, int nelgt, int ndim, int nfields, int *group, double *xc,
+             double *yc, double *zc, double *curve, char *ccurve, char *cbc, double *bc);
+int mesh_vertex_ids(int nel, int ndim, const double *xc, const double *yc, const double *zc,
+                    const char *cbc, const double *bc, double tol_rel, int64_t *vertex,
+                    int64_t *nvert);
+int mesh_xyz_from_corners(int lx1, int nel, const double *xc, const double *yc,
+                          const double *zc, const char *ccurve, const double *curve,
+                          double *xm1, double *ym1, double *zm1);
+
 // ---- proj.hip ----
 int proj_create(int mesh, int mxprev, int *id);
 int proj_destroy(int id);

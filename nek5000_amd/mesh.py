@@ -78,7 +78,54 @@ def setvert3d(vertex, lx1):
     return glo, int(ngv.value)
 
 
-class BoxMesh:
+class _HexMesh:
+    """What the solve set-up needs from any hexahedral mesh: vertex ids
+    (nel,2,2,2) [e,k,j,i] (`vertex`), GLL coordinates (`xyz()`), and
+    per-face Dirichlet flags in Nek's preprocessor face order
+    (`dirichlet_faces()`)."""
+
+    def glo_num(self):
+        return setvert3d(self.vertex, self.lx1)
+
+    # symmetric face order (r-, r+, s-, s+, t-, t+) -> preprocessor face
+    # (connect1.f:1009 facind numbering: s-, r+, s+, r-, t-, t+)
+    SYM_TO_PREP = (3, 1, 0, 2, 4, 5)
+
+    def fast_bc(self, code=1):
+        """get_fast_bc codes (fast3d.f:802-877) per element face in symmetric
+        order: `code` on domain-boundary faces (1 = 'O', Dirichlet for the
+        pressure), 0 elsewhere.  Shape (nel, 6) int32."""
+        prep = self.dirichlet_faces()
+        return np.ascontiguousarray(prep[:, list(self.SYM_TO_PREP)].astype(np.int32) * code)
+
+    def face_mask(self):
+        """bdry.f:317- BCMASK before its dsop: 1 everywhere, 0 on the nodes of
+        Dirichlet faces (facev, connect1.f:1058).  Shape (nel, lz1, ly1, lx1)."""
+        n = self.lx1
+        m = np.ones((self.nel, n, n, n))
+        f = self.dirichlet_faces()
+        m[f[:, 0], :, 0, :] = 0.0
+        m[f[:, 1], :, :, n - 1] = 0.0
+        m[f[:, 2], :, n - 1, :] = 0.0
+        m[f[:, 3], :, :, 0] = 0.0
+        m[f[:, 4], 0, :, :] = 0.0
+        m[f[:, 5], n - 1, :, :] = 0.0
+        return m
+
+    def unique_ids(self, glo, ngv):
+        """A global id for every local node: glo_num where nonzero, else a
+        per-element interior id after ngv (setvert3d ifcenter=.true. order)."""
+        n = self.lx1
+        uid = glo.reshape(self.nel, n, n, n).copy()
+        ni = (n - 2) ** 3
+        if ni > 0:
+            base = ngv + np.arange(self.nel, dtype=np.int64)[:, None] * ni
+            inner = base + 1 + np.arange(ni, dtype=np.int64)[None, :]
+            uid[:, 1:-1, 1:-1, 1:-1] = inner.reshape(self.nel, n - 2, n - 2, n - 2)
+        return uid
+
+
+class BoxMesh(_HexMesh):
     """nelx x nely x nelz hexahedra on [0,lx]x[0,ly]x[0,lz].
 
     deform: amplitude a of the map x_i <- x_i + a*h*sin(2 pi x)sin(2 pi y)sin(2 pi z)
@@ -180,9 +227,6 @@ class BoxMesh:
         self._xyz = (x, y, zz)
         return self._xyz
 
-    def glo_num(self):
-        return setvert3d(self.vertex, self.lx1)
-
     def dirichlet_faces(self):
         """(nel, 6) bool in Nek's preprocessor face order (connect1.f:1009
         facind: 1 y=-1, 2 x=+1, 3 y=+1, 4 x=-1, 5 z=-1, 6 z=+1)."""
@@ -200,39 +244,88 @@ class BoxMesh:
         f[:, 5] = ez == self.nelz - 1
         return f
 
-    # symmetric face order (r-, r+, s-, s+, t-, t+) -> preprocessor face
-    # (connect1.f:1009 facind numbering: s-, r+, s+, r-, t-, t+)
-    SYM_TO_PREP = (3, 1, 0, 2, 4, 5)
 
-    def fast_bc(self, code=1):
-        """get_fast_bc codes (fast3d.f:802-877) per element face in symmetric
-        order: `code` on domain-boundary faces (1 = 'O', Dirichlet for the
-        pressure), 0 elsewhere.  Shape (nel, 6) int32."""
-        prep = self.dirichlet_faces()
-        return np.ascontiguousarray(prep[:, list(self.SYM_TO_PREP)].astype(np.int32) * code)
+class Re2Mesh(_HexMesh):
+    """A Nek case mesh from its .re2 file (reader_re2.f) through the library:
+    corners, curve data and boundary conditions as Nek's /INPUT/ holds them,
+    vertex ids by corner coincidence, GLL coordinates by genxyz's linquad
+    ('m' midside edges supported).
 
-    def face_mask(self):
-        """bdry.f:317- BCMASK before its dsop: 1 everywhere, 0 on the nodes of
-        Dirichlet faces (facev, connect1.f:1058).  Shape (nel, lz1, ly1, lx1)."""
-        n = self.lx1
-        m = np.ones((self.nel, n, n, n))
-        f = self.dirichlet_faces()
-        m[f[:, 0], :, 0, :] = 0.0
-        m[f[:, 1], :, :, n - 1] = 0.0
-        m[f[:, 2], :, n - 1, :] = 0.0
-        m[f[:, 3], :, :, 0] = 0.0
-        m[f[:, 4], 0, :, :] = 0.0
-        m[f[:, 5], n - 1, :, :] = 0.0
-        return m
+    field: which boundary-condition section drives dirichlet_faces (0 = the
+            first, Nek's fluid field when the case has one).
+    dirichlet: BC codes whose faces are Dirichlet for the solve; None means
+            every face with a boundary condition other than 'E  ' and 'P  '.
+    """
 
-    def unique_ids(self, glo, ngv):
-        """A global id for every local node: glo_num where nonzero, else a
-        per-element interior id after ngv (setvert3d ifcenter=.true. order)."""
-        n = self.lx1
-        uid = glo.reshape(self.nel, n, n, n).copy()
-        ni = (n - 2) ** 3
-        if ni > 0:
-            base = ngv + np.arange(self.nel, dtype=np.int64)[:, None] * ni
-            inner = base + 1 + np.arange(ni, dtype=np.int64)[None, :]
-            uid[:, 1:-1, 1:-1, 1:-1] = inner.reshape(self.nel, n - 2, n - 2, n - 2)
-        return uid
+    def __init__(self, path, lx1, field=0, dirichlet=None, tol_rel=1e-2):
+        lib = load()
+        self.path = path
+        self.lx1 = lx1
+        p = path.encode()
+        nelgt, ndim, nelgv, nfld = (ctypes.c_int() for _ in range(4))
+        check(lib.nkh_re2_header(p, ctypes.byref(nelgt), ctypes.byref(ndim), ctypes.byref(nelgv),
+                                 ctypes.byref(nfld)), "nkh_re2_header")
+        self.nel, self.ndim, self.nelgv, self.nfields = (nelgt.value, ndim.value, nelgv.value,
+                                                         nfld.value)
+        nel, nv = self.nel, 2 ** self.ndim
+        self.group = np.zeros(nel, dtype=np.int32)
+        self.xc = np.zeros((nel, nv))
+        self.yc = np.zeros((nel, nv))
+        self.zc = np.zeros((nel, nv))
+        self.curve = np.zeros((nel, 12, 5))
+        self.ccurve = np.zeros((nel, 12), dtype="S1")
+        self.cbc = np.zeros((max(self.nfields, 1), nel, 6), dtype="S3")
+        self.bc = np.zeros((max(self.nfields, 1), nel, 6, 5))
+        check(lib.nkh_re2_read(p, nel, self.ndim, self.nfields, self.group.ctypes.data,
+                               self.xc.ctypes.data, self.yc.ctypes.data, self.zc.ctypes.data,
+                               self.curve.ctypes.data, self.ccurve.ctypes.data,
+                               self.cbc.ctypes.data, self.bc.ctypes.data), "nkh_re2_read")
+        self.field = field
+        self.dirichlet = None if dirichlet is None else {c.encode() if isinstance(c, str) else c
+                                                         for c in dirichlet}
+        self._vertex = None
+        self._xyz = None
+        self.tol_rel = tol_rel
+
+    @property
+    def n(self):
+        return self.nel * self.lx1 ** 3
+
+    @property
+    def vertex(self):
+        """Global vertex ids (1-based), shape (nel,2,2,2) indexed [e,k,j,i]."""
+        if self._vertex is None:
+            lib = load()
+            v = np.zeros(self.nel * 8, dtype=np.int64)
+            nv = c_int64(0)
+            f = self.field if self.nfields else None
+            check(lib.nkh_mesh_vertex_ids(
+                self.nel, self.ndim, self.xc.ctypes.data, self.yc.ctypes.data,
+                self.zc.ctypes.data, self.cbc[f].ctypes.data if f is not None else None,
+                self.bc[f].ctypes.data if f is not None else None, float(self.tol_rel),
+                v.ctypes.data, ctypes.byref(nv)), "nkh_mesh_vertex_ids")
+            self.nvert = int(nv.value)
+            self._vertex = v.reshape(self.nel, 2, 2, 2)
+        return self._vertex
+
+    def xyz(self):
+        """GLL coordinates, each (nel, lz1, ly1, lx1) float64."""
+        if self._xyz is None:
+            lib = load()
+            n = self.lx1
+            out = [np.zeros((self.nel, n, n, n)) for _ in range(3)]
+            check(lib.nkh_mesh_xyz_from_corners(
+                n, self.nel, self.xc.ctypes.data, self.yc.ctypes.data, self.zc.ctypes.data,
+                self.ccurve.ctypes.data, self.curve.ctypes.data, *[a.ctypes.data for a in out]),
+                "nkh_mesh_xyz_from_corners")
+            self._xyz = tuple(out)
+        return self._xyz
+
+    def dirichlet_faces(self):
+        """(nel, 6) bool in Nek's preprocessor face order."""
+        if not self.nfields:
+            return np.zeros((self.nel, 6), dtype=bool)
+        cb = self.cbc[self.field]
+        if self.dirichlet is None:
+            return (cb != b"E  ") & (cb != b"P  ")
+        return np.isin(cb, list(self.dirichlet))
