@@ -36,6 +36,17 @@ struct AmgLevel {
   double *b = nullptr, *x = nullptr, *y = nullptr, *r = nullptr;
 };
 
+// The PCG update folded into a V-cycle's first smoothing kernel: rnew =
+// rold - alpha w, x += alpha pn, alpha = (prz sum) / (ppw sum); partial sums
+// of r_new . r_new into prr.
+struct PcgUpdate {
+  const double *rold, *w, *pn;
+  double *x, *rnew;  // rnew is the V-cycle's right-hand side
+  const double *prz, *ppw;
+  int nb;
+  double *prr;
+};
+
 // Scalars of one coarse PCG solve, resident on the device.
 struct CrsPcg {
   double rz, pAp, alpha, beta, rr, tol2;
@@ -49,7 +60,8 @@ struct CoarseSolver {
   double *dinv_dense = nullptr;
   double *xd = nullptr, *bd = nullptr;
   // PCG vectors and state
-  double *x = nullptr, *r = nullptr, *z = nullptr, *w = nullptr;
+  double *x = nullptr, *z = nullptr, *w = nullptr;
+  double *rb[2] = {nullptr, nullptr};  // residual, alternating by iteration
   double *p[2] = {nullptr, nullptr};
   double *part = nullptr;  // 4 partial-sum arrays of kCrsBlocks
   CrsPcg *st = nullptr;
@@ -66,7 +78,8 @@ struct CoarseSolver {
   int solve(double *xout, const double *b, hipStream_t s);
   // x = M b, one AMG V-cycle (no host wait)
   int vcycle_only(double *xout, const double *b, hipStream_t s);
-  void vcycle(const double *rhs, double *out, double *part_out, hipStream_t s);
+  void vcycle(const double *rhs, double *out, double *part_out, hipStream_t s,
+              const PcgUpdate *up = nullptr);
   void release();
 };
 

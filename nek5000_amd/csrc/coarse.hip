@@ -346,6 +346,42 @@ __global__ void __launch_bounds__(256)
   }
 }
 
+// The PCG update of iteration k fused into the first smoothing step of the
+// next V-cycle: alpha = r.z / p.w from the partial sums; x += alpha p;
+// b = r_old - alpha w (formed on the fly for the neighbours too, with the
+// same expression, and stored as the new residual of this row); then the
+// smoothing step of k_amg_pre on b; partial sums of b.b for the exit test.
+// r_old and r_new are different arrays (neighbours read r_old).
+__global__ void __launch_bounds__(256)
+    k_amg_pre_upd(const int32_t *__restrict__ col, const double *__restrict__ val, int width,
+                  int n, const double *__restrict__ wd, const double *__restrict__ rold,
+                  const double *__restrict__ w, const double *__restrict__ pn,
+                  double *__restrict__ xs, double *__restrict__ rnew, double *__restrict__ x,
+                  double *__restrict__ r, const CrsPcg *st, const double *prz,
+                  const double *ppw, int nb, double *prr) {
+  if (st->done) return;
+  const double rz = part_sum(prz, nb);
+  const double pAp = part_sum(ppw, nb);
+  const double al = rz / pAp;
+  double a = 0.0;
+  NK_ROWS(i, n) {
+    double s = 0.0;
+    for (int c = 0; c < width; ++c) {
+      const int64_t q = (int64_t)c * n + i;
+      const int j = col[q];
+      const double bj = rold[j] - al * w[j];
+      s += val[q] * (wd[j] * bj);
+    }
+    xs[i] += al * pn[i];
+    const double bi = rold[i] - al * w[i];
+    rnew[i] = bi;
+    x[i] = wd[i] * bi;
+    r[i] = bi - s;
+    a += bi * bi;
+  }
+  part_store(a, prr);
+}
+
 // y = x + wd*(b - A x); with part: partial sum of b.y
 __global__ void __launch_bounds__(256)
     k_amg_post(const int32_t *__restrict__ col, const double *__restrict__ val, int width, int n,
@@ -544,26 +580,6 @@ __global__ void __launch_bounds__(256)
   part_store(a, ppw);
 }
 
-// alpha = r.z / p.w; x += alpha p; r -= alpha w; partial r.r
-__global__ void __launch_bounds__(256)
-    k_pcg_update(double *x, double *r, const double *w, const double *cp0, const double *cp1,
-                 int n, const CrsPcg *st, const double *prz0, const double *prz1,
-                 const double *ppw, int nb, int k, double *prr) {
-  if (st->done) return;
-  const double rz = part_sum((k - 1) & 1 ? prz1 : prz0, nb);
-  const double pAp = part_sum(ppw, nb);
-  const double al = rz / pAp;
-  const double *pn = k & 1 ? cp1 : cp0;
-  double a = 0.0;
-  NK_ROWS(i, n) {
-    x[i] += al * pn[i];
-    const double ri = r[i] - al * w[i];
-    r[i] = ri;
-    a += ri * ri;
-  }
-  part_store(a, prr);
-}
-
 // the exit test alone (after a batch of iterations)
 __global__ void __launch_bounds__(256) k_pcg_check(CrsPcg *st, const double *prr, int nb, int k) {
   double rr = 0.0;
@@ -649,7 +665,7 @@ int CoarseSolver::setup(const std::vector<int32_t> &rowptr, const std::vector<in
     NK_TRY(dalloc(&L.b, L.n));
   }
   if (nlevels_amg == 0) return NKH_OK;  // dense solve of the whole problem
-  for (double **q : {&x, &r, &z, &w, &p[0], &p[1]}) NK_TRY(dalloc(q, n));
+  for (double **q : {&x, &rb[0], &rb[1], &z, &w, &p[0], &p[1]}) NK_TRY(dalloc(q, n));
   NK_TRY(dalloc(&part, 4 * kCrsBlocks));
   NK_HIP(hipMalloc(&st, sizeof(CrsPcg)));
   CrsPcg init{};
@@ -668,21 +684,28 @@ void CoarseSolver::release() {
       if (q) (void)hipFree(q);
   }
   lev.clear();
-  for (double *q : {dinv_dense, xd, bd, x, r, z, w, p[0], p[1], part})
+  for (double *q : {dinv_dense, xd, bd, x, rb[0], rb[1], z, w, p[0], p[1], part})
     if (q) (void)hipFree(q);
   if (st) (void)hipFree(st);
-  dinv_dense = xd = bd = x = r = z = w = p[0] = p[1] = part = nullptr;
+  dinv_dense = xd = bd = x = rb[0] = rb[1] = z = w = p[0] = p[1] = part = nullptr;
   st = nullptr;
   n = ndense = nlevels_amg = 0;
 }
 
-// z = M rhs: one V(1,1)-cycle; the partial sums of rhs.z land in part_out
-void CoarseSolver::vcycle(const double *rhs, double *out, double *part_out, hipStream_t s) {
+// z = M rhs: one V(1,1)-cycle; the partial sums of rhs.z land in part_out.
+// With up, rhs is first formed as the PCG update r_old - alpha w of the
+// level-0 smoothing kernel (k_amg_pre_upd).
+void CoarseSolver::vcycle(const double *rhs, double *out, double *part_out, hipStream_t s,
+                          const PcgUpdate *up) {
   const dim3 B(256);
   const double *bk = rhs;
   for (int k = 0; k < nlevels_amg; ++k) {
     AmgLevel &L = lev[k];
-    if (L.wide)
+    if (k == 0 && up)
+      hipLaunchKernelGGL(k_amg_pre_upd, dim3(nblocks(L.n)), B, 0, s, L.A.col, L.A.val,
+                         L.A.width, L.n, L.wd, up->rold, up->w, up->pn, up->x, up->rnew, L.x,
+                         L.r, st, up->prz, up->ppw, up->nb, up->prr);
+    else if (L.wide)
       hipLaunchKernelGGL(k_amg_pre_w, dim3(wblocks(L.n)), B, 0, s, L.Aw.rp, L.Aw.col, L.Aw.val,
                          L.n, L.wd, bk, L.x, L.r, st);
     else
@@ -740,9 +763,10 @@ int CoarseSolver::solve(double *xout, const double *b, hipStream_t s) {
   double *p_rr = part, *p_pw = part + kCrsBlocks;
   double *p_rz0 = part + 2 * kCrsBlocks, *p_rz1 = part + 3 * kCrsBlocks;
   const EllDev &A0 = lev[0].A;
-  hipLaunchKernelGGL(k_pcg_init, dim3(nb0), B, 0, s, x, r, p[0], b, n, p_rr);
+  // residual of iteration k in rb[k & 1]
+  hipLaunchKernelGGL(k_pcg_init, dim3(nb0), B, 0, s, x, rb[0], p[0], b, n, p_rr);
   hipLaunchKernelGGL(k_pcg_start, dim3(1), B, 0, s, st, p_rr, nb0, rtol, maxit);
-  vcycle(r, z, p_rz0, s);
+  vcycle(rb[0], z, p_rz0, s);
   NK_HIP(hipGetLastError());
   CrsPcg hs{};
   int k = 1;
@@ -751,9 +775,11 @@ int CoarseSolver::solve(double *xout, const double *b, hipStream_t s) {
     for (int q = 0; q < chunk; ++q, ++k) {
       hipLaunchKernelGGL(k_pcg_spmv, dim3(nb0), B, 0, s, A0.col, A0.val, A0.width, n, w, p[0],
                          p[1], z, st, p_rz0, p_rz1, p_rr, nb0, k, p_pw);
-      hipLaunchKernelGGL(k_pcg_update, dim3(nb0), B, 0, s, x, r, w, p[0], p[1], n, st, p_rz0,
-                         p_rz1, p_pw, nb0, k, p_rr);
-      vcycle(r, z, (k & 1) ? p_rz1 : p_rz0, s);
+      // the update (x += alpha p, r -= alpha w) runs inside the V-cycle's
+      // first smoothing kernel
+      PcgUpdate up{rb[(k - 1) & 1], w, (k & 1) ? p[1] : p[0], x, rb[k & 1],
+                   ((k - 1) & 1) ? p_rz1 : p_rz0, p_pw, nb0, p_rr};
+      vcycle(rb[k & 1], z, (k & 1) ? p_rz1 : p_rz0, s, &up);
     }
     hipLaunchKernelGGL(k_pcg_check, dim3(1), B, 0, s, st, p_rr, nb0, k);
     NK_HIP(hipGetLastError());

@@ -252,7 +252,7 @@ __global__ void k_sch_out(R *__restrict__ eo, const R *__restrict__ wint,
 template <int NV, int NU, int MODE, typename EB>
 __global__ void k_tnsr(double *__restrict__ v, const double *__restrict__ u,
                        const double *__restrict__ w, const double *__restrict__ A, int64_t nel,
-                       const EB *__restrict__ eb) {
+                       const EB *__restrict__ eb, const double *__restrict__ om) {
   constexpr int NV3 = NV * NV * NV, NU3 = NU * NU * NU;
   __shared__ double sA[NV * NU];
   __shared__ double su[NU3];
@@ -293,7 +293,7 @@ __global__ void k_tnsr(double *__restrict__ v, const double *__restrict__ u,
         const bool bnd = (i == 0 || i == NV - 1 || j == 0 || j == NV - 1 || k == 0 || k == NV - 1);
         v[o] = bnd ? s : double(eb[o]) * w[o] + s;
       } else {
-        v[o] = s;
+        v[o] = om ? s * om[o] : s;
       }
     }
   }
@@ -382,14 +382,16 @@ int mg_schwarz_local(Mg &mg, int l, const double *r, double *e, hipStream_t s) {
   return schwarz_local<double>(mg, l, r, e, s);
 }
 
+// om (MODE 1 only, may be null): 0/1 mask applied to the output
 template <int MODE, typename EB = double>
 static int run_tnsr(double *v, int nv, const double *u, int nu, const double *w, const double *A,
-                    int64_t nel, hipStream_t s, const EB *eb = nullptr) {
+                    int64_t nel, hipStream_t s, const EB *eb = nullptr,
+                    const double *om = nullptr) {
   const int grid = (int)std::min<int64_t>(nel, 1 << 20);
 #define NK_T(V, U)                                                                       \
   if (nv == V && nu == U) {                                                              \
     hipLaunchKernelGGL((k_tnsr<V, U, MODE, EB>), dim3(grid), dim3(V * V * V > 256 ? 256 : 64), \
-                       0, s, v, u, w, A, nel, eb);                                       \
+                       0, s, v, u, w, A, nel, eb, om);                                   \
     NK_HIP(hipGetLastError());                                                           \
     return NKH_OK;                                                                       \
   }
@@ -454,9 +456,11 @@ int mg_solve(Mesh &mesh, double *z, const double *r, hipStream_t s, int crs_mode
   const double *rsrc = r;
   for (int l = Lm - 1; l >= 2; --l) {
     MgLevel &C = mg.lev[l], &F = mg.lev[l + 1];
-    NK_TRY(run_tnsr<1>(C.r, C.nh, rsrc, F.nh, F.rwt, C.Jt, nel, b));
+    // mask * dssum(v) = dssum(mask * v): the level mask is 0/1 and equal at
+    // every copy of a node (made so by a product gs at set-up)
+    NK_TRY(run_tnsr<1>(C.r, C.nh, rsrc, F.nh, F.rwt, C.Jt, nel, b, (const double *)nullptr,
+                       C.mask));
     NK_TRY(gs_op(C.gs, C.r, 0, b));
-    NK_TRY(mul_inplace(C.r, C.mask, C.n, b));
     NK_HIP(hipEventRecord(mg.ev_lr[l], b));
     NK_HIP(hipStreamWaitEvent(c, mg.ev_lr[l], 0));
     NK_TRY(mg_schwarz_local(mg, l, C.r, C.e, c));
@@ -465,10 +469,11 @@ int mg_solve(Mesh &mesh, double *z, const double *r, hipStream_t s, int crs_mode
     rsrc = C.r;
   }
   MgLevel &L1 = mg.lev[1], &L2 = mg.lev[2];
+  // the level-1 mask needs no pass of its own: the coarse gather reads only
+  // unmasked vertices and the scatter writes 0 at masked ones (their slot is
+  // -1), which is what masking r1 before and e1 after the solve would give
   NK_TRY(run_tnsr<1>(L1.r, L1.nh, rsrc, L2.nh, L2.rwt, L1.Jt, nel, b));
-  NK_TRY(mul_inplace(L1.r, L1.mask, L1.n, b));
   NK_TRY(coarse_solve(mg, L1.e, L1.r, crs_mode, b));
-  NK_TRY(mul_inplace(L1.e, L1.mask, L1.n, b));
   for (int l = 2; l < Lm; ++l) {  // e_l = e_l * (mask*wt) + J e_{l-1}
     MgLevel &C = mg.lev[l - 1], &F = mg.lev[l];
     NK_HIP(hipStreamWaitEvent(b, mg.ev_le[l], 0));
