@@ -158,12 +158,15 @@ def test_hmholtz_variants(gpu, case, h2v, h1var):
         h1 = 1.0 + 0.5 * (P.uniform_field(12) + 1.0)
     h2 = np.full(P.shape, h2v)
     (u_o, r_o, n_o, hist_o, _), (u, r, out) = run_both(P, F, rhs, h1, h2, -1e-8)
-    # the oracle's own sensitivity to roundoff: the same solve with the rhs
-    # perturbed at the 1e-16 level (CG amplifies last-bit differences)
+    # the spread of FP64 round-off trajectories: the oracle's solve with the
+    # rhs perturbed at the 1e-16 level, four times (CG amplifies last-bit
+    # differences; the oracle's own OpenMP sums vary at that level too)
     rng = np.random.default_rng(0)
-    u_p = P.field.hmholtz(rhs * (1 + 1e-16 * rng.standard_normal(P.shape)), h1, h2, -1e-8,
-                          900)[0]
-    sens = l2rel(u_p, u_o)
+    sens = 0.0
+    for _ in range(4):
+        u_p = P.field.hmholtz(rhs * (1 + 1e-16 * rng.standard_normal(P.shape)), h1, h2, -1e-8,
+                              900)[0]
+        sens = max(sens, l2rel(u_p, u_o))
     say(f"[hmholtz] {case} h2={h2v} oracle={n_o} gpu={out['niter']} "
         f"diff={l2rel(u, u_o):.2e} oracle-sensitivity={sens:.2e}")
     assert abs(out["niter"] - n_o) <= 1
