@@ -41,7 +41,10 @@ typedef struct {
   double param22;  /* param(22): chktcg1 gate (hmholtz.f:54) and relative tol when <0 (:764) */
   double restol;   /* restol(ifield): overrides tin when nonzero and tin>0 (hmholtz.f:677) */
   int precision;   /* 0: FP64 (the drop-in); 1: multigrid smoother (fine-level local FDM
-                      solves and overlap sum) in FP32 inside the FP64 Krylov iteration */
+                      solves and overlap sum) in FP32 inside the FP64 Krylov iteration;
+                      2 (flexible CG only): FP64 defect correction around single-precision
+                      inner solves (FP32 operator, fine-level V-cycle and vectors; FP64
+                      inner products, coarse solve and true residuals), same tolerance */
   int param42;     /* param(42): PRES solver, 2 -> hmh_flex_cg with h1mg (hmholtz.f:646-653) */
   double param21;  /* param(21): PRES tolerance cap (>0) or relative tolerance (<0) (hmholtz.f:2206-2220) */
   int param40;     /* param(40): coarse solver of h1mg, 0 (XXT) -> solved to 1e-13, 1 (AMG) -> one

@@ -13,7 +13,7 @@ OUT = os.path.join(HERE, "libnekamd.so")
 OBJ = os.path.join(HERE, "_objs")
 SOURCES = ["capi.cpp", "kernels.hip", "axhelm.hip", "axhelm_cg.hip", "gs.hip", "cg.hip", "hsmg.hip",
            "hsmg_setup.hip", "coarse.hip",
-           "gmres.hip", "proj.hip", "adjoint.hip", "mesh_io.cpp"]
+           "gmres.hip", "proj.hip", "adjoint.hip", "mesh_io.cpp", "mixed.hip"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("NEKAMD_ARCH", "gfx950")

@@ -646,6 +646,7 @@ int nkd_mesh_geom_from_xyz(int id, const double *x, const double *y, const doubl
   NK_TRY(launch_geom(*m, x, y, z, g_stream));
   NK_HIP(hipStreamSynchronize(g_stream));
   m->has_geom = true;
+  m->geom_version += 1;
   return finish_mass(*m);
 }
 
@@ -680,6 +681,7 @@ int nkh_mesh_set_geom(int id, const double *g1, const double *g2, const double *
                      hipMemcpyHostToDevice));
   NK_HIP(hipMemcpy(m->bm1, bm1, sizeof(double) * m->n, hipMemcpyHostToDevice));
   m->has_geom = true;
+  m->geom_version += 1;
   return finish_mass(*m);
 }
 
@@ -785,14 +787,19 @@ static int cggo_branches(Mesh &m, bool pres, double *x, const double *f, Coef c1
   if (pres) {
     NK_HIP(hipMemcpyAsync(x, f, sizeof(double) * m.n, hipMemcpyDeviceToDevice, s));
     if (p.param42 == 0) return run_gmres(m, x, c1, c2, mult, mask, maxit, p, res, history);
+    if (p.precision == 2)
+      return run_flex_cg_mixed(m, x, c1, c2, mult, mask, maxit, p, res, history);
     return run_flex_cg(m, x, c1, c2, mult, mask, maxit, p, res, history);
   }
   return run_cggo(m, x, f, c1, c2, mask, mult, tin, maxit, p, res, history);
 }
 
 static int check_solver_params(const nkh_params &p, bool pres, const char *who) {
-  NK_CHECK(p.precision == 0 || (pres && p.precision == 1), NKH_ERR_UNSUPPORTED,
-           "%s: precision=%d (1 only for the multigrid pressure solvers)", who, p.precision);
+  NK_CHECK(p.precision == 0 || (pres && p.precision == 1) ||
+               (pres && p.precision == 2 && p.param42 == 2),
+           NKH_ERR_UNSUPPORTED,
+           "%s: precision=%d (1 for the multigrid pressure solvers, 2 for the flexible CG)", who,
+           p.precision);
   NK_CHECK(!pres || p.param42 == 2 || p.param42 == 0, NKH_ERR_UNSUPPORTED,
            "%s: PRES with param42=%d (0 GMRES and 2 flexible CG with h1mg are built)", who,
            p.param42);
@@ -943,10 +950,12 @@ int nkd_hmh_flex_cg(int id, double *res, const double *h1, const double *h2, con
   NK_CHECK(res && h1 && h2 && wt && pmask, NKH_ERR_ARG, "nkd_hmh_flex_cg: null array");
   NK_CHECK(maxit >= 0, NKH_ERR_ARG, "nkd_hmh_flex_cg: maxit=%d", maxit);
   const nkh_params &p = prm ? *prm : kDefaultParams;
-  NK_CHECK(p.precision == 0 || p.precision == 1, NKH_ERR_UNSUPPORTED,
+  NK_CHECK(p.precision >= 0 && p.precision <= 2, NKH_ERR_UNSUPPORTED,
            "nkd_hmh_flex_cg: precision=%d", p.precision);
   Coef c1, c2;
   NK_TRY(classify_coefs(*m, h1, h2, &c1, &c2, g_stream));
+  if (p.precision == 2)
+    return run_flex_cg_mixed(*m, res, c1, c2, wt, pmask, maxit, p, out, rnorm_history);
   return run_flex_cg(*m, res, c1, c2, wt, pmask, maxit, p, out, rnorm_history);
 }
 

@@ -167,8 +167,9 @@ int gs_op(const GsMap &gs, double *u, int op, hipStream_t s) {
 // of groups; chunks are handed out so that the blocks resident on one XCD
 // (workgroups are dealt to the 8 XCDs round-robin) walk neighbouring chunks
 // and share their L2 lines.
+template <typename T>
 __global__ void __launch_bounds__(256)
-    k_gs_assemble(double *__restrict__ w, const int32_t *__restrict__ off,
+    k_gs_assemble(T *__restrict__ w, const int32_t *__restrict__ off,
                   const int32_t *__restrict__ idx, const uint8_t *__restrict__ flag,
                   int64_t ngroups, int64_t chunk, const CgState *__restrict__ st) {
   if (st && st->done) return;
@@ -181,9 +182,9 @@ __global__ void __launch_bounds__(256)
     if (f == 2) continue;
     const int32_t a = off[s], e = off[s + 1];
     if (f == 1) {
-      for (int32_t q = a; q < e; ++q) w[idx[q]] = 0.0;
+      for (int32_t q = a; q < e; ++q) w[idx[q]] = T(0);
     } else {
-      double acc = w[idx[a]];
+      T acc = w[idx[a]];
       for (int32_t q = a + 1; q < e; ++q) acc += w[idx[q]];
       for (int32_t q = a; q < e; ++q) w[idx[q]] = acc;
     }
@@ -195,9 +196,9 @@ __global__ void __launch_bounds__(256)
 // the weighted value plus the coarse correction already in z,
 // S*w_i + z_i, and dsavg of that (ic.f:1871-1897: dssum, times
 // 1/multiplicity).  Every id group, single members included.
-template <typename E>
+template <typename E, typename Z, typename W>
 __global__ void __launch_bounds__(256)
-    k_gs_fin(double *__restrict__ z, const E *__restrict__ e, const double *__restrict__ w,
+    k_gs_fin(Z *__restrict__ z, const E *__restrict__ e, const W *__restrict__ w,
              const int32_t *__restrict__ off, const int32_t *__restrict__ idx, int64_t ngroups,
              int64_t chunk) {
   const int nb = gridDim.x, b = blockIdx.x;
@@ -209,13 +210,13 @@ __global__ void __launch_bounds__(256)
     E Se = e[idx[a]];
     for (int32_t q = a + 1; q < c; ++q) Se += e[idx[q]];
     const double S = double(Se);
-    double acc = S * w[idx[a]] + z[idx[a]];
+    double acc = S * double(w[idx[a]]) + double(z[idx[a]]);
     for (int32_t q = a + 1; q < c; ++q) {
       const int32_t l = idx[q];
-      acc += S * w[l] + z[l];
+      acc += S * double(w[l]) + double(z[l]);
     }
     acc = acc * (1.0 / (double)(c - a));
-    for (int32_t q = a; q < c; ++q) z[idx[q]] = acc;
+    for (int32_t q = a; q < c; ++q) z[idx[q]] = Z(acc);
   }
 }
 
@@ -267,22 +268,31 @@ int gs_prepare(GsMap &gs, const double *mask, const double *mult, double *red, b
   return NKH_OK;
 }
 
-int gs_assemble(const GsMap &gs, double *w, const CgState *st, hipStream_t s) {
+template <typename T>
+static int gs_assemble_t(const GsMap &gs, T *w, const CgState *st, hipStream_t s) {
   if (gs.ngroups == 0) return NKH_OK;
   const int nb = kGsBlocks;
   const int64_t chunk = (gs.ngroups + nb - 1) / nb;
-  hipLaunchKernelGGL(k_gs_assemble, dim3(nb), dim3(256), 0, s, w, gs.off, gs.idx, gs.flag,
+  hipLaunchKernelGGL(k_gs_assemble<T>, dim3(nb), dim3(256), 0, s, w, gs.off, gs.idx, gs.flag,
                      gs.ngroups, chunk, st);
   NK_HIP(hipGetLastError());
   return NKH_OK;
 }
 
-template <typename E>
-static int gs_fin_t(const GsMap &gs, double *z, const E *e, const double *w, hipStream_t s) {
+int gs_assemble(const GsMap &gs, double *w, const CgState *st, hipStream_t s) {
+  return gs_assemble_t(gs, w, st, s);
+}
+
+int gs_assemble(const GsMap &gs, float *w, const CgState *st, hipStream_t s) {
+  return gs_assemble_t(gs, w, st, s);
+}
+
+template <typename E, typename Z, typename W>
+static int gs_fin_t(const GsMap &gs, Z *z, const E *e, const W *w, hipStream_t s) {
   if (gs.ngroups == 0) return NKH_OK;
   const int nb = kGsBlocks;
   const int64_t chunk = (gs.ngroups + nb - 1) / nb;
-  hipLaunchKernelGGL(k_gs_fin<E>, dim3(nb), dim3(256), 0, s, z, e, w, gs.off, gs.idx,
+  hipLaunchKernelGGL((k_gs_fin<E, Z, W>), dim3(nb), dim3(256), 0, s, z, e, w, gs.off, gs.idx,
                      gs.ngroups, chunk);
   NK_HIP(hipGetLastError());
   return NKH_OK;
@@ -293,6 +303,10 @@ int gs_fin(const GsMap &gs, double *z, const double *e, const double *w, hipStre
 }
 
 int gs_fin(const GsMap &gs, double *z, const float *e, const double *w, hipStream_t s) {
+  return gs_fin_t(gs, z, e, w, s);
+}
+
+int gs_fin(const GsMap &gs, float *z, const float *e, const float *w, hipStream_t s) {
   return gs_fin_t(gs, z, e, w, s);
 }
 

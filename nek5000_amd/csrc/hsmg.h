@@ -45,6 +45,7 @@ struct MgLevel {
   double *e = nullptr, *r = nullptr;
   // single-precision copies for the mixed-precision smoother (fine level)
   float *fs32 = nullptr, *fl32 = nullptr, *wint32 = nullptr, *wgh32 = nullptr, *e32 = nullptr;
+  float *mswt32 = nullptr;  // mask * Schwarz weight, single precision
 };
 
 struct Mg {
@@ -70,6 +71,15 @@ struct Mg {
   // GMRES work: V (kGmresM+1 fields), Z (kGmresM fields), x, r, w, partials
   double *gV = nullptr, *gZ = nullptr, *gx = nullptr, *gr = nullptr, *gw = nullptr;
   double *gpart = nullptr, *gsm = nullptr;
+  // mixed-precision solver: single-precision inner vectors, the operator's
+  // geometric factors in single precision (rebuilt when the mesh geometry
+  // changes), partial sums and scalars
+  float *xr = nullptr, *xp = nullptr, *xz = nullptr, *xw = nullptr, *xx = nullptr,
+        *xwt = nullptr;
+  float *g32 = nullptr, *bm1_32 = nullptr;
+  int g32_version = -1;
+  double *xpart = nullptr;
+  void *xst = nullptr;
 };
 
 constexpr int kGmresM = 30;  // SIZE lgmres: Krylov dimension before a restart

@@ -56,9 +56,9 @@ struct Fdm {
 // and read there as wave-uniform operands.  Output: the interior of w, and
 // each ghost face value of w sent to the neighbour's slot it coincides with
 // (gout), where the overlap sum reads it.
-template <int NL, typename R>
+template <int NL, typename R, typename RI>
 __global__ void __launch_bounds__(Fdm<NL>::kThreads)
-    k_sch_fdm(R *__restrict__ wint, R *__restrict__ wgh, const double *__restrict__ r,
+    k_sch_fdm(R *__restrict__ wint, R *__restrict__ wgh, const RI *__restrict__ r,
               const int32_t *__restrict__ gin, const int32_t *__restrict__ gout,
               const R *__restrict__ S, const R *__restrict__ lam, int64_t nel) {
   constexpr int N = NL - 2, N2 = N * N, N3 = N2 * N, M2 = NL * NL, NLAM = 3 * NL + 1;
@@ -140,7 +140,7 @@ __global__ void __launch_bounds__(Fdm<NL>::kThreads)
     const int64_t x8 = ep % 8;
     const int64_t e = x8 * q8 + (x8 < r8 ? x8 : r8) + ep / 8;
     __syncthreads();
-    const double *re = r + e * N3;
+    const RI *re = r + e * N3;
     const int32_t *ge = gin + e * 6 * N2;
     const R *Se = S + e * 3 * M2;
 #pragma unroll
@@ -221,25 +221,60 @@ __global__ void __launch_bounds__(Fdm<NL>::kThreads)
 // coincides with it (delivered to this element's ghost slots by k_sch_fdm;
 // 0 where there is no neighbour), face by face in the order r-, r+, s-, s+,
 // t-, t+.
-template <typename R>
-__global__ void k_sch_out(R *__restrict__ eo, const R *__restrict__ wint,
-                          const R *__restrict__ wgh, int n, int64_t nel) {
-  const int64_t n2 = (int64_t)n * n, n3 = n2 * n;
-  for (int64_t l = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; l < n3 * nel;
-       l += (int64_t)gridDim.x * blockDim.x) {
-    const int64_t e = l / n3;
-    const int q = (int)(l - e * n3);
-    const int i = q % n, j = (q / n) % n, k = q / (int)n2;
-    const R *g = wgh + e * 6 * n2;
-    R v = wint[l];
-    if (i == 1) v += g[0 * n2 + j + n * k];
-    if (i == n - 2) v += g[1 * n2 + j + n * k];
-    if (j == 1) v += g[2 * n2 + i + n * k];
-    if (j == n - 2) v += g[3 * n2 + i + n * k];
-    if (k == 1) v += g[4 * n2 + i + n * j];
-    if (k == n - 2) v += g[5 * n2 + i + n * j];
-    eo[l] = v;
+// Blocks of 64 walk elements; the element size N is a compile-time constant
+// so the node coordinates cost no divisions, and each lane issues all its
+// loads of an element before using them.
+template <typename R, int N>
+__global__ void __launch_bounds__(64)
+    k_sch_out(R *__restrict__ eo, const R *__restrict__ wint, const R *__restrict__ wgh,
+              int64_t nel) {
+  constexpr int N2 = N * N, N3 = N2 * N, U = (N3 + 63) / 64;
+  for (int64_t e = blockIdx.x; e < nel; e += gridDim.x) {
+    const R *g = wgh + e * 6 * N2;
+    const R *wi = wint + e * N3;
+    R *out = eo + e * N3;
+    R v[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int q = (int)threadIdx.x + 64 * u;
+      v[u] = q < N3 ? wi[q] : R(0);
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int q = (int)threadIdx.x + 64 * u;
+      if (q >= N3) continue;
+      const int i = q % N, j = (q / N) % N, k = q / N2;
+      R a = v[u];
+      if (i == 1) a += g[0 * N2 + j + N * k];
+      if (i == N - 2) a += g[1 * N2 + j + N * k];
+      if (j == 1) a += g[2 * N2 + i + N * k];
+      if (j == N - 2) a += g[3 * N2 + i + N * k];
+      if (k == 1) a += g[4 * N2 + i + N * j];
+      if (k == N - 2) a += g[5 * N2 + i + N * j];
+      out[q] = a;
+    }
   }
+}
+
+template <typename R>
+int launch_sch_out(R *e, const R *wint, const R *wgh, int nh, int64_t nel, hipStream_t s) {
+  // a few elements per block: one small element per block is dominated by
+  // block launch
+  const int grid = (int)std::min<int64_t>(nel, 16384);
+  switch (nh) {
+#define NK_O(N)                                                                             \
+  case N:                                                                                   \
+    hipLaunchKernelGGL((k_sch_out<R, N>), dim3(grid), dim3(64), 0, s, e, wint, wgh, nel); \
+    break;
+    NK_O(3) NK_O(4) NK_O(5) NK_O(6) NK_O(7) NK_O(8) NK_O(9) NK_O(10) NK_O(11) NK_O(12)
+    NK_O(13) NK_O(14) NK_O(15) NK_O(16)
+#undef NK_O
+    default:
+      set_error("overlap sum of %d points not instantiated", nh);
+      return NKH_ERR_UNSUPPORTED;
+  }
+  NK_HIP(hipGetLastError());
+  return NKH_OK;
 }
 
 // Per-element tensor apply with one matrix in all directions,
@@ -249,8 +284,8 @@ __global__ void k_sch_out(R *__restrict__ eo, const R *__restrict__ wint,
 //           finer level): v = v * w + s.
 //   MODE 3 (prolongation into the fine level before gs_fin): v = s on
 //           element-boundary nodes, v = eb * w + s inside.
-template <int NV, int NU, int MODE, typename EB>
-__global__ void k_tnsr(double *__restrict__ v, const double *__restrict__ u,
+template <int NV, int NU, int MODE, typename EB, typename UT, typename VT>
+__global__ void k_tnsr(VT *__restrict__ v, const UT *__restrict__ u,
                        const double *__restrict__ w, const double *__restrict__ A, int64_t nel,
                        const EB *__restrict__ eb, const double *__restrict__ om) {
   constexpr int NV3 = NV * NV * NV, NU3 = NU * NU * NU;
@@ -262,7 +297,7 @@ __global__ void k_tnsr(double *__restrict__ v, const double *__restrict__ u,
   for (int64_t e = blockIdx.x; e < nel; e += gridDim.x) {
     __syncthreads();
     for (int t = threadIdx.x; t < NU3; t += blockDim.x) {
-      double x = u[(int64_t)e * NU3 + t];
+      double x = double(u[(int64_t)e * NU3 + t]);
       if constexpr (MODE == 1) x = x * w[(int64_t)e * NU3 + t];
       su[t] = x;
     }
@@ -287,13 +322,13 @@ __global__ void k_tnsr(double *__restrict__ v, const double *__restrict__ u,
       for (int m = 0; m < NU; ++m) s += s2[ij + NV * NV * m] * sA[k + NV * m];
       const int64_t o = (int64_t)e * NV3 + t;
       if constexpr (MODE == 2) {
-        v[o] = v[o] * w[o] + s;
+        v[o] = VT(double(v[o]) * w[o] + s);
       } else if constexpr (MODE == 3) {
         const int i = t % NV, j = (t / NV) % NV, k = t / (NV * NV);
         const bool bnd = (i == 0 || i == NV - 1 || j == 0 || j == NV - 1 || k == 0 || k == NV - 1);
-        v[o] = bnd ? s : double(eb[o]) * w[o] + s;
+        v[o] = VT(bnd ? s : double(eb[o]) * w[o] + s);
       } else {
-        v[o] = om ? s * om[o] : s;
+        v[o] = VT(om ? s * om[o] : s);
       }
     }
   }
@@ -330,28 +365,27 @@ __global__ void k_neq(const double *a, const double *b, int64_t n, int *flag) {
     if (a[i] != b[i]) atomicOr(flag, 1);
 }
 
-template <int NL, typename R>
-void launch_sch_fdm(R *wint, R *wgh, const R *S, const R *lam, const MgLevel &L, const double *r,
+template <int NL, typename R, typename RI>
+void launch_sch_fdm(R *wint, R *wgh, const R *S, const R *lam, const MgLevel &L, const RI *r,
                     int64_t nel, hipStream_t s) {
   const int grid = (int)std::min<int64_t>((nel + 7) / 8 * 8, 1 << 20);
-  hipLaunchKernelGGL((k_sch_fdm<NL, R>), dim3(grid), dim3(Fdm<NL>::kThreads), 0, s, wint, wgh, r,
-                     L.gin, L.gout, S, lam, nel);
+  hipLaunchKernelGGL((k_sch_fdm<NL, R, RI>), dim3(grid), dim3(Fdm<NL>::kThreads), 0, s, wint,
+                     wgh, r, L.gin, L.gout, S, lam, nel);
 }
 
 }  // namespace
 
 int mg_overlap_out(Mg &mg, int l, double *e, hipStream_t s) {
   MgLevel &L = mg.lev[l];
-  hipLaunchKernelGGL(k_sch_out<double>, dim3(mg_grid(L.n)), dim3(256), 0, s, e, L.wint, L.wgh,
-                     L.nh, mg.nel);
+  NK_TRY(launch_sch_out<double>(e, L.wint, L.wgh, L.nh, mg.nel, s));
   NK_HIP(hipGetLastError());
   return NKH_OK;
 }
 
 // Local solves + overlap sum in precision R: double uses the level's own
 // arrays, float the single-precision copies (fine level, mixed precision).
-template <typename R>
-static int schwarz_local(Mg &mg, int l, const double *r, R *e, hipStream_t s) {
+template <typename R, typename RI = double>
+static int schwarz_local(Mg &mg, int l, const RI *r, R *e, hipStream_t s) {
   MgLevel &L = mg.lev[l];
   R *wint, *wgh;
   const R *S, *lam;
@@ -363,7 +397,7 @@ static int schwarz_local(Mg &mg, int l, const double *r, R *e, hipStream_t s) {
   }
   switch (L.nh + 2) {
 #define NK_C(NL) \
-  case NL: launch_sch_fdm<NL, R>(wint, wgh, S, lam, L, r, mg.nel, s); break;
+  case NL: launch_sch_fdm<NL, R, RI>(wint, wgh, S, lam, L, r, mg.nel, s); break;
     NK_C(5) NK_C(6) NK_C(7) NK_C(8) NK_C(9) NK_C(10) NK_C(11) NK_C(12) NK_C(13) NK_C(14)
     NK_C(15) NK_C(16) NK_C(17) NK_C(18)
 #undef NK_C
@@ -372,8 +406,7 @@ static int schwarz_local(Mg &mg, int l, const double *r, R *e, hipStream_t s) {
       return NKH_ERR_UNSUPPORTED;
   }
   NK_HIP(hipGetLastError());
-  hipLaunchKernelGGL(k_sch_out<R>, dim3(mg_grid(L.n)), dim3(256), 0, s, e, wint, wgh, L.nh,
-                     mg.nel);
+  NK_TRY(launch_sch_out<R>(e, wint, wgh, L.nh, mg.nel, s));
   NK_HIP(hipGetLastError());
   return NKH_OK;
 }
@@ -382,15 +415,17 @@ int mg_schwarz_local(Mg &mg, int l, const double *r, double *e, hipStream_t s) {
   return schwarz_local<double>(mg, l, r, e, s);
 }
 
-// om (MODE 1 only, may be null): 0/1 mask applied to the output
-template <int MODE, typename EB = double>
-static int run_tnsr(double *v, int nv, const double *u, int nu, const double *w, const double *A,
+// om (MODE 1 only, may be null): 0/1 mask applied to the output.  UT/VT:
+// storage of the input / output field (arithmetic in FP64).
+template <int MODE, typename EB = double, typename UT = double, typename VT = double>
+static int run_tnsr(VT *v, int nv, const UT *u, int nu, const double *w, const double *A,
                     int64_t nel, hipStream_t s, const EB *eb = nullptr,
                     const double *om = nullptr) {
   const int grid = (int)std::min<int64_t>(nel, 1 << 20);
 #define NK_T(V, U)                                                                       \
   if (nv == V && nu == U) {                                                              \
-    hipLaunchKernelGGL((k_tnsr<V, U, MODE, EB>), dim3(grid), dim3(V * V * V > 256 ? 256 : 64), \
+    hipLaunchKernelGGL((k_tnsr<V, U, MODE, EB, UT, VT>), dim3(grid),                    \
+                       dim3(V * V * V > 256 ? 256 : 64),                                  \
                        0, s, v, u, w, A, nel, eb, om);                                   \
     NK_HIP(hipGetLastError());                                                           \
     return NKH_OK;                                                                       \
@@ -426,40 +461,56 @@ static int coarse_solve(Mg &mg, double *e1, const double *r1, int crs_mode, hipS
 
 // hsmg.f:1855-1949 H1MG_SOLVE (additive V-cycle, sigma = 1): z = M r, r
 // already masked with the fine-level mask.
-int mg_solve(Mesh &mesh, double *z, const double *r, hipStream_t s, int crs_mode,
-             int precision) {
+// The V-cycle with the fine-level input r and output z stored as FT: double
+// (the drop-in; precision 1 runs the fine local solves in FP32 inside), or
+// float (the single-precision inner solve of the mixed solver: fine level
+// in FP32 throughout, levels 2.. and the coarse solve in FP64).
+template <typename FT>
+static int mg_solve_t(Mesh &mesh, FT *z, const FT *r, hipStream_t s, int crs_mode,
+                      int precision) {
+  constexpr bool f32io = std::is_same<FT, float>::value;
   NK_CHECK(mesh.mg, NKH_ERR_STATE, "mesh has no multigrid hierarchy (nkh_mg_setup)");
   Mg &mg = *mesh.mg;
   const int Lm = mg.lmax;
   const int64_t nel = mg.nel;
   hipStream_t b = mg.sb;
-  NK_HIP(hipEventRecord(mg.ev_fork, s));
-  NK_HIP(hipStreamWaitEvent(b, mg.ev_fork, 0));
   // fine level on s: local solves and overlap sum.  With ids zero exactly
   // at element interiors, the dssum of the Schwarz sum, the weights, the
   // coarse correction and dsavg are done together after the join (gs_fin);
   // otherwise dssum now and the plain sequence after the join.
   MgLevel &T = mg.lev[Lm];
   const bool merged = mesh.gs.interior_zero;
-  const bool fp32 = precision == 1 && merged;
-  if (fp32) {
-    NK_TRY(mg_prepare_fp32(mg, s));
-    NK_TRY(schwarz_local<float>(mg, Lm, r, T.e32, s));
-  } else {
-    NK_TRY(schwarz_local<double>(mg, Lm, r, merged ? T.e : z, s));
+  NK_CHECK(merged || !f32io, NKH_ERR_UNSUPPORTED,
+           "single-precision V-cycle needs global ids zero exactly at element interiors");
+  const bool fp32 = (precision == 1 || f32io) && merged;
+  if (fp32) NK_TRY(mg_prepare_fp32(mg, s));
+  // the restriction of the fine residual runs first, on s: beside the fine
+  // local solves it would be starved of the compute units they occupy
+  if (Lm - 1 >= 2) {
+    MgLevel &C = mg.lev[Lm - 1];
+    // mask * dssum(v) = dssum(mask * v): the level mask is 0/1 and equal at
+    // every copy of a node (made so by a product gs at set-up)
+    NK_TRY((run_tnsr<1, double, FT, double>(C.r, C.nh, r, T.nh, T.rwt, C.Jt, nel, s,
+                                            (const double *)nullptr, C.mask)));
   }
-  if (!merged) NK_TRY(gs_op(T.gs, z, 0, s));
+  NK_HIP(hipEventRecord(mg.ev_fork, s));
+  NK_HIP(hipStreamWaitEvent(b, mg.ev_fork, 0));
+  if (fp32) {
+    NK_TRY((schwarz_local<float, FT>(mg, Lm, r, T.e32, s)));
+  } else if constexpr (!f32io) {
+    NK_TRY((schwarz_local<double, FT>(mg, Lm, r, merged ? T.e : z, s)));
+    if (!merged) NK_TRY(gs_op(T.gs, z, 0, s));
+  }
   // coarser levels on b: restrict, dssum, mask; the Schwarz solve of each
   // intermediate level on c (it only reads the masked residual), beside the
   // restriction to level 1 and the coarse solve on b
   hipStream_t c = mg.sc;
-  const double *rsrc = r;
+  const double *rsrc = nullptr;  // restriction source below the fine level
   for (int l = Lm - 1; l >= 2; --l) {
     MgLevel &C = mg.lev[l], &F = mg.lev[l + 1];
-    // mask * dssum(v) = dssum(mask * v): the level mask is 0/1 and equal at
-    // every copy of a node (made so by a product gs at set-up)
-    NK_TRY(run_tnsr<1>(C.r, C.nh, rsrc, F.nh, F.rwt, C.Jt, nel, b, (const double *)nullptr,
-                       C.mask));
+    if (l < Lm - 1)  // (the level below the fine one was restricted above)
+      NK_TRY(run_tnsr<1>(C.r, C.nh, rsrc, F.nh, F.rwt, C.Jt, nel, b, (const double *)nullptr,
+                         C.mask));
     NK_TRY(gs_op(C.gs, C.r, 0, b));
     NK_HIP(hipEventRecord(mg.ev_lr[l], b));
     NK_HIP(hipStreamWaitEvent(c, mg.ev_lr[l], 0));
@@ -472,7 +523,10 @@ int mg_solve(Mesh &mesh, double *z, const double *r, hipStream_t s, int crs_mode
   // the level-1 mask needs no pass of its own: the coarse gather reads only
   // unmasked vertices and the scatter writes 0 at masked ones (their slot is
   // -1), which is what masking r1 before and e1 after the solve would give
-  NK_TRY(run_tnsr<1>(L1.r, L1.nh, rsrc, L2.nh, L2.rwt, L1.Jt, nel, b));
+  if (Lm == 2)
+    NK_TRY((run_tnsr<1, double, FT, double>(L1.r, L1.nh, r, L2.nh, L2.rwt, L1.Jt, nel, b)));
+  else
+    NK_TRY(run_tnsr<1>(L1.r, L1.nh, rsrc, L2.nh, L2.rwt, L1.Jt, nel, b));
   NK_TRY(coarse_solve(mg, L1.e, L1.r, crs_mode, b));
   for (int l = 2; l < Lm; ++l) {  // e_l = e_l * (mask*wt) + J e_{l-1}
     MgLevel &C = mg.lev[l - 1], &F = mg.lev[l];
@@ -484,16 +538,28 @@ int mg_solve(Mesh &mesh, double *z, const double *r, hipStream_t s, int crs_mode
   // z = dssum(e)*(mask*wt) + J e_{lmax-1}, then dsavg (ic.f:1871-1897)
   MgLevel &C = mg.lev[Lm - 1];
   if (fp32) {
-    NK_TRY((run_tnsr<3, float>(z, T.nh, C.e, C.nh, T.mswt, C.J, nel, s, T.e32)));
-    NK_TRY(gs_fin(mesh.gs, z, T.e32, T.mswt, s));
-  } else if (merged) {
-    NK_TRY(run_tnsr<3>(z, T.nh, C.e, C.nh, T.mswt, C.J, nel, s, T.e));
-    NK_TRY(gs_fin(mesh.gs, z, T.e, T.mswt, s));
-  } else {
-    NK_TRY(run_tnsr<2>(z, T.nh, C.e, C.nh, T.mswt, C.J, nel, s));
-    NK_TRY(gs_op(mesh.gs, z, 4, s));
+    NK_TRY((run_tnsr<3, float, double, FT>(z, T.nh, C.e, C.nh, T.mswt, C.J, nel, s, T.e32)));
+    if constexpr (f32io) NK_TRY(gs_fin(mesh.gs, z, T.e32, T.mswt32, s));
+    else NK_TRY(gs_fin(mesh.gs, z, T.e32, T.mswt, s));
+  } else if constexpr (!f32io) {
+    if (merged) {
+      NK_TRY(run_tnsr<3>(z, T.nh, C.e, C.nh, T.mswt, C.J, nel, s, T.e));
+      NK_TRY(gs_fin(mesh.gs, z, T.e, T.mswt, s));
+    } else {
+      NK_TRY(run_tnsr<2>(z, T.nh, C.e, C.nh, T.mswt, C.J, nel, s));
+      NK_TRY(gs_op(mesh.gs, z, 4, s));
+    }
   }
   return NKH_OK;
+}
+
+int mg_solve(Mesh &mesh, double *z, const double *r, hipStream_t s, int crs_mode,
+             int precision) {
+  return mg_solve_t(mesh, z, r, s, crs_mode, precision);
+}
+
+int mg_solve32(Mesh &mesh, float *z, const float *r, hipStream_t s, int crs_mode) {
+  return mg_solve_t(mesh, z, r, s, crs_mode, 1);
 }
 
 int mg_mask_fine(Mesh &mesh, double *r, hipStream_t s) {

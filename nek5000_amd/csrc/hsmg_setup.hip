@@ -482,6 +482,8 @@ int mg_prepare_fp32(Mg &mg, hipStream_t s) {
   NK_TRY(falloc(&L.wint32, L.n));
   NK_TRY(falloc(&L.wgh32, ngh));
   NK_TRY(falloc(&L.e32, L.n));
+  NK_TRY(falloc(&L.mswt32, L.n));
+  hipLaunchKernelGGL(k_to_f32, dim3(mg_grid(L.n)), dim3(256), 0, s, L.mswt32, L.mswt, L.n);
   hipLaunchKernelGGL(k_to_f32, dim3(mg_grid(ns)), dim3(256), 0, s, L.fs32, L.fs, ns);
   hipLaunchKernelGGL(k_to_f32, dim3(mg_grid(nlam)), dim3(256), 0, s, L.fl32, L.fl, nlam);
   // ghost slots without a neighbour are never written: 0
@@ -495,7 +497,7 @@ static void free_level(MgLevel &L) {
     if (p) (void)hipFree(p);
   for (int32_t *p : {L.gin, L.gout})
     if (p) (void)hipFree(p);
-  for (float *p : {L.fs32, L.fl32, L.wint32, L.wgh32, L.e32})
+  for (float *p : {L.fs32, L.fl32, L.wint32, L.wgh32, L.e32, L.mswt32})
     if (p) (void)hipFree(p);
   if (L.own_gs) gs_free(L.gs);
 }
@@ -511,7 +513,10 @@ void mg_free(Mesh &mesh) {
                   (void *)mg->cb, (void *)mg->cx, (void *)mg->fr, (void *)mg->fp,
                   (void *)mg->fz, (void *)mg->fw, (void *)mg->fpart, (void *)mg->fst,
                   (void *)mg->fflag, (void *)mg->gV, (void *)mg->gZ, (void *)mg->gx,
-                  (void *)mg->gr, (void *)mg->gw, (void *)mg->gpart, (void *)mg->gsm})
+                  (void *)mg->gr, (void *)mg->gw, (void *)mg->gpart, (void *)mg->gsm,
+                  (void *)mg->xr, (void *)mg->xp, (void *)mg->xz, (void *)mg->xw,
+                  (void *)mg->xx, (void *)mg->xwt, (void *)mg->g32, (void *)mg->bm1_32,
+                  (void *)mg->xpart, mg->xst})
     if (p) (void)hipFree(p);
   if (mg->ev_fork) (void)hipEventDestroy(mg->ev_fork);
   if (mg->ev_join) (void)hipEventDestroy(mg->ev_join);

@@ -108,6 +108,7 @@ struct Mesh {
   double *binv = nullptr;      // device
   double vol = 0.0;            // sum of bm1
   bool has_geom = false;
+  int geom_version = 0;        // bumped whenever g/bm1 change
   GsMap gs;
   bool has_gs = false;
   // solve workspace (allocated on first solve)
@@ -175,10 +176,12 @@ int gs_op(const GsMap &gs, double *u, int op, hipStream_t s);
 // CG assembly: w <- mask * dssum(w) using the flags of gs_prepare; skipped
 // when st is given and st->done.
 int gs_assemble(const GsMap &gs, double *w, const CgState *st, hipStream_t s);
+int gs_assemble(const GsMap &gs, float *w, const CgState *st, hipStream_t s);
 // Per id group (single members included): S = sum of e; z_i <- mean over
 // the members of S*w_i + z_i.  The fine-level end of the multigrid V-cycle.
 int gs_fin(const GsMap &gs, double *z, const double *e, const double *w, hipStream_t s);
 int gs_fin(const GsMap &gs, double *z, const float *e, const double *w, hipStream_t s);
+int gs_fin(const GsMap &gs, float *z, const float *e, const float *w, hipStream_t s);
 // Flags per group from mask/mult; *ok when every group has a 0/1 mask equal
 // at all members and mult summing to 1 (red: kRedBlocks scratch).
 int gs_prepare(GsMap &gs, const double *mask, const double *mult, double *red, bool *ok,
@@ -211,6 +214,9 @@ void mg_free(Mesh &m);
 // sum) in FP32, everything else FP64
 int mg_solve(Mesh &m, double *z, const double *rhs, hipStream_t s, int crs_mode = 0,
              int precision = 0);
+// the same V-cycle with the fine level in single precision throughout
+// (input r and output z float; levels 2.. and the coarse solve FP64)
+int mg_solve32(Mesh &m, float *z, const float *r, hipStream_t s, int crs_mode);
 int mg_mask_fine(Mesh &m, double *r, hipStream_t s);
 int mg_info(const Mesh &m, int *lmax, int *nh, int *ncrs, double *crs_avg_iters);
 int run_flex_cg(Mesh &m, double *res, Coef h1, Coef h2, const double *wt, const double *mask,
@@ -235,6 +241,12 @@ int proj_project1(int id, double *b, const double *h1, const double *h2, const d
                   const double *w, int ifmvbd, double *norms, int *ireset_out);
 int proj_project2(int id, double *x, const double *h1, const double *h2, const double *msk,
                   const double *w);
+// ---- mixed.hip ----
+// hmh_flex_cg to the same FP64 tolerance by defect correction: FP64 true
+// residuals around inner flexible-CG solves in single precision.
+int run_flex_cg_mixed(Mesh &m, double *res, Coef h1, Coef h2, const double *wt,
+                      const double *mask, int maxit, const nkh_params &prm, nkh_result *out,
+                      double *rnorm_history);
 // ---- gmres.hip ----
 int run_gmres(Mesh &m, double *res, Coef h1, Coef h2, const double *wt, const double *mask,
               int maxit, const nkh_params &prm, nkh_result *out, double *rnorm_history);

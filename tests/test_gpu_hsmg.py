@@ -289,3 +289,52 @@ def test_flex_cg_fp32_smoother(gpu, case):
         assert out["niter"] <= ito + 3
         assert l2rel(host(u), uo) < 1e-8
     F.destroy()
+
+
+@pytest.mark.parametrize("case", [MG_CASES[0], MG_CASES[1], MG_CASES[6]])
+@pytest.mark.parametrize("param40", [0, 1])
+def test_flex_cg_mixed_defect_correction(gpu, case, param40):
+    """precision = 2: FP64 defect correction around single-precision inner
+    flexible-CG solves (FP32 operator, fine-level V-cycle and vectors; FP64
+    inner products and coarse solve).  The true FP64 residual, recomputed by
+    the oracle's operator on the host, meets the FP64 tolerance; the solution
+    agrees with the oracle's FP64 solution to that tolerance; the inner
+    iterations (restarted per correction) stay within a few of the FP64
+    count."""
+    P, mg, F = build(gpu, case)
+    h1 = np.ones(P.shape)
+    h2 = np.full(P.shape, 10.0)
+    b = P.bm1 * P.uniform_field(61)
+    rhs = P.field.dssum(b) * P.mask
+    tol = 1e-10
+    uo, ito, _, div0 = mg.flex_cg(rhs, h1, h2, 300, -tol)
+    u = F.zeros()
+    out = F.hmholtz("PRES", u, F.to_dev(b), F.to_dev(h1), F.to_dev(h2), F.mask, F.mult, -tol,
+                    300, params=gpu.default_params(param42=2, param21=-tol, precision=2,
+                                                   param40=param40))
+    ug = host(u)
+    r = rhs - P.field.dssum(P.field.axhelm(ug, h1, h2)) * P.mask
+    rn = float(np.sqrt(np.sum(r * r * P.mult) / P.field.vol))
+    say(f"[mixed] {case} param40={param40}: inner iterations {out['niter']} (FP64 oracle {ito}), "
+        f"true residual {rn / div0:.2e} of the initial")
+    assert out["converged"]
+    assert abs(out["rbn0"] - div0) <= 1e-12 * div0
+    assert rn < tol * div0 * 1.0001
+    assert abs(out["rbn2"] - rn) <= 1e-6 * rn + 1e-15 * div0
+    if param40 == 0:
+        assert out["niter"] <= ito + 8
+    assert l2rel(ug, uo) < 1e-7
+    F.destroy()
+
+
+def test_flex_cg_mixed_needs_uniform_coefficients(gpu):
+    P, mg, F = build(gpu, MG_CASES[0])
+    h1 = 1.0 + 0.1 * P.uniform_field(3)
+    u = F.zeros()
+    with pytest.raises(gpu._lib.NekAmdError, match="uniform"):
+        F.hmholtz("PRES", u, F.to_dev(P.bm1), F.to_dev(h1), F.zeros(), F.mask, F.mult, -1e-8,
+                  50, params=gpu.default_params(param42=2, param21=-1e-8, precision=2))
+    with pytest.raises(gpu._lib.NekAmdError, match="precision"):
+        F.hmholtz("PRES", u, F.to_dev(P.bm1), F.zeros() + 1.0, F.zeros(), F.mask, F.mult, -1e-8,
+                  50, params=gpu.default_params(param42=0, param21=-1e-8, precision=2))
+    F.destroy()
