@@ -134,13 +134,25 @@ def order_sweep():
         free = float(torch.sum(F.mask * F.mult))
         setup = time.time() - t0
         au = torch.zeros_like(rhs)
-        F.axhelm(au, rhs, one, one)
+        F.axhelm(au, rhs, one, one)  # the first apply of the order tunes it
         torch.cuda.synchronize()
-        t = time.perf_counter()
-        for _ in range(5):
+        chosen = solver.axhelm_tuning(N + 1)
+
+        def time_apply():
             F.axhelm(au, rhs, one, one)
-        torch.cuda.synchronize()
-        ax_ms = (time.perf_counter() - t) / 5 * 1e3
+            torch.cuda.synchronize()
+            t = time.perf_counter()
+            for _ in range(5):
+                F.axhelm(au, rhs, one, one)
+            torch.cuda.synchronize()
+            return (time.perf_counter() - t) / 5 * 1e3
+
+        per_variant = {}
+        for v in (0, 1):
+            solver.set_axhelm_tuning(N + 1, v, 0)
+            per_variant[f"variant{v}_ms"] = time_apply()
+        solver.set_axhelm_tuning(N + 1, chosen[0], chosen[1])
+        ax_ms = time_apply()
         # u, au, g1..g6, bm1 (h2 uniform nonzero): 9 words per local node
         ax_gbs = 9 * 8 * box.n / (ax_ms * 1e-3) / 1e9
         u = torch.zeros_like(rhs)
@@ -151,6 +163,7 @@ def order_sweep():
         torch.cuda.synchronize()
         pcg_s = time.perf_counter() - t
         row = dict(N=N, E=box.nel, local_nodes=box.n, free_dof=free, setup_s=round(setup, 1),
+                   axhelm_variant=chosen[0], axhelm_grid=chosen[1], **per_variant,
                    axhelm_ms=ax_ms, axhelm_GBs=ax_gbs, axhelm_hbm_frac=ax_gbs / HBM_PEAK_GBS,
                    pcg_iterations=out["niter"], pcg_ms=pcg_s * 1e3,
                    pcg_GDOF_iter_s=free * out["niter"] / pcg_s / 1e9)
@@ -160,6 +173,149 @@ def order_sweep():
         del F, rhs, one, au, u, bm1
         torch.cuda.empty_cache()
     print(json.dumps({"sweep": "order", "precision": "fp64", "rows": rows}), flush=True)
+
+
+def adjoint_run(nel1d=32, lx1=8):
+    """Adjoint coefficient gradient at E = 32^3, N = 7: -div(h1 grad u) = 1
+    (h2 = 0, Dirichlet), h1 = exp(0.5 * sum of 16 Fourier modes with
+    wavenumbers 1..4 / 4), J = 1/2 (u - u*)^T B (u - u*) with u* ~ U(-1,1)/2.
+    Forward and adjoint by the flexible CG with the h1mg V-cycle to 1e-12
+    relative (the same solver for both: the operator is self-adjoint), then
+    the gradient kernel; central differences (step 1e-5) along two
+    directions check it.  Prints one JSON line."""
+    import torch
+    from nek5000_amd import solver
+    from nek5000_amd.mesh import BoxMesh, hash_uniform
+
+    solver.init(0)
+    box = BoxMesh(nel1d, nel1d, nel1d, lx1, deform=0.1)
+    F = solver.box_field(box, multigrid=True)
+    x, y, z = box.xyz()
+    rng = np.random.default_rng(5)
+    s = np.zeros_like(x)
+    for _ in range(16):
+        k = rng.integers(1, 5, size=3)
+        ph = rng.uniform(0, 2 * np.pi, size=3)
+        s += np.cos(np.pi * k[0] * x + ph[0]) * np.cos(np.pi * k[1] * y + ph[1]) * \
+            np.cos(np.pi * k[2] * z + ph[2])
+    h1 = F.to_dev(np.exp(0.5 * s / 4.0))
+    del x, y, z, s
+    h2 = F.zeros()
+    bm1, _ = F.mass()
+    f = bm1.clone()
+    uid = box.unique_ids(F.glo_num, F.ngv).reshape(-1)
+    ustar = F.to_dev((hash_uniform(7, uid) - 0.5).reshape(box.nel, lx1, lx1, lx1)) * F.mask
+    prm = solver.default_params(istep=20, param42=2, param21=-1e-12)
+
+    def hsolve(rhs, h):
+        u = F.zeros()
+        out = F.hmholtz("PRES", u, rhs.clone(), h, h2, F.mask, F.mult, -1e-12, 300, params=prm)
+        assert out["converged"]
+        return u, out["niter"]
+
+    def J(h):
+        u, _ = hsolve(f, h)
+        d = u - ustar
+        return 0.5 * float(torch.sum(bm1 * d * d))
+
+    hsolve(f, h1)  # warm-up
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    u, it_f = hsolve(f, h1)
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    lam, it_a = hsolve(bm1 * (u - ustar), h1)
+    torch.cuda.synchronize()
+    t2 = time.perf_counter()
+    g = F.zeros()
+    F.adjoint_coef_grad(g, u, lam)
+    torch.cuda.synchronize()
+    t3 = time.perf_counter()
+    for _ in range(5):
+        F.adjoint_coef_grad(g, u, lam)
+    torch.cuda.synchronize()
+    kern_ms = (time.perf_counter() - t3) / 5 * 1e3
+    checks = []
+    for seed in (11, 12):
+        w = F.to_dev(hash_uniform(seed, uid).reshape(box.nel, lx1, lx1, lx1))
+        dh = (0.5 + w) * h1
+        eps = 1e-5
+        fd = (J(h1 + eps * dh) - J(h1 - eps * dh)) / (2 * eps)
+        ad = float(torch.sum(g * dh))
+        checks.append(dict(adjoint=ad, central_difference=fd, rel_diff=abs(fd - ad) / abs(ad)))
+    line = dict(run="adjoint", workload=f"E={nel1d}^3 N={lx1 - 1} deformed box, h1=exp(Fourier)",
+                forward_ms=(t1 - t0) * 1e3, forward_iterations=it_f,
+                adjoint_ms=(t2 - t1) * 1e3, adjoint_iterations=it_a,
+                gradient_kernel_ms=kern_ms,
+                gradient_kernel_GBs=9 * 8 * box.n / (kern_ms * 1e-3) / 1e9,
+                finite_difference_checks=checks)
+    print(json.dumps(line), flush=True)
+
+
+def batch_run(steps=2):
+    """Five solves of one time step solved together (nkd_hmholtz_batch, the
+    iteration of all of them in one hipGraph): three velocity components on
+    a 48^3 box (h1 = 1e-3, h2 = 11/(6e-3)), a passive scalar on 24^3 (h1 =
+    1e-3) and one on a 40x40x20 x-periodic box (h1 = 1e-2); N = 7, f ~
+    U(-1,1) per unique node, Jacobi PCG to 1e-8 relative each, FP64.  The
+    same solves one after another through hmholtz for comparison.  Prints one
+    JSON line."""
+    import torch
+    from nek5000_amd import solver
+    from nek5000_amd.mesh import BoxMesh
+
+    solver.init(0)
+    specs = [("vel", BoxMesh(48, 48, 48, 8, deform=0.1), 1e-3, 11.0 / (6.0 * 1e-3)),
+             ("A", BoxMesh(24, 24, 24, 8, deform=0.1), 1e-3, 0.0),
+             ("B", BoxMesh(40, 40, 20, 8, periodic_x=True), 1e-2, 0.0)]
+    fields, solves, dofs = [], [], []
+    for name, box, h1v, h2v in specs:
+        F = solver.box_field(box)
+        bm1, _ = F.mass()
+        free = float(torch.sum(F.mask * F.mult))
+        ncomp = 3 if name == "vel" else 1
+        for c in range(ncomp):
+            f = make_rhs(box, F.glo_num, F.ngv, seed=100 + len(solves))
+            rhs = bm1 * torch.as_tensor(f, device="cuda")
+            fields.append(F)
+            solves.append(dict(u=F.zeros(), rhs=rhs, h1=F.zeros() + h1v, h2=F.zeros() + h2v,
+                               mask=F.mask, mult=F.mult, tli=-1e-8))
+            dofs.append(free)
+    prm = solver.default_params(istep=20, param22=0.0, restol=0.0)
+    B = solver.Batch(fields)
+
+    def run_batch():
+        qs = [dict(q, rhs=q["rhs"].clone()) for q in solves]
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        outs, launches = B.hmholtz(qs, 900, params=prm)
+        torch.cuda.synchronize()
+        return time.perf_counter() - t, outs, launches
+
+    def run_seq():
+        qs = [dict(q, rhs=q["rhs"].clone()) for q in solves]
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        outs = [F.hmholtz("VELX", q["u"], q["rhs"], q["h1"], q["h2"], q["mask"], q["mult"],
+                          q["tli"], 900, params=prm) for F, q in zip(fields, qs)]
+        torch.cuda.synchronize()
+        return time.perf_counter() - t, outs
+
+    run_batch()
+    run_seq()
+    tb = min(run_batch()[0] for _ in range(steps))
+    _, outs, launches = run_batch()
+    ts = min(run_seq()[0] for _ in range(steps))
+    its = [o["niter"] for o in outs]
+    work = sum(d * i for d, i in zip(dofs, its))
+    line = dict(run="batch", workload="3 x velocity 48^3 + scalar 24^3 + scalar 40x40x20 "
+                                      "periodic, N=7, Jacobi PCG to 1e-8 relative, FP64",
+                iterations=its, graph_launches=launches,
+                batched_s=tb, sequential_s=ts, speedup=ts / tb,
+                batched_GDOF_iter_s=work / tb / 1e9, sequential_GDOF_iter_s=work / ts / 1e9,
+                converged=[bool(o["converged"]) for o in outs])
+    print(json.dumps(line), flush=True)
+    B.destroy()
 
 
 def main():
@@ -182,6 +338,12 @@ def main():
                     help="main line's nkh_params.precision (profiling the mixed solvers)")
     ap.add_argument("--no-variants", action="store_true",
                     help="skip the extra short run of the other coarse solver")
+    ap.add_argument("--batch", action="store_true",
+                    help="instead of the bench line: five solves of a time step batched in one "
+                         "graph vs one after another")
+    ap.add_argument("--adjoint", action="store_true",
+                    help="instead of the bench line: adjoint gradient timing and finite-"
+                         "difference check at E=32^3, N=7")
     ap.add_argument("--sweep", action="store_true",
                     help="instead of the bench line: polynomial-order sweep N=3..15 of the "
                          "FP64 axhelm and 100 Jacobi-PCG iterations")
@@ -197,6 +359,12 @@ def main():
         sys.exit(2)
     if args.sweep:
         order_sweep()
+        return
+    if args.adjoint:
+        adjoint_run()
+        return
+    if args.batch:
+        batch_run()
         return
 
     import torch

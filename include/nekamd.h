@@ -151,6 +151,15 @@ int nkh_mesh_ptrs(int mesh, const double **bm1, const double **binvm1);
    isd is accepted for signature parity; only the non-axisymmetric path exists. */
 int nkd_axhelm(int mesh, double *au, const double *u, const double *h1, const double *h2,
                int isd);
+/* Kernel choice for the AXHELM applies of order lx1 (no counterpart in the
+   reference, whose axhelm has one form): variant 0 stages the element in LDS,
+   1 stages one t-plane at a time (3 lx1^2 instead of 3 lx1^3 doubles of LDS);
+   both give identical results.  grid = blocks of the plain apply (0: one per
+   element group).  Unless set, the first nkd_axhelm or solve of an order on
+   a mesh of >= 2^20 nodes times both variants at two grids and keeps the
+   fastest (*tuned = 1 afterwards). */
+int nkh_axhelm_tuning(int lx1, int *variant, int *grid, int *tuned);
+int nkh_axhelm_set_tuning(int lx1, int variant, int grid);
 /* dssum.f:33-98 DSSUM: u_i <- sum over nodes sharing glo_num(i). */
 int nkd_dssum(int mesh, double *u);
 /* dssum.f:100-161 DSOP: op '+', '*', 'm' (min), 'M' (max). */
@@ -245,6 +254,32 @@ int nkd_hsolve(int mesh, const char *name, double *u, double *r, const double *h
 /* Hierarchy facts: number of levels, points per direction per level (nh[0]
    = level 1), coarse unknowns, mean coarse CG iterations per solve so far. */
 int nkh_mg_info(int mesh, int *lmax, int *nh, int *ncrs, double *crs_avg_iters);
+
+/* ---- batched solves --------------------------------------------------- */
+
+/* Several hmholtz solves at once (no reference counterpart: Nek solves its
+   velocity components and passive scalars one after another through
+   hmholtz.f:2-69): a batch fixes the meshes of its solves (nb <= 64; a mesh
+   may appear more than once, e.g. the three velocity components) and owns a
+   work space, state and stream per solve. */
+int nkh_batch_create(int nb, const int *meshes, int *batch);
+int nkh_batch_destroy(int batch);
+/* One solve of a batch: hmholtz's arguments (device arrays). */
+typedef struct {
+  double *u;           /* solution (out) */
+  double *rhs;         /* right-hand side, dssum'd and masked in place */
+  const double *h1, *h2, *mask, *mult;
+  double tli;          /* tolerance, < 0 relative, as hmholtz */
+} nkh_batch_solve;
+/* hmholtz.f:2-69 + the Jacobi branch of cggo (:611-846) for every solve of
+   the batch (non-PRES names): the set-up of each solve in order, then all
+   solves iterate together -- the steady-state iteration of the whole batch
+   (five launches per solve, each solve on its own stream) is captured once
+   in a hipGraph and replayed until every solve has stopped.  Iterates and
+   results equal those of nkd_hmholtz on each solve.  res: nb results;
+   *graph_launches (optional): iterations run for the slowest solve. */
+int nkd_hmholtz_batch(int batch, const nkh_batch_solve *solves, int maxit,
+                      const nkh_params *prm, nkh_result *res, int *graph_launches);
 
 /* hmholtz.f:611-846 CGGO (Jacobi branch): f is the assembled, masked rhs
    (left unchanged), binv = binvm1/bintm1 of the mesh. */

@@ -38,6 +38,13 @@ class NkhParams(ctypes.Structure):
     ]
 
 
+class NkhBatchSolve(ctypes.Structure):
+    _fields_ = [
+        ("u", c_void_p), ("rhs", c_void_p), ("h1", c_void_p), ("h2", c_void_p),
+        ("mask", c_void_p), ("mult", c_void_p), ("tli", c_double),
+    ]
+
+
 class NkhResult(ctypes.Structure):
     _fields_ = [
         ("niter", c_int),
@@ -73,6 +80,8 @@ SIGNATURES = {
     "nkh_mesh_info": (c_int, [c_int, P(c_int64), P(c_double), P(c_int64), P(c_int64), P(c_int)]),
     "nkh_mesh_ptrs": (c_int, [c_int, P(c_void_p), P(c_void_p)]),
     "nkd_axhelm": (c_int, [c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_int]),
+    "nkh_axhelm_tuning": (c_int, [c_int, P(c_int), P(c_int), P(c_int)]),
+    "nkh_axhelm_set_tuning": (c_int, [c_int, c_int, c_int]),
     "nkd_dssum": (c_int, [c_int, c_void_p]),
     "nkd_dsop": (c_int, [c_int, c_void_p, c_char]),
     "nkd_setprec": (c_int, [c_int, c_void_p, c_void_p, c_void_p]),
@@ -80,6 +89,9 @@ SIGNATURES = {
     "nkd_hmholtz": (c_int, [c_int, c_char_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
                             c_void_p, c_double, c_int, c_int, P(NkhParams), P(NkhResult),
                             c_void_p]),
+    "nkh_batch_create": (c_int, [c_int, c_void_p, P(c_int)]),
+    "nkh_batch_destroy": (c_int, [c_int]),
+    "nkd_hmholtz_batch": (c_int, [c_int, c_void_p, c_int, P(NkhParams), c_void_p, P(c_int)]),
     "nkd_cggo": (c_int, [c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
                          c_double, c_int, P(NkhParams), P(NkhResult), c_void_p]),
     "nkh_mg_setup": (c_int, [c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]),

@@ -734,6 +734,14 @@ int nkd_axhelm(int id, double *au, const double *u, const double *h1, const doub
   return NKH_OK;
 }
 
+int nkh_axhelm_tuning(int lx1, int *variant, int *grid, int *tuned) {
+  return axhelm_tuning(lx1, variant, grid, tuned);
+}
+
+int nkh_axhelm_set_tuning(int lx1, int variant, int grid) {
+  return axhelm_set_tuning(lx1, variant, grid);
+}
+
 int nkd_dssum(int id, double *u) {
   Mesh *m = get_mesh(id);
   NK_CHECK(g_init && m && m->has_gs, NKH_ERR_STATE, "nkd_dssum: mesh %d not ready", id);
@@ -886,6 +894,51 @@ int nkd_hsolve(int id, const char *name, double *u, double *r, const double *h1,
   NK_TRY(run_chktcg1(*m, &t, r, c1, c2, vmk, vml));
   NK_TRY(cggo_branches(*m, pres, u, r, c1, c2, vmk, vml, t, maxit, p, res, history, s));
   return proj_project2(proj, u, h1, h2, vmk, vml);
+}
+
+int nkh_batch_create(int nb, const int *meshes, int *batch) {
+  NK_CHECK(g_init, NKH_ERR_STATE, "nkh_batch_create: library not initialised");
+  return batch_create(nb, meshes, batch);
+}
+int nkh_batch_destroy(int batch) { return batch_destroy(batch); }
+
+int nkd_hmholtz_batch(int batch, const nkh_batch_solve *solves, int maxit,
+                      const nkh_params *prm, nkh_result *res, int *graph_launches) {
+  NK_CHECK(g_init, NKH_ERR_STATE, "nkd_hmholtz_batch: library not initialised");
+  NK_CHECK(solves && res && maxit >= 0, NKH_ERR_ARG, "nkd_hmholtz_batch: bad arguments");
+  const nkh_params &p = prm ? *prm : kDefaultParams;
+  NK_CHECK(p.precision == 0, NKH_ERR_UNSUPPORTED, "nkd_hmholtz_batch: precision=%d",
+           p.precision);
+  int nb = 0;
+  std::vector<int> meshes;
+  NK_TRY(batch_meshes(batch, &meshes));
+  nb = (int)meshes.size();
+  hipStream_t s = g_stream;
+  std::vector<const double *> x(nb), f(nb), mk(nb), mu(nb);
+  std::vector<Coef> c1(nb), c2(nb);
+  std::vector<double> tin(nb);
+  for (int k = 0; k < nb; ++k) {
+    Mesh *m = get_mesh(meshes[k]);
+    NK_TRY(ready(m, meshes[k], true));
+    const nkh_batch_solve &q = solves[k];
+    NK_CHECK(q.u && q.rhs && q.h1 && q.h2 && q.mask && q.mult, NKH_ERR_ARG,
+             "nkd_hmholtz_batch: null array in solve %d", k);
+    NK_TRY(classify_coefs(*m, q.h1, q.h2, &c1[k], &c2[k], s));
+    // hmholtz.f:50-57: dssum(rhs); rhs *= mask; chktcg1; tli < 0 relative
+    NK_TRY(gs_op(m->gs, q.rhs, 0, s));
+    NK_TRY(mul_inplace(q.rhs, q.mask, m->n, s));
+    double tol = std::fabs(q.tli);
+    if (p.param22 == 0.0 || p.istep <= 10)
+      NK_TRY(run_chktcg1(*m, &tol, q.rhs, c1[k], c2[k], q.mask, q.mult));
+    if (q.tli < 0) tol = q.tli;
+    x[k] = q.u;
+    f[k] = q.rhs;
+    mk[k] = q.mask;
+    mu[k] = q.mult;
+    tin[k] = tol;
+  }
+  return run_cggo_batch(batch, x.data(), f.data(), c1.data(), c2.data(), mk.data(), mu.data(),
+                        tin.data(), maxit, p, res, graph_launches);
 }
 
 int nkh_re2_header(const char *path, int *nelgt, int *ndim, int *nelgv, int *nfields) {

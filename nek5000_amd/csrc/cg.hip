@@ -473,75 +473,155 @@ int run_chktcg1(Mesh &m, double *tol, const double *res, Coef h1, Coef h2,
   return NKH_OK;
 }
 
+// One Jacobi-PCG solve whose iterations can be enqueued one at a time (the
+// single solve below, and the batch of solves captured in a graph).
+struct CgRun {
+  Mesh *m = nullptr;
+  double *x = nullptr, *r = nullptr, *w = nullptr, *p = nullptr, *d = nullptr, *dm = nullptr,
+         *bm = nullptr;
+  double *red = nullptr;
+  CgState *st = nullptr;
+  const uint8_t *flag = nullptr;  // gs group flags for this solve's mask/mult
+  Coef h1, h2;
+  const double *mask = nullptr, *mult = nullptr;
+  bool fused = false, vec_ok = false, trivial = false;
+  int niter = 0, launched = 0, ax_grid_parts = 0;
+  double *hist = nullptr;
+
+  // hmholtz.f:611-720 up to the loop: preconditioner, r = f, null-space check,
+  // choice of the fused iteration, device state.  ws: 6 n doubles, red:
+  // kRedSize doubles, st: the state; all on stream s (host waits).
+  int begin(Mesh &mesh, double *ws, double *red_, CgState *st_, double *x_, const double *f,
+            Coef h1_, Coef h2_, const double *mask_, const double *mult_, double tin, int maxit,
+            const nkh_params &prm, bool want_hist, hipStream_t s) {
+    m = &mesh;
+    const int64_t n = mesh.n;
+    x = x_;
+    r = ws;
+    w = ws + n;
+    p = ws + 2 * n;
+    d = ws + 3 * n;
+    dm = ws + 4 * n;
+    bm = ws + 5 * n;
+    red = red_;
+    st = st_;
+    h1 = h1_;
+    h2 = h2_;
+    mask = mask_;
+    mult = mult_;
+    double tol = std::fabs(tin);
+    if (prm.restol != 0.0) tol = prm.restol;
+    if (tin < 0) tol = std::fabs(tin);
+    niter = maxit < kMaxCg ? maxit : kMaxCg;
+    NK_TRY(run_setprec(mesh, d, h1, h2, s));
+    hipLaunchKernelGGL(k_cg_init, dim3(grid_for(n)), dim3(256), 0, s, x, r, p, f, d, mult,
+                       mesh.binv, dm, bm, prm.param18 == 1 ? 1 : 0, n);
+    NK_HIP(hipGetLastError());
+    double fmax;
+    NK_TRY(reduce_host(mesh, f, nullptr, nullptr, n, 1, &fmax, s));
+    trivial = fmax == 0.0;
+    if (trivial) return NKH_OK;
+    // null-space check (hmholtz.f:705-720): pure Neumann with h2 == 0
+    double h2max, skmin;
+    if (h2.uniform()) h2max = h2.s;
+    else NK_TRY(reduce_host(mesh, h2.f, nullptr, nullptr, n, 2, &h2max, s));
+    NK_TRY(reduce_host(mesh, mask, nullptr, nullptr, n, 3, &skmin, s));
+    NK_CHECK(!(skmin > 0 && h2max == 0), NKH_ERR_UNSUPPORTED,
+             "cggo: pure-Neumann mean correction (ifmcor) is not implemented");
+    // Fused iteration (rho inside axhelm, mask inside the assembly) when its
+    // conditions hold: ids zero exactly at element interiors, mask 1 and
+    // mult 1 there, f zero where masked, and per group a 0/1 mask and mult
+    // summing to 1.  Checked once per solve.
+    NK_TRY(fused_conditions(mesh, mask, mult, f, &fused, s));
+    flag = mesh.gs.flag;
+    ax_grid_parts = axhelm_cg_grid(mesh);
+    auto al16 = [](const void *q) { return (reinterpret_cast<uintptr_t>(q) & 15) == 0; };
+    vec_ok = al16(x) && al16(r) && al16(p) && al16(w) && al16(dm) && al16(bm);
+    CgState h{};
+    h.rtz1 = 1.0;
+    h.tol = tol;
+    h.tin = tin;
+    h.vol = mesh.vol;
+    h.param22 = prm.param22;
+    h.maxit = niter;
+    h.istep = prm.istep;
+    h.param18 = prm.param18;
+    NK_HIP(hipMemcpyAsync(st, &h, sizeof(CgState), hipMemcpyHostToDevice, s));
+    hist = nullptr;
+    if (want_hist) {
+      hist = red + kRedHist;
+      hipLaunchKernelGGL(k_fill, dim3(grid_for(5 * kMaxCg)), dim3(256), 0, s, hist, 0.0,
+                         (int64_t)5 * kMaxCg);
+    }
+    NK_HIP(hipGetLastError());
+    return NKH_OK;
+  }
+
+  // one iteration's launches on s; ev (optional, 3 events) brackets axhelm
+  // and the assembly
+  int enqueue(hipStream_t s, hipEvent_t *ev = nullptr) {
+    const int64_t n = m->n;
+    double *part1 = red + kRedPart1, *part2 = red + kRedPart2, *rho1 = red + kRedRho1,
+           *rho2 = red + kRedRho2;
+    auto kern = vec_ok ? (launched == 0 ? k_cg_update_dots<true, false>
+                                        : k_cg_update_dots<true, true>)
+                       : (launched == 0 ? k_cg_update_dots<false, false>
+                                        : k_cg_update_dots<false, true>);
+    hipLaunchKernelGGL(kern, dim3(kUpdBlocks), dim3(256), 0, s, x, r, p, w, dm, bm, n, st, part1,
+                       part2);
+    hipLaunchKernelGGL(k_cg_scalars1, dim3(1), dim3(kScalThreads), 0, s, st, part1, part2, hist);
+    if (fused) {
+      if (ev) NK_HIP(hipEventRecord(ev[0], s));
+      NK_TRY(launch_axhelm_cg(*m, w, p, r, d, h1, h2, mult, st, rho1, s));
+      if (ev) NK_HIP(hipEventRecord(ev[1], s));
+      NK_TRY(gs_assemble(m->gs, w, st, s, flag));
+      if (ev) NK_HIP(hipEventRecord(ev[2], s));
+      hipLaunchKernelGGL(k_cg_scalars2, dim3(1), dim3(kScalThreads), 0, s, st, rho1,
+                         ax_grid_parts, rho2, 0, hist);
+    } else {
+      NK_TRY(launch_axhelm_cg(*m, w, p, r, d, h1, h2, mult, st, nullptr, s));
+      NK_TRY(gs_op(m->gs, w, 0, s));
+      hipLaunchKernelGGL(k_mask_guarded, dim3(grid_for(n)), dim3(256), 0, s, w, mask, n, st);
+      hipLaunchKernelGGL(k_dot3_guarded, dim3(kRedBlocks), dim3(256), 0, s, w, p, mult, n, st,
+                         rho1);
+      hipLaunchKernelGGL(k_cg_scalars2, dim3(1), dim3(kScalThreads), 0, s, st, rho1, kRedBlocks,
+                         rho2, 0, hist);
+    }
+    NK_HIP(hipGetLastError());
+    ++launched;
+    return NKH_OK;
+  }
+
+  void result(const CgState &hs, nkh_result *res) const {
+    nkh_result out{};
+    if (!trivial) {
+      out.niter = hs.niter;
+      out.converged = hs.converged;
+      out.rbn0 = hs.rbn0;
+      out.rbn2 = hs.rbn2;
+      out.tol = hs.tol;
+    }
+    if (res) *res = out;
+  }
+};
+
 int run_cggo(Mesh &m, double *x, const double *f, Coef h1, Coef h2, const double *mask,
              const double *mult, double tin, int maxit, const nkh_params &prm,
              nkh_result *res, double *history_host) {
   hipStream_t s = cur_stream();
   NK_TRY(ensure_workspace(m));
-  const int64_t n = m.n;
-  double *r = m.ws, *w = m.ws + n, *p = m.ws + 2 * n, *d = m.ws + 3 * n, *dm = m.ws + 4 * n,
-         *bm = m.ws + 5 * n;
-  double *part1 = m.red + kRedPart1, *part2 = m.red + kRedPart2, *rho1 = m.red + kRedRho1,
-         *rho2 = m.red + kRedRho2;
-
-  double tol = std::fabs(tin);
-  if (prm.restol != 0.0) tol = prm.restol;
-  if (tin < 0) tol = std::fabs(tin);
-  const int niter = maxit < kMaxCg ? maxit : kMaxCg;
-
-  NK_TRY(run_setprec(m, d, h1, h2, s));
-  hipLaunchKernelGGL(k_cg_init, dim3(grid_for(n)), dim3(256), 0, s, x, r, p, f, d, mult,
-                     m.binv, dm, bm, prm.param18 == 1 ? 1 : 0, n);
-  NK_HIP(hipGetLastError());
-
-  double fmax;
-  NK_TRY(reduce_host(m, f, nullptr, nullptr, n, 1, &fmax, s));
-  nkh_result out{};
-  if (fmax == 0.0) {
-    if (res) *res = out;
+  CgRun run;
+  NK_TRY(run.begin(m, m.ws, m.red, m.st, x, f, h1, h2, mask, mult, tin, maxit, prm,
+                   history_host != nullptr, s));
+  if (run.trivial) {
+    run.result(CgState{}, res);
     return NKH_OK;
   }
-  // null-space check (hmholtz.f:705-720): pure Neumann with h2 == 0
-  double h2max, skmin;
-  if (h2.uniform()) h2max = h2.s;
-  else NK_TRY(reduce_host(m, h2.f, nullptr, nullptr, n, 2, &h2max, s));
-  NK_TRY(reduce_host(m, mask, nullptr, nullptr, n, 3, &skmin, s));
-  NK_CHECK(!(skmin > 0 && h2max == 0), NKH_ERR_UNSUPPORTED,
-           "cggo: pure-Neumann mean correction (ifmcor) is not implemented");
-
-  // Fused iteration (rho inside axhelm, mask inside the assembly) when its
-  // conditions hold: ids zero exactly at element interiors, mask 1 and mult 1
-  // there, f zero where masked, and per group a 0/1 mask and mult summing
-  // to 1.  Checked once per solve.
-  bool fused = false;
-  NK_TRY(fused_conditions(m, mask, mult, f, &fused, s));
-  const int ax_grid_parts = axhelm_cg_grid(m);
-  auto al16 = [](const void *q) { return (reinterpret_cast<uintptr_t>(q) & 15) == 0; };
-  const bool vec_ok = al16(x) && al16(r) && al16(p) && al16(w) && al16(dm) && al16(bm);
-
-  CgState h{};
-  h.rtz1 = 1.0;
-  h.tol = tol;
-  h.tin = tin;
-  h.vol = m.vol;
-  h.param22 = prm.param22;
-  h.maxit = niter;
-  h.istep = prm.istep;
-  h.param18 = prm.param18;
-  NK_HIP(hipMemcpyAsync(m.st, &h, sizeof(CgState), hipMemcpyHostToDevice, s));
-
-  double *hist = nullptr;
-  if (history_host) {
-    hist = m.red + kRedHist;
-    hipLaunchKernelGGL(k_fill, dim3(grid_for(5 * kMaxCg)), dim3(256), 0, s, hist, 0.0,
-                       (int64_t)5 * kMaxCg);
-  }
-
   CgState hs{};
-  int launched = 0;
   int check_every = 1;
   // CTIMER mirror: event pairs around axhelm and gs of every launched iteration
   const bool timed = timers().on;
+  const int niter = run.niter;
   std::vector<hipEvent_t> ev;
   if (timed) {
     ev.resize((size_t)4 * (niter + 1));
@@ -554,40 +634,13 @@ int run_cggo(Mesh &m, double *x, const double *f, Coef h1, Coef h2, const double
     NK_HIP(hipEventRecord(t_solve0, s));
   }
   while (true) {
-    {
-      auto kern = vec_ok ? (launched == 0 ? k_cg_update_dots<true, false>
-                                          : k_cg_update_dots<true, true>)
-                         : (launched == 0 ? k_cg_update_dots<false, false>
-                                          : k_cg_update_dots<false, true>);
-      hipLaunchKernelGGL(kern, dim3(kUpdBlocks), dim3(256), 0, s, x, r, p, w, dm, bm, n, m.st,
-                         part1, part2);
-    }
-    hipLaunchKernelGGL(k_cg_scalars1, dim3(1), dim3(kScalThreads), 0, s, m.st, part1, part2, hist);
-    if (fused) {
-      if (timed) NK_HIP(hipEventRecord(ev[4 * launched + 0], s));
-      NK_TRY(launch_axhelm_cg(m, w, p, r, d, h1, h2, mult, m.st, rho1, s));
-      if (timed) NK_HIP(hipEventRecord(ev[4 * launched + 1], s));
-      NK_TRY(gs_assemble(m.gs, w, m.st, s));
-      if (timed) NK_HIP(hipEventRecord(ev[4 * launched + 2], s));
-      hipLaunchKernelGGL(k_cg_scalars2, dim3(1), dim3(kScalThreads), 0, s, m.st, rho1, ax_grid_parts,
-                         rho2, 0, hist);
-    } else {
-      NK_TRY(launch_axhelm_cg(m, w, p, r, d, h1, h2, mult, m.st, nullptr, s));
-      NK_TRY(gs_op(m.gs, w, 0, s));
-      hipLaunchKernelGGL(k_mask_guarded, dim3(grid_for(n)), dim3(256), 0, s, w, mask, n, m.st);
-      hipLaunchKernelGGL(k_dot3_guarded, dim3(kRedBlocks), dim3(256), 0, s, w, p, mult, n,
-                         m.st, rho1);
-      hipLaunchKernelGGL(k_cg_scalars2, dim3(1), dim3(kScalThreads), 0, s, m.st, rho1, kRedBlocks,
-                         rho2, 0, hist);
-    }
-    NK_HIP(hipGetLastError());
-    ++launched;
-    if (launched % check_every == 0 || launched > niter) {
-      NK_HIP(hipMemcpyAsync(&hs, m.st, sizeof(CgState), hipMemcpyDeviceToHost, s));
+    NK_TRY(run.enqueue(s, timed ? &ev[4 * run.launched] : nullptr));
+    if (run.launched % check_every == 0 || run.launched > niter) {
+      NK_HIP(hipMemcpyAsync(&hs, run.st, sizeof(CgState), hipMemcpyDeviceToHost, s));
       NK_HIP(hipStreamSynchronize(s));
       if (hs.done) break;
-      NK_CHECK(launched <= niter + 1, NKH_ERR_STATE, "cggo: state machine did not terminate");
-      if (launched >= 8) check_every = 4;
+      NK_CHECK(run.launched <= niter + 1, NKH_ERR_STATE, "cggo: state machine did not terminate");
+      if (run.launched >= 8) check_every = 4;
     }
   }
   if (timed) {
@@ -595,7 +648,7 @@ int run_cggo(Mesh &m, double *x, const double *f, Coef h1, Coef h2, const double
     NK_HIP(hipEventSynchronize(t_solve1));
     float ms = 0.f;
     // axhelm and gs did work in launches 1..niter (later ones returned at once)
-    for (int it = 0; it < hs.niter && fused; ++it) {
+    for (int it = 0; it < hs.niter && run.fused; ++it) {
       NK_HIP(hipEventElapsedTime(&ms, ev[4 * it + 0], ev[4 * it + 1]));
       timers().taxhm += ms * 1e-3;
       timers().naxhm += 1;
@@ -610,17 +663,233 @@ int run_cggo(Mesh &m, double *x, const double *f, Coef h1, Coef h2, const double
     (void)hipEventDestroy(t_solve0);
     (void)hipEventDestroy(t_solve1);
   }
-  out.niter = hs.niter;
-  out.converged = hs.converged;
-  out.rbn0 = hs.rbn0;
-  out.rbn2 = hs.rbn2;
-  out.tol = hs.tol;
-  if (res) *res = out;
+  run.result(hs, res);
   if (history_host) {
-    NK_HIP(hipMemcpyAsync(history_host, hist, sizeof(double) * 5 * kMaxCg,
+    NK_HIP(hipMemcpyAsync(history_host, run.hist, sizeof(double) * 5 * kMaxCg,
                           hipMemcpyDeviceToHost, s));
     NK_HIP(hipStreamSynchronize(s));
   }
+  return NKH_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Batched Jacobi-PCG: several hmholtz solves (different meshes, h1/h2,
+// masks, sizes; the same lx1 or not) iterate together.  Each solve keeps its
+// own work space, state and gs group flags; the steady-state iteration of
+// all of them -- five launches per solve, each solve on its own stream,
+// forked from and joined to the caller's stream -- is captured once in a
+// hipGraph and replayed until every solve has stopped (a finished solve's
+// kernels return at once).  No memset is recorded in the graph.
+// ---------------------------------------------------------------------------
+struct CgBatch {
+  int nb = 0;
+  std::vector<int> mesh;
+  std::vector<double *> ws, red;
+  std::vector<CgState *> st;
+  std::vector<uint8_t *> flag;
+  std::vector<int64_t> nflag;
+  std::vector<hipStream_t> streams;
+  std::vector<hipEvent_t> ev;  // fork + one join per solve
+  // the graph is captured and launched on a library stream (capture is not
+  // allowed on the caller's stream when that is the null stream), ordered
+  // after and before the caller's stream by ev_sync
+  hipStream_t cap = nullptr;
+  hipEvent_t ev_sync = nullptr;
+  void release() {
+    if (cap) (void)hipStreamDestroy(cap);
+    if (ev_sync) (void)hipEventDestroy(ev_sync);
+    cap = nullptr;
+    ev_sync = nullptr;
+    for (double *q : ws) (void)hipFree(q);
+    for (double *q : red) (void)hipFree(q);
+    for (CgState *q : st) (void)hipFree(q);
+    for (uint8_t *q : flag) (void)hipFree(q);
+    for (hipStream_t q : streams) (void)hipStreamDestroy(q);
+    for (hipEvent_t q : ev) (void)hipEventDestroy(q);
+    ws.clear();
+    red.clear();
+    st.clear();
+    flag.clear();
+    streams.clear();
+    ev.clear();
+  }
+};
+
+namespace {
+std::vector<CgBatch *> g_batches;
+}
+
+int batch_create(int nb, const int *meshes, int *id) {
+  NK_CHECK(nb >= 1 && nb <= 64 && meshes && id, NKH_ERR_ARG, "nkh_batch_create: bad arguments");
+  CgBatch *B = new CgBatch();
+  B->nb = nb;
+  auto fail = [&](int rc) {
+    B->release();
+    delete B;
+    return rc;
+  };
+  for (int k = 0; k < nb; ++k) {
+    Mesh *m = get_mesh(meshes[k]);
+    if (!m || !m->has_geom || !m->has_gs) {
+      set_error("nkh_batch_create: mesh %d is not ready", meshes[k]);
+      return fail(NKH_ERR_STATE);
+    }
+    NK_TRY(ensure_small(*m));
+    double *w = nullptr, *r = nullptr;
+    CgState *c = nullptr;
+    uint8_t *fl = nullptr;
+    hipStream_t q = nullptr;
+    if (hipMalloc(&w, sizeof(double) * 6 * (size_t)m->n) != hipSuccess ||
+        hipMalloc(&r, sizeof(double) * kRedSize) != hipSuccess ||
+        hipMalloc(&c, sizeof(CgState)) != hipSuccess ||
+        hipMalloc(&fl, (size_t)std::max<int64_t>(m->gs.ngroups, 1)) != hipSuccess ||
+        hipStreamCreateWithFlags(&q, hipStreamNonBlocking) != hipSuccess) {
+      (void)hipGetLastError();
+      for (void *v : {(void *)w, (void *)r, (void *)c, (void *)fl}) (void)hipFree(v);
+      if (q) (void)hipStreamDestroy(q);
+      set_error("nkh_batch_create: out of device memory");
+      return fail(NKH_ERR_NOMEM);
+    }
+    B->mesh.push_back(meshes[k]);
+    B->ws.push_back(w);
+    B->red.push_back(r);
+    B->st.push_back(c);
+    B->flag.push_back(fl);
+    B->nflag.push_back(m->gs.ngroups);
+    B->streams.push_back(q);
+  }
+  for (int k = 0; k <= nb; ++k) {
+    hipEvent_t e;
+    if (hipEventCreateWithFlags(&e, hipEventDisableTiming) != hipSuccess) {
+      set_error("nkh_batch_create: event creation failed");
+      return fail(NKH_ERR_HIP);
+    }
+    B->ev.push_back(e);
+  }
+  if (hipStreamCreateWithFlags(&B->cap, hipStreamNonBlocking) != hipSuccess ||
+      hipEventCreateWithFlags(&B->ev_sync, hipEventDisableTiming) != hipSuccess) {
+    set_error("nkh_batch_create: stream creation failed");
+    return fail(NKH_ERR_HIP);
+  }
+  g_batches.push_back(B);
+  *id = (int)g_batches.size() - 1;
+  return NKH_OK;
+}
+
+int batch_meshes(int id, std::vector<int> *meshes) {
+  NK_CHECK(id >= 0 && id < (int)g_batches.size() && g_batches[(size_t)id], NKH_ERR_ARG,
+           "batch: bad handle %d", id);
+  *meshes = g_batches[(size_t)id]->mesh;
+  return NKH_OK;
+}
+
+int batch_destroy(int id) {
+  NK_CHECK(id >= 0 && id < (int)g_batches.size() && g_batches[(size_t)id], NKH_ERR_ARG,
+           "nkh_batch_destroy: bad handle %d", id);
+  (void)hipDeviceSynchronize();
+  g_batches[(size_t)id]->release();
+  delete g_batches[(size_t)id];
+  g_batches[(size_t)id] = nullptr;
+  return NKH_OK;
+}
+
+int run_cggo_batch(int id, const double *const *x, const double *const *f, const Coef *h1,
+                   const Coef *h2, const double *const *mask, const double *const *mult,
+                   const double *tin, int maxit, const nkh_params &prm, nkh_result *res,
+                   int *launches_out) {
+  NK_CHECK(id >= 0 && id < (int)g_batches.size() && g_batches[(size_t)id], NKH_ERR_ARG,
+           "batch: bad handle %d", id);
+  CgBatch &B = *g_batches[(size_t)id];
+  hipStream_t s = cur_stream();
+  std::vector<CgRun> runs((size_t)B.nb);
+  std::vector<int> active;
+  for (int k = 0; k < B.nb; ++k) {
+    Mesh *m = get_mesh(B.mesh[(size_t)k]);
+    NK_CHECK(m && m->n > 0, NKH_ERR_STATE, "batch: mesh of solve %d is gone", k);
+    NK_CHECK(m->gs.ngroups == B.nflag[(size_t)k], NKH_ERR_STATE,
+             "batch: the gather-scatter map of solve %d changed", k);
+    // set-up of solve k on s (it uses the mesh's scratch reductions)
+    NK_TRY(runs[(size_t)k].begin(*m, B.ws[(size_t)k], B.red[(size_t)k], B.st[(size_t)k],
+                                 const_cast<double *>(x[k]), f[k], h1[k], h2[k], mask[k],
+                                 mult[k], tin[k], maxit, prm, false, s));
+    if (runs[(size_t)k].trivial) continue;
+    if (runs[(size_t)k].fused && m->gs.ngroups > 0) {
+      NK_HIP(hipMemcpyAsync(B.flag[(size_t)k], m->gs.flag, (size_t)m->gs.ngroups,
+                            hipMemcpyDeviceToDevice, s));
+      runs[(size_t)k].flag = B.flag[(size_t)k];
+    }
+    active.push_back(k);
+  }
+  // the first iteration (no pending update) eagerly, then the graph; all of
+  // it on the batch's own stream c, after the set-up on s
+  hipStream_t c = B.cap;
+  NK_HIP(hipEventRecord(B.ev_sync, s));
+  NK_HIP(hipStreamWaitEvent(c, B.ev_sync, 0));
+  auto fork_join = [&](auto &&body) -> int {
+    NK_HIP(hipEventRecord(B.ev[0], c));
+    for (int k : active) {
+      hipStream_t q = B.streams[(size_t)k];
+      NK_HIP(hipStreamWaitEvent(q, B.ev[0], 0));
+      NK_TRY(body(k, q));
+      NK_HIP(hipEventRecord(B.ev[(size_t)k + 1], q));
+    }
+    for (int k : active) NK_HIP(hipStreamWaitEvent(c, B.ev[(size_t)k + 1], 0));
+    return NKH_OK;
+  };
+  int launches = 0;
+  if (!active.empty()) {
+    NK_TRY(fork_join([&](int k, hipStream_t q) { return runs[(size_t)k].enqueue(q); }));
+    launches = 1;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    NK_HIP(hipStreamBeginCapture(c, hipStreamCaptureModeThreadLocal));
+    const int rc_cap =
+        fork_join([&](int k, hipStream_t q) { return runs[(size_t)k].enqueue(q); });
+    const hipError_t end = hipStreamEndCapture(c, &graph);
+    NK_TRY(rc_cap);
+    NK_HIP(end);
+    NK_HIP(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
+    int niter_max = 0;
+    for (int k : active) niter_max = std::max(niter_max, runs[(size_t)k].niter);
+    std::vector<CgState> hs((size_t)B.nb);
+    int check_every = 1, rc = NKH_OK;
+    while (true) {
+      if (hipGraphLaunch(exec, c) != hipSuccess) {
+        set_error("batch: graph launch failed");
+        rc = NKH_ERR_HIP;
+        break;
+      }
+      ++launches;
+      if (launches % check_every == 0 || launches > niter_max) {
+        for (int k : active)
+          (void)hipMemcpyAsync(&hs[(size_t)k], B.st[(size_t)k], sizeof(CgState),
+                               hipMemcpyDeviceToHost, c);
+        if (hipStreamSynchronize(c) != hipSuccess) {
+          set_error("batch: state read-back failed");
+          rc = NKH_ERR_HIP;
+          break;
+        }
+        bool all = true;
+        for (int k : active) all = all && hs[(size_t)k].done;
+        if (all) break;
+        if (launches > niter_max + 1) {
+          set_error("batch: state machines did not terminate");
+          rc = NKH_ERR_STATE;
+          break;
+        }
+        if (launches >= 8) check_every = 4;
+      }
+    }
+    (void)hipGraphExecDestroy(exec);
+    (void)hipGraphDestroy(graph);
+    NK_TRY(rc);
+    NK_HIP(hipEventRecord(B.ev_sync, c));
+    NK_HIP(hipStreamWaitEvent(s, B.ev_sync, 0));
+    for (int k : active) runs[(size_t)k].result(hs[(size_t)k], &res[k]);
+  }
+  for (int k = 0; k < B.nb; ++k)
+    if (runs[(size_t)k].trivial) runs[(size_t)k].result(CgState{}, &res[k]);
+  if (launches_out) *launches_out = launches;
   return NKH_OK;
 }
 

@@ -169,6 +169,144 @@ __global__ void __launch_bounds__(NP * NP * epb<NP>())
   }
 }
 
+// The same operator with the element staged in LDS one t-plane at a time:
+// 3 NP^2 doubles of LDS per element instead of 3 NP^3 (at NP = 16 the
+// element-staged kernel holds 96 KB and one block per CU).  ur, us, wr and
+// ws of plane k need only plane k; the t-direction terms stay in registers.
+// Arithmetic and its order are those of k_axhelm, so results are identical.
+template <int NP, int MODE, bool H1F, int H2M>
+__global__ void __launch_bounds__(NP * NP * epb<NP>())
+    k_axhelm_pl(DMat<NP> Dm, const double *__restrict__ Dg, double *__restrict__ au,
+                const double *__restrict__ uin, const double *__restrict__ g, int64_t gstride,
+                const double *__restrict__ h1f, double h1s, const double *__restrict__ h2f,
+                double h2s, const double *__restrict__ bm1, int64_t nel,
+                double *__restrict__ pout, const double *__restrict__ r,
+                const double *__restrict__ dinv, const double *__restrict__ mult,
+                const CgState *__restrict__ st, double *__restrict__ rho_part,
+                const double *__restrict__ betap) {
+  constexpr int NP2 = NP * NP, EPB = epb<NP>();
+  __shared__ double s_D[NP][NP + 1];
+  __shared__ double s_u[EPB][NP2];
+  __shared__ double s_wr[EPB][NP2];
+  __shared__ double s_ws[EPB][NP2];
+  __shared__ double s_red[NP2 * EPB];
+  double beta = 0.0;
+  if constexpr (MODE == 1) {
+    if (st->done) return;
+    beta = st->beta;
+  }
+  if constexpr (MODE == 2) beta = *betap;
+  const int tid = threadIdx.x;
+  const int le = tid / NP2;
+  const int ij = tid - le * NP2;
+  const int i = ij % NP, j = ij / NP;
+  for (int t = tid; t < NP2; t += blockDim.x) s_D[t % NP][t / NP] = Dg[t];
+  double rho = 0.0;
+  for (int64_t eb = blockIdx.x; eb * EPB < nel; eb += gridDim.x) {
+    const int64_t e = eb * EPB + le;
+    const bool valid = (le < EPB) && (e < nel);
+    const int64_t base = e * (int64_t)(NP2 * NP) + ij;
+    double r_u[NP], r_au[NP], r_o[NP];
+#pragma unroll
+    for (int k = 0; k < NP; ++k) {
+      double v = 0.0;
+      if (valid) {
+        const int64_t l = base + k * NP2;
+        if constexpr (MODE == 1) {
+          v = beta * uin[l] + dinv[l] * r[l];
+          pout[l] = v;
+        } else if constexpr (MODE == 2) {
+          v = beta * uin[l] + r[l];
+          pout[l] = v;
+        } else {
+          v = uin[l];
+        }
+      }
+      r_u[k] = v;
+      r_au[k] = 0.0;
+    }
+#pragma unroll
+    for (int k = 0; k < NP; ++k) {
+      // one plane at a time: keeps the scheduler from hoisting the next
+      // planes' geometric-factor loads (6 doubles each) into registers
+      __builtin_amdgcn_sched_barrier(0);
+      __syncthreads();  // the previous plane (or element) is read
+      s_u[le][ij] = r_u[k];
+      __syncthreads();
+      double ur = 0.0, us = 0.0, ut = 0.0;
+#pragma unroll
+      for (int m = 0; m < NP; ++m) {
+        ur += s_D[i][m] * s_u[le][j * NP + m];
+        us += s_D[j][m] * s_u[le][m * NP + i];
+        ut += Dm.d[k + NP * m] * r_u[m];
+      }
+      double wr = 0.0, ws = 0.0, wt = 0.0;
+      if (valid) {
+        const int64_t l = base + k * NP2;
+        const double G1 = g[l], G2 = g[gstride + l], G3 = g[2 * gstride + l];
+        const double G4 = g[3 * gstride + l], G5 = g[4 * gstride + l],
+                     G6 = g[5 * gstride + l];
+        wr = ur * G1;
+        ws = us * G2;
+        wt = ut * G3;
+        wr = wr + us * G4;
+        wr = wr + ut * G5;
+        ws = ws + ur * G4;
+        ws = ws + ut * G6;
+        wt = wt + ur * G5;
+        wt = wt + us * G6;
+        const double h1 = H1F ? h1f[l] : h1s;
+        wr *= h1;
+        ws *= h1;
+        wt *= h1;
+      }
+      s_wr[le][ij] = wr;
+      s_ws[le][ij] = ws;
+#pragma unroll
+      for (int kk = 0; kk < NP; ++kk) r_au[kk] += Dm.d[k + NP * kk] * wt;
+      __syncthreads();
+      double ar = 0.0, as = 0.0;
+#pragma unroll
+      for (int m = 0; m < NP; ++m) {
+        ar += s_D[m][i] * s_wr[le][j * NP + m];
+        as += s_D[m][j] * s_ws[le][m * NP + i];
+      }
+      r_o[k] = ar + as;
+      // pin this plane's sums here: the compiler would otherwise sink every
+      // plane's contractions to the final stores and keep their LDS operands
+      // (2 NP doubles per plane) live in registers
+      asm volatile("" : "+v"(r_o[k]));
+#pragma unroll
+      for (int kk = 0; kk < NP; ++kk) asm volatile("" : "+v"(r_au[kk]));
+    }
+#pragma unroll
+    for (int k = 0; k < NP; ++k) {
+      const int64_t l = base + k * NP2;
+      double a = r_o[k] + r_au[k];
+      if constexpr (H2M == 1) a = a + (h2s * (valid ? bm1[l] : 0.0)) * r_u[k];
+      if constexpr (H2M == 2) a = a + ((valid ? h2f[l] : 0.0) * (valid ? bm1[l] : 0.0)) * r_u[k];
+      if (valid) au[l] = a;
+      if constexpr (MODE >= 1) rho += a * r_u[k];  // a = r_u = 0 off the mesh
+    }
+  }
+  if constexpr (MODE >= 1) {
+    if (rho_part != nullptr) {
+      const double t = block_sum(rho, s_red);
+      if (threadIdx.x == 0) rho_part[blockIdx.x] = t;
+    }
+  }
+}
+
+// Per polynomial order: which of the two kernels and (for the plain apply)
+// how many blocks, chosen by the autotuner of launch_axhelm on the first
+// large apply of that order.  variant 0: element staged in LDS, 1: planes.
+struct AxhelmChoice {
+  int variant = 0;
+  int grid = 0;  // plain apply: blocks (0 -> one per element group, up to 2^20)
+  bool tuned = false;
+};
+inline AxhelmChoice g_axhelm_choice[kMaxNP + 1];
+
 template <int NP, int MODE>
 static int axhelm_dispatch(const Mesh &m, double *au, const double *u, Coef h1, Coef h2,
                            double *pout, const double *r, const double *dinv,
@@ -178,18 +316,28 @@ static int axhelm_dispatch(const Mesh &m, double *au, const double *u, Coef h1, 
   for (int t = 0; t < NP * NP; ++t) Dm.d[t] = m.D[t];
   constexpr int EPB = epb<NP>();
   const int64_t ngroups = (m.nel + EPB - 1) / EPB;
+  const AxhelmChoice ch = g_axhelm_choice[NP];
   int grid;
-  if (MODE >= 1) grid = axhelm_cg_grid(m);
-  else grid = (int)((ngroups < (int64_t)1 << 20) ? ngroups : ((int64_t)1 << 20));
+  if (MODE >= 1) {
+    grid = axhelm_cg_grid(m);
+  } else {
+    const int64_t cap = ch.grid > 0 ? ch.grid : ((int64_t)1 << 20);
+    grid = (int)(ngroups < cap ? ngroups : cap);
+  }
   if (grid < 1) grid = 1;
   const dim3 block(NP * NP * EPB);
   // h2 mode: 0 none (reference ifh2 false), 1 uniform, 2 field
   const int h2m = h2.uniform() ? (h2.s != 0.0 ? 1 : 0) : 2;
   const bool h1f = !h1.uniform();
-#define NK_LAUNCH(H1F_, H2M_)                                                            \
-  hipLaunchKernelGGL((k_axhelm<NP, MODE, H1F_, H2M_>), dim3(grid), block, 0, s, Dm, m.Dd, \
-                     au, u, m.g, m.n, h1.f, h1.s, h2.f, h2.s, m.bm1, m.nel, pout, r, dinv, \
-                     mult, st, rho_part, betap)
+#define NK_LAUNCH(H1F_, H2M_)                                                              \
+  if (ch.variant == 1)                                                                      \
+    hipLaunchKernelGGL((k_axhelm_pl<NP, MODE, H1F_, H2M_>), dim3(grid), block, 0, s, Dm,    \
+                       m.Dd, au, u, m.g, m.n, h1.f, h1.s, h2.f, h2.s, m.bm1, m.nel, pout, r, \
+                       dinv, mult, st, rho_part, betap);                                    \
+  else                                                                                      \
+    hipLaunchKernelGGL((k_axhelm<NP, MODE, H1F_, H2M_>), dim3(grid), block, 0, s, Dm, m.Dd, \
+                       au, u, m.g, m.n, h1.f, h1.s, h2.f, h2.s, m.bm1, m.nel, pout, r, dinv, \
+                       mult, st, rho_part, betap)
   if (h1f) {
     if (h2m == 0) NK_LAUNCH(true, 0);
     else if (h2m == 1) NK_LAUNCH(true, 1);

@@ -269,22 +269,24 @@ int gs_prepare(GsMap &gs, const double *mask, const double *mult, double *red, b
 }
 
 template <typename T>
-static int gs_assemble_t(const GsMap &gs, T *w, const CgState *st, hipStream_t s) {
+static int gs_assemble_t(const GsMap &gs, T *w, const CgState *st, hipStream_t s,
+                         const uint8_t *flag) {
   if (gs.ngroups == 0) return NKH_OK;
   const int nb = kGsBlocks;
   const int64_t chunk = (gs.ngroups + nb - 1) / nb;
-  hipLaunchKernelGGL(k_gs_assemble<T>, dim3(nb), dim3(256), 0, s, w, gs.off, gs.idx, gs.flag,
-                     gs.ngroups, chunk, st);
+  hipLaunchKernelGGL(k_gs_assemble<T>, dim3(nb), dim3(256), 0, s, w, gs.off, gs.idx,
+                     flag ? flag : gs.flag, gs.ngroups, chunk, st);
   NK_HIP(hipGetLastError());
   return NKH_OK;
 }
 
-int gs_assemble(const GsMap &gs, double *w, const CgState *st, hipStream_t s) {
-  return gs_assemble_t(gs, w, st, s);
+int gs_assemble(const GsMap &gs, double *w, const CgState *st, hipStream_t s,
+                const uint8_t *flag) {
+  return gs_assemble_t(gs, w, st, s, flag);
 }
 
 int gs_assemble(const GsMap &gs, float *w, const CgState *st, hipStream_t s) {
-  return gs_assemble_t(gs, w, st, s);
+  return gs_assemble_t(gs, w, st, s, (const uint8_t *)nullptr);
 }
 
 template <typename E, typename Z, typename W>

@@ -4,6 +4,7 @@
 
 #include <cstdint>
 #include <cstdio>
+#include <vector>
 
 #include "../../include/nekamd.h"
 
@@ -150,6 +151,11 @@ struct Coef {
 
 int launch_axhelm(const Mesh &m, double *au, const double *u, Coef h1, Coef h2,
                   hipStream_t s);
+// Kernel choice per order (0: element staged in LDS, 1: t-planes staged) and
+// the plain apply's grid (0: one block per element group); tuned on the
+// first apply of the order on a mesh of >= 2^20 nodes unless set first.
+int axhelm_tuning(int np, int *variant, int *grid, int *tuned);
+int axhelm_set_tuning(int np, int variant, int grid);
 int launch_geom(Mesh &m, const double *x, const double *y, const double *z, hipStream_t s);
 int launch_setprec(const Mesh &m, double *d, Coef h1, Coef h2, hipStream_t s);
 // axhelm fused into the CG iteration: p = d*r + beta*p; w = A p (h1/h2 as
@@ -175,7 +181,9 @@ void gs_free(GsMap &gs);
 int gs_op(const GsMap &gs, double *u, int op, hipStream_t s);
 // CG assembly: w <- mask * dssum(w) using the flags of gs_prepare; skipped
 // when st is given and st->done.
-int gs_assemble(const GsMap &gs, double *w, const CgState *st, hipStream_t s);
+// flag: group flags to use instead of gs.flag (a solve with its own mask)
+int gs_assemble(const GsMap &gs, double *w, const CgState *st, hipStream_t s,
+                const uint8_t *flag = nullptr);
 int gs_assemble(const GsMap &gs, float *w, const CgState *st, hipStream_t s);
 // Per id group (single members included): S = sum of e; z_i <- mean over
 // the members of S*w_i + z_i.  The fine-level end of the multigrid V-cycle.
@@ -191,6 +199,14 @@ int gs_prepare(GsMap &gs, const double *mask, const double *mult, double *red, b
 int run_cggo(Mesh &m, double *x, const double *f, Coef h1, Coef h2, const double *mask,
              const double *mult, double tin, int maxit, const nkh_params &prm,
              nkh_result *res, double *history_host);
+// batched Jacobi-PCG solves (graph-captured iteration), see nkd_hmholtz_batch
+int batch_create(int nb, const int *meshes, int *id);
+int batch_destroy(int id);
+int batch_meshes(int id, std::vector<int> *meshes);
+int run_cggo_batch(int id, const double *const *x, const double *const *f, const Coef *h1,
+                   const Coef *h2, const double *const *mask, const double *const *mult,
+                   const double *tin, int maxit, const nkh_params &prm, nkh_result *res,
+                   int *launches_out);
 int run_chktcg1(Mesh &m, double *tol, const double *res, Coef h1, Coef h2,
                 const double *mask, const double *mult);
 int classify_coefs(Mesh &m, const double *h1, const double *h2, Coef *c1, Coef *c2,

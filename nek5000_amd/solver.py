@@ -27,7 +27,7 @@ import ctypes
 import numpy as np
 
 from . import _lib
-from ._lib import NkhParams, NkhResult, check, load
+from ._lib import NekAmdError, NkhParams, NkhResult, check, load  # noqa: F401
 from .mesh import BoxMesh
 
 _initialised = {"device": None}
@@ -64,6 +64,23 @@ def _ptr(t):
     if t.dtype != torch.float64 or not t.is_contiguous():
         raise TypeError("expected a contiguous float64 tensor")
     return ctypes.c_void_p(t.data_ptr())
+
+
+def axhelm_tuning(lx1):
+    """(variant, grid, tuned) of the axhelm kernels for order lx1: variant 0
+    stages the element in LDS, 1 one t-plane at a time (identical results);
+    grid = blocks of the plain apply (0: one per element group)."""
+    lib = init()
+    v, g, t = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+    check(lib.nkh_axhelm_tuning(int(lx1), ctypes.byref(v), ctypes.byref(g), ctypes.byref(t)),
+          "nkh_axhelm_tuning")
+    return v.value, g.value, bool(t.value)
+
+
+def set_axhelm_tuning(lx1, variant, grid=0):
+    """Fix the axhelm kernel choice for order lx1 (no autotuning afterwards)."""
+    check(init().nkh_axhelm_set_tuning(int(lx1), int(variant), int(grid)),
+          "nkh_axhelm_set_tuning")
 
 
 def default_params(istep=1, param18=0, param22=1e-7, restol=1e-7, precision=0, param42=0,
@@ -307,6 +324,45 @@ class Field:
     def destroy(self):
         if self.handle is not None and self.handle >= 0:
             check(self.lib.nkh_mesh_destroy(self.handle), "nkh_mesh_destroy")
+            self.handle = None
+
+
+class Batch:
+    """Several hmholtz solves at once (nkh_batch_create / nkd_hmholtz_batch):
+    one entry per solve, each on a Field (a Field may appear more than once)."""
+
+    def __init__(self, fields):
+        self.fields = list(fields)
+        self.lib = init()
+        ids = (ctypes.c_int * max(len(self.fields), 1))(*[f.handle for f in self.fields])
+        h = ctypes.c_int(-1)
+        check(self.lib.nkh_batch_create(len(self.fields), ids, ctypes.byref(h)),
+              "nkh_batch_create")
+        self.handle = h.value
+
+    def hmholtz(self, solves, maxit, params=None):
+        """solves: list of dicts u, rhs, h1, h2, mask, mult, tli (tensors on
+        the GPU).  Returns (list of result dicts, graph launches)."""
+        nb = len(self.fields)
+        assert len(solves) == nb
+        arr = (_lib.NkhBatchSolve * nb)()
+        for k, q in enumerate(solves):
+            arr[k] = _lib.NkhBatchSolve(_ptr(q["u"]), _ptr(q["rhs"]), _ptr(q["h1"]),
+                                        _ptr(q["h2"]), _ptr(q["mask"]), _ptr(q["mult"]),
+                                        float(q["tli"]))
+        res = (NkhResult * nb)()
+        launches = ctypes.c_int(0)
+        prm = params if params is not None else default_params()
+        _sync_stream(self.lib)
+        check(self.lib.nkd_hmholtz_batch(self.handle, arr, int(maxit), ctypes.byref(prm), res,
+                                         ctypes.byref(launches)), "nkd_hmholtz_batch")
+        out = [dict(niter=r.niter, converged=r.converged, rbn0=r.rbn0, rbn2=r.rbn2, tol=r.tol)
+               for r in res]
+        return out, launches.value
+
+    def destroy(self):
+        if self.handle is not None and self.handle >= 0:
+            check(self.lib.nkh_batch_destroy(self.handle), "nkh_batch_destroy")
             self.handle = None
 
 

@@ -1,0 +1,149 @@
+"""The two axhelm kernels (element staged in LDS / one t-plane at a time) and
+the per-order autotuner that picks between them, through the C ABI:
+
+  * the plain apply of both variants is bit-identical and within 1e-13 of the
+    oracle, for every coefficient mode, at orders 3..16;
+  * the Jacobi PCG (the rho-fused apply) and the multigrid flexible CG (the
+    p-update-fused apply) give bit-identical iterates with either variant;
+  * the first apply of an order on a mesh of >= 2^20 nodes tunes it, after
+    which the choice is reported and the results do not depend on it.
+"""
+import sys
+
+import numpy as np
+import pytest
+
+from problems import Problem, box
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def host(t):
+    return t.detach().cpu().numpy()
+
+
+def rel(a, b):
+    return float(np.max(np.abs(a - b)) / max(np.max(np.abs(b)), 1e-300))
+
+
+@pytest.mark.parametrize("lx1", [3, 5, 7, 8, 10, 12, 16])
+def test_plain_apply_variants_identical(gpu, lx1):
+    b = box((3, 2, 2), lx1, deform=0.1)
+    P = Problem(b)
+    F = gpu.box_field(b)
+    rng = np.random.default_rng(lx1)
+    u = rng.standard_normal(P.shape)
+    h1f = 1.0 + rng.random(P.shape)
+    h2f = rng.random(P.shape)
+    one, zero = np.ones(P.shape), np.zeros(P.shape)
+    saved = gpu.axhelm_tuning(lx1)
+    try:
+        for h1, h2, tag in ((one, zero, "h1=1,h2=0"), (one * 0.3, one * 10.0, "uniform h2"),
+                            (h1f, h2f, "field h1,h2")):
+            ref = P.field.axhelm(u, h1, h2)
+            outs = []
+            for variant, grid in ((0, 0), (1, 0), (1, 2)):
+                say(f"[axhelm variants] lx1={lx1} {tag} variant={variant} grid={grid}")
+                gpu.set_axhelm_tuning(lx1, variant, grid)
+                au = F.zeros()
+                F.axhelm(au, F.to_dev(u), F.to_dev(h1), F.to_dev(h2))
+                outs.append(host(au))
+            assert rel(outs[0], ref) < 1e-13, tag
+            for o in outs[1:]:
+                assert np.array_equal(o, outs[0]), tag
+    finally:
+        gpu.set_axhelm_tuning(lx1, saved[0], saved[1])
+    F.destroy()
+
+
+def test_jacobi_pcg_variants_identical(gpu):
+    b = box((4, 3, 3), 8, deform=0.1)
+    P = Problem(b)
+    F = gpu.box_field(b)
+    h1 = F.to_dev(1.0 + 0.5 * (P.uniform_field(11) + 1.0))
+    h2 = F.zeros() + 2.0
+    rhs = F.to_dev(P.bm1 * P.uniform_field(12))
+    saved = gpu.axhelm_tuning(8)
+    outs = []
+    try:
+        for variant in (0, 1):
+            say(f"[pcg variants] variant={variant}")
+            gpu.set_axhelm_tuning(8, variant, 0)
+            u = F.zeros()
+            out = F.hmholtz("VELX", u, rhs.clone(), h1, h2, F.mask, F.mult, -1e-10, 500,
+                            history=True)
+            assert out["converged"]
+            outs.append((host(u), out))
+    finally:
+        gpu.set_axhelm_tuning(8, saved[0], saved[1])
+    assert outs[0][1]["niter"] == outs[1][1]["niter"]
+    assert np.array_equal(outs[0][0], outs[1][0])
+    assert np.array_equal(np.asarray(outs[0][1]["history"]), np.asarray(outs[1][1]["history"]))
+    F.destroy()
+
+
+def test_flex_cg_variants_identical(gpu):
+    b = box((3, 3, 3), 7, deform=0.1)
+    P = Problem(b)
+    F = gpu.box_field(b)
+    x, y, z = b.xyz()
+    F.h1mg_setup(b.fast_bc(), b.vertex, x, y, z)
+    h1, h2 = F.zeros() + 1.0, F.zeros()
+    rhs = F.to_dev(P.field.dssum(P.bm1 * P.uniform_field(3)) * P.mask)
+    saved = gpu.axhelm_tuning(7)
+    outs = []
+    try:
+        for variant in (0, 1):
+            say(f"[flex-cg variants] variant={variant}")
+            gpu.set_axhelm_tuning(7, variant, 0)
+            res = rhs.clone()
+            prm = gpu.default_params(istep=1, param42=2, param21=-1e-10)
+            out = F.hmh_flex_cg(res, h1, h2, F.mult, 100, params=prm, history=True)
+            assert out["converged"]
+            outs.append((host(res), out))
+    finally:
+        gpu.set_axhelm_tuning(7, saved[0], saved[1])
+    assert outs[0][1]["niter"] == outs[1][1]["niter"]
+    assert np.array_equal(outs[0][0], outs[1][0])
+    F.destroy()
+
+
+def test_autotuner_picks_and_keeps_results(gpu):
+    # 12 x 12 x 10 elements at lx1 = 9: 1,049,760 nodes, above the 2^20 threshold
+    lx1 = 9
+    b = box((12, 12, 10), lx1, deform=0.05)
+    F = gpu.box_field(b)
+    rng = np.random.default_rng(1)
+    u = F.to_dev(rng.standard_normal(F.shape))
+    h1, h2 = F.zeros() + 1.0, F.zeros() + 0.5
+    before = gpu.axhelm_tuning(lx1)
+    say(f"[autotune] lx1={lx1} nodes={F.shape} before={before}")
+    au = F.zeros()
+    F.axhelm(au, u, h1, h2)
+    variant, grid, tuned = gpu.axhelm_tuning(lx1)
+    say(f"[autotune] chosen variant={variant} grid={grid} tuned={tuned}")
+    assert tuned and variant in (0, 1) and grid >= 0
+    a_tuned = host(au)
+    for v in (0, 1):
+        gpu.set_axhelm_tuning(lx1, v, 0)
+        au2 = F.zeros()
+        F.axhelm(au2, u, h1, h2)
+        assert np.array_equal(host(au2), a_tuned)
+    gpu.set_axhelm_tuning(lx1, variant, grid)
+    F.destroy()
+
+
+def test_tuning_arguments_checked(gpu):
+    with pytest.raises(gpu.NekAmdError):
+        gpu.set_axhelm_tuning(17, 0, 0)
+    with pytest.raises(gpu.NekAmdError):
+        gpu.set_axhelm_tuning(8, 2, 0)
+    with pytest.raises(gpu.NekAmdError):
+        gpu.set_axhelm_tuning(8, 0, -1)
+    with pytest.raises(gpu.NekAmdError):
+        gpu.axhelm_tuning(1)
