@@ -177,41 +177,45 @@ def order_sweep():
 
 def adjoint_run(nel1d=32, lx1=8):
     """Adjoint coefficient gradient at E = 32^3, N = 7: -div(h1 grad u) = 1
-    (h2 = 0, Dirichlet), h1 = exp(0.5 * sum of 16 Fourier modes with
-    wavenumbers 1..4 / 4), J = 1/2 (u - u*)^T B (u - u*) with u* ~ U(-1,1)/2.
-    Forward and adjoint by the flexible CG with the h1mg V-cycle to 1e-12
-    relative (the same solver for both: the operator is self-adjoint), then
-    the gradient kernel; central differences (step 1e-5) along two
+    (h2 = 0, Dirichlet), h1 = exp(g), g = sum of 16 seeded Fourier modes of
+    amplitude 0.5 with wavenumbers 1..4; J = 1/2 (u - u*)^T B (u - u*) with
+    u* the solution for a second seeded coefficient field.  Forward and
+    adjoint by the flexible CG with the h1mg V-cycle to 1e-12 relative (the
+    same solver for both: the operator is self-adjoint), then the gradient
+    kernel; central differences (step 1e-6) along 20 seeded random
     directions check it.  Prints one JSON line."""
     import torch
     from nek5000_amd import solver
-    from nek5000_amd.mesh import BoxMesh, hash_uniform
+    from nek5000_amd.mesh import BoxMesh
 
     solver.init(0)
     box = BoxMesh(nel1d, nel1d, nel1d, lx1, deform=0.1)
     F = solver.box_field(box, multigrid=True)
     x, y, z = box.xyz()
-    rng = np.random.default_rng(5)
-    s = np.zeros_like(x)
-    for _ in range(16):
-        k = rng.integers(1, 5, size=3)
-        ph = rng.uniform(0, 2 * np.pi, size=3)
-        s += np.cos(np.pi * k[0] * x + ph[0]) * np.cos(np.pi * k[1] * y + ph[1]) * \
-            np.cos(np.pi * k[2] * z + ph[2])
-    h1 = F.to_dev(np.exp(0.5 * s / 4.0))
-    del x, y, z, s
+
+    def fourier(seed, nmodes):
+        rng = np.random.default_rng(seed)
+        s = np.zeros_like(x)
+        for _ in range(nmodes):
+            k = rng.integers(1, 5, size=3)
+            ph = rng.uniform(0, 2 * np.pi, size=3)
+            s += 0.5 * np.cos(np.pi * k[0] * x + ph[0]) * np.cos(np.pi * k[1] * y + ph[1]) * \
+                np.cos(np.pi * k[2] * z + ph[2])
+        return s
+
+    h1, h1star = F.to_dev(np.exp(fourier(5, 16))), F.to_dev(np.exp(fourier(6, 16)))
     h2 = F.zeros()
     bm1, _ = F.mass()
     f = bm1.clone()
-    uid = box.unique_ids(F.glo_num, F.ngv).reshape(-1)
-    ustar = F.to_dev((hash_uniform(7, uid) - 0.5).reshape(box.nel, lx1, lx1, lx1)) * F.mask
     prm = solver.default_params(istep=20, param42=2, param21=-1e-12)
 
     def hsolve(rhs, h):
         u = F.zeros()
-        out = F.hmholtz("PRES", u, rhs.clone(), h, h2, F.mask, F.mult, -1e-12, 300, params=prm)
+        out = F.hmholtz("PRES", u, rhs.clone(), h, h2, F.mask, F.mult, -1e-12, 2000, params=prm)
         assert out["converged"]
         return u, out["niter"]
+
+    ustar, _ = hsolve(f, h1star)
 
     def J(h):
         u, _ = hsolve(f, h)
@@ -236,29 +240,37 @@ def adjoint_run(nel1d=32, lx1=8):
     torch.cuda.synchronize()
     kern_ms = (time.perf_counter() - t3) / 5 * 1e3
     checks = []
-    for seed in (11, 12):
-        w = F.to_dev(hash_uniform(seed, uid).reshape(box.nel, lx1, lx1, lx1))
-        dh = (0.5 + w) * h1
-        eps = 1e-5
+    t4 = time.perf_counter()
+    for seed in range(11, 31):
+        # seeded smooth relative perturbations (4 random Fourier modes): a
+        # pointwise-random sign would cancel dJ down to the round-off of J
+        dh = F.to_dev(fourier(seed, 4)) * h1
+        eps = 1e-6
         fd = (J(h1 + eps * dh) - J(h1 - eps * dh)) / (2 * eps)
         ad = float(torch.sum(g * dh))
-        checks.append(dict(adjoint=ad, central_difference=fd, rel_diff=abs(fd - ad) / abs(ad)))
+        checks.append(dict(seed=seed, adjoint=ad, central_difference=fd,
+                           rel_diff=abs(fd - ad) / abs(ad)))
+    fd_s = time.perf_counter() - t4
+    del x, y, z
     line = dict(run="adjoint", workload=f"E={nel1d}^3 N={lx1 - 1} deformed box, h1=exp(Fourier)",
                 forward_ms=(t1 - t0) * 1e3, forward_iterations=it_f,
                 adjoint_ms=(t2 - t1) * 1e3, adjoint_iterations=it_a,
                 gradient_kernel_ms=kern_ms,
                 gradient_kernel_GBs=9 * 8 * box.n / (kern_ms * 1e-3) / 1e9,
+                fd_directions=len(checks), fd_step=1e-6,
+                fd_max_rel_diff=max(c["rel_diff"] for c in checks), fd_seconds=fd_s,
                 finite_difference_checks=checks)
     print(json.dumps(line), flush=True)
 
 
 def batch_run(steps=2):
-    """Five solves of one time step solved together (nkd_hmholtz_batch, the
-    iteration of all of them in one hipGraph): three velocity components on
-    a 48^3 box (h1 = 1e-3, h2 = 11/(6e-3)), a passive scalar on 24^3 (h1 =
-    1e-3) and one on a 40x40x20 x-periodic box (h1 = 1e-2); N = 7, f ~
-    U(-1,1) per unique node, Jacobi PCG to 1e-8 relative each, FP64.  The
-    same solves one after another through hmholtz for comparison.  Prints one
+    """Five solves of one time step (three velocity components on a 48^3 box,
+    h1 = 1e-3, h2 = 11/(6e-3); a passive scalar on 24^3, h1 = 1e-3; one on a
+    40x40x20 x-periodic box, h1 = 1e-2; N = 7, f ~ U(-1,1) per unique node,
+    Jacobi PCG to 1e-8 relative, at most 900 iterations as cggo's maxcg):
+    one after another through hmholtz in FP64 (the baseline), and batched
+    (nkd_hmholtz_batch: the iteration of all five in one hipGraph) in FP64
+    and in mixed precision (FP32 iteration, FP64 refreshes).  Prints one
     JSON line."""
     import torch
     from nek5000_amd import solver
@@ -281,39 +293,56 @@ def batch_run(steps=2):
             solves.append(dict(u=F.zeros(), rhs=rhs, h1=F.zeros() + h1v, h2=F.zeros() + h2v,
                                mask=F.mask, mult=F.mult, tli=-1e-8))
             dofs.append(free)
-    prm = solver.default_params(istep=20, param22=0.0, restol=0.0)
     B = solver.Batch(fields)
+    prm = {p: solver.default_params(istep=20, param22=0.0, restol=0.0, precision=p)
+           for p in (0, 2)}
 
-    def run_batch():
+    def run_batch(precision):
         qs = [dict(q, rhs=q["rhs"].clone()) for q in solves]
         torch.cuda.synchronize()
         t = time.perf_counter()
-        outs, launches = B.hmholtz(qs, 900, params=prm)
+        outs, launches = B.hmholtz(qs, 900, params=prm[precision])
         torch.cuda.synchronize()
         return time.perf_counter() - t, outs, launches
 
-    def run_seq():
+    def run_seq(precision):
         qs = [dict(q, rhs=q["rhs"].clone()) for q in solves]
         torch.cuda.synchronize()
         t = time.perf_counter()
         outs = [F.hmholtz("VELX", q["u"], q["rhs"], q["h1"], q["h2"], q["mask"], q["mult"],
-                          q["tli"], 900, params=prm) for F, q in zip(fields, qs)]
+                          q["tli"], 900, params=prm[precision]) for F, q in zip(fields, qs)]
         torch.cuda.synchronize()
-        return time.perf_counter() - t, outs
+        return time.perf_counter() - t, outs, None
 
-    run_batch()
-    run_seq()
-    tb = min(run_batch()[0] for _ in range(steps))
-    _, outs, launches = run_batch()
-    ts = min(run_seq()[0] for _ in range(steps))
-    its = [o["niter"] for o in outs]
-    work = sum(d * i for d, i in zip(dofs, its))
+    rows = {}
+    sols = {}
+    for tag, fn, precision in (("sequential_fp64", run_seq, 0), ("batched_fp64", run_batch, 0),
+                               ("batched_mixed", run_batch, 2),
+                               ("sequential_mixed", run_seq, 2)):
+        fn(precision)  # warm-up (graph capture, autotuning, FP32 factors)
+        best = None
+        for _ in range(steps):
+            r = fn(precision)
+            if best is None or r[0] < best[0]:
+                best = r
+        t, outs, launches = best
+        its = [o["niter"] for o in outs]
+        work = sum(d * i for d, i in zip(dofs, its))
+        rows[tag] = dict(seconds=t, iterations=its, graph_launches=launches,
+                         GDOF_iter_s=work / t / 1e9,
+                         converged=[bool(o["converged"]) for o in outs],
+                         rel_residual=[o["rbn2"] / o["rbn0"] for o in outs])
+        sols[tag] = [q["u"].clone() for q in solves]
+        log(f"[batch] {tag}: {rows[tag]}")
+    base = rows["sequential_fp64"]["seconds"]
+    for tag, row in rows.items():
+        row["speedup_vs_sequential_fp64"] = base / row["seconds"]
+        row["max_rel_diff_vs_fp64"] = max(
+            float(torch.linalg.norm(a - b) / torch.linalg.norm(b))
+            for a, b in zip(sols[tag], sols["sequential_fp64"]))
     line = dict(run="batch", workload="3 x velocity 48^3 + scalar 24^3 + scalar 40x40x20 "
-                                      "periodic, N=7, Jacobi PCG to 1e-8 relative, FP64",
-                iterations=its, graph_launches=launches,
-                batched_s=tb, sequential_s=ts, speedup=ts / tb,
-                batched_GDOF_iter_s=work / tb / 1e9, sequential_GDOF_iter_s=work / ts / 1e9,
-                converged=[bool(o["converged"]) for o in outs])
+                                      "periodic, N=7, Jacobi PCG to 1e-8 relative (maxcg 900)",
+                local_nodes=[int(F.n) for F in fields], free_dof=dofs, variants=rows)
     print(json.dumps(line), flush=True)
     B.destroy()
 

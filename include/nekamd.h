@@ -42,9 +42,12 @@ typedef struct {
   double restol;   /* restol(ifield): overrides tin when nonzero and tin>0 (hmholtz.f:677) */
   int precision;   /* 0: FP64 (the drop-in); 1: multigrid smoother (fine-level local FDM
                       solves and overlap sum) in FP32 inside the FP64 Krylov iteration;
-                      2 (flexible CG only): FP64 defect correction around single-precision
+                      2, flexible CG: FP64 defect correction around single-precision
                       inner solves (FP32 operator, fine-level V-cycle and vectors; FP64
-                      inner products, coarse solve and true residuals), same tolerance */
+                      inner products, coarse solve and true residuals); 2, Jacobi PCG:
+                      the iteration in FP32 (operator, vectors, preconditioner) with FP64
+                      inner products, solution and residual refreshes (the exit test is
+                      on the FP64 residual); both need uniform h1/h2, same tolerance */
   int param42;     /* param(42): PRES solver, 2 -> hmh_flex_cg with h1mg (hmholtz.f:646-653) */
   double param21;  /* param(21): PRES tolerance cap (>0) or relative tolerance (<0) (hmholtz.f:2206-2220) */
   int param40;     /* param(40): coarse solver of h1mg, 0 (XXT) -> solved to 1e-13, 1 (AMG) -> one
@@ -276,7 +279,7 @@ typedef struct {
    solves iterate together -- the steady-state iteration of the whole batch
    (five launches per solve, each solve on its own stream) is captured once
    in a hipGraph and replayed until every solve has stopped.  Iterates and
-   results equal those of nkd_hmholtz on each solve.  res: nb results;
+   results equal those of nkd_hmholtz on each solve (prm->precision 0 or 2).  res: nb results;
    *graph_launches (optional): iterations run for the slowest solve. */
 int nkd_hmholtz_batch(int batch, const nkh_batch_solve *solves, int maxit,
                       const nkh_params *prm, nkh_result *res, int *graph_launches);

@@ -11,7 +11,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 OUT = os.path.join(HERE, "libnekamd.so")
 OBJ = os.path.join(HERE, "_objs")
-SOURCES = ["capi.cpp", "kernels.hip", "axhelm.hip", "axhelm_cg.hip", "gs.hip", "cg.hip", "hsmg.hip",
+SOURCES = ["capi.cpp", "kernels.hip", "axhelm.hip", "axhelm_cg.hip", "axhelm_gated.hip", "gs.hip", "cg.hip", "hsmg.hip",
            "hsmg_setup.hip", "coarse.hip",
            "gmres.hip", "proj.hip", "adjoint.hip", "mesh_io.cpp", "mixed.hip"]
 

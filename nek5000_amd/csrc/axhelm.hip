@@ -47,14 +47,20 @@ static int tune_axhelm(const Mesh &m, double *au, const double *u, Coef h1, Coef
   return NKH_OK;
 }
 
-int launch_axhelm(const Mesh &m, double *au, const double *u, Coef h1, Coef h2,
-                  hipStream_t s) {
+int axhelm_autotune(const Mesh &m, double *au, const double *u, Coef h1, Coef h2,
+                    hipStream_t s) {
   if (m.np >= 2 && m.np <= kMaxNP && !g_axhelm_choice[m.np].tuned && m.n >= ((int64_t)1 << 20) &&
       au != u) {
     hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
     NK_HIP(hipStreamIsCapturing(s, &cs));
     if (cs == hipStreamCaptureStatusNone) NK_TRY(tune_axhelm(m, au, u, h1, h2, s));
   }
+  return NKH_OK;
+}
+
+int launch_axhelm(const Mesh &m, double *au, const double *u, Coef h1, Coef h2,
+                  hipStream_t s) {
+  NK_TRY(axhelm_autotune(m, au, u, h1, h2, s));
   return axhelm_apply(m, au, u, h1, h2, s);
 }
 

@@ -503,22 +503,21 @@ int nkh_init(int device) {
   return NKH_OK;
 }
 
+static void mesh_release(Mesh *m) {
+  mg_free(*m);
+  gs_free(m->gs);
+  for (void *p : {(void *)m->Dd, (void *)m->g, (void *)m->bm1, (void *)m->binv, (void *)m->ws,
+                  (void *)m->st, (void *)m->red, (void *)m->g32, (void *)m->bm1_32})
+    if (p) (void)hipFree(p);
+  delete m;
+}
+
 int nkh_finalize(void) {
   std::lock_guard<std::mutex> lk(g_mu);
   if (!g_init) return NKH_OK;
-  for (size_t i = 0; i < g_meshes.size(); ++i) {
-    Mesh *m = g_meshes[i];
-    if (!m) continue;
-    gs_free(m->gs);
-    if (m->Dd) (void)hipFree(m->Dd);
-    if (m->g) (void)hipFree(m->g);
-    if (m->bm1) (void)hipFree(m->bm1);
-    if (m->binv) (void)hipFree(m->binv);
-    if (m->ws) (void)hipFree(m->ws);
-    if (m->st) (void)hipFree(m->st);
-    if (m->red) (void)hipFree(m->red);
-    delete m;
-  }
+  (void)hipDeviceSynchronize();
+  for (size_t i = 0; i < g_meshes.size(); ++i)
+    if (g_meshes[i]) mesh_release(g_meshes[i]);
   g_meshes.clear();
   if (g_lib_stream) (void)hipStreamDestroy(g_lib_stream);
   g_lib_stream = g_stream = nullptr;
@@ -593,16 +592,7 @@ int nkh_mesh_destroy(int id) {
   Mesh *m = get_mesh(id);
   NK_CHECK(m, NKH_ERR_ARG, "nkh_mesh_destroy: bad mesh %d", id);
   NK_HIP(hipStreamSynchronize(g_stream));
-  mg_free(*m);
-  gs_free(m->gs);
-  if (m->Dd) (void)hipFree(m->Dd);
-  if (m->g) (void)hipFree(m->g);
-  if (m->bm1) (void)hipFree(m->bm1);
-  if (m->binv) (void)hipFree(m->binv);
-  if (m->ws) (void)hipFree(m->ws);
-  if (m->st) (void)hipFree(m->st);
-  if (m->red) (void)hipFree(m->red);
-  delete m;
+  mesh_release(m);
   g_meshes[(size_t)id] = nullptr;
   return NKH_OK;
 }
@@ -804,10 +794,11 @@ static int cggo_branches(Mesh &m, bool pres, double *x, const double *f, Coef c1
 
 static int check_solver_params(const nkh_params &p, bool pres, const char *who) {
   NK_CHECK(p.precision == 0 || (pres && p.precision == 1) ||
-               (pres && p.precision == 2 && p.param42 == 2),
+               (p.precision == 2 && (!pres || p.param42 == 2)),
            NKH_ERR_UNSUPPORTED,
-           "%s: precision=%d (1 for the multigrid pressure solvers, 2 for the flexible CG)", who,
-           p.precision);
+           "%s: precision=%d (1 for the multigrid pressure solvers, 2 for the Jacobi PCG and "
+           "the flexible CG)",
+           who, p.precision);
   NK_CHECK(!pres || p.param42 == 2 || p.param42 == 0, NKH_ERR_UNSUPPORTED,
            "%s: PRES with param42=%d (0 GMRES and 2 flexible CG with h1mg are built)", who,
            p.param42);
@@ -822,7 +813,8 @@ int nkd_cggo(int id, double *x, const double *f, const double *h1, const double 
   NK_CHECK(x && f && h1 && h2 && mask && mult, NKH_ERR_ARG, "nkd_cggo: null array");
   NK_CHECK(maxit >= 0, NKH_ERR_ARG, "nkd_cggo: maxit=%d", maxit);
   const nkh_params &p = prm ? *prm : kDefaultParams;
-  NK_CHECK(p.precision == 0, NKH_ERR_UNSUPPORTED, "nkd_cggo: precision=%d", p.precision);
+  NK_CHECK(p.precision == 0 || p.precision == 2, NKH_ERR_UNSUPPORTED,
+           "nkd_cggo: precision=%d", p.precision);
   Coef c1, c2;
   NK_TRY(classify_coefs(*m, h1, h2, &c1, &c2, g_stream));
   return run_cggo(*m, x, f, c1, c2, mask, mult, tin, maxit, p, res, history);
@@ -907,8 +899,8 @@ int nkd_hmholtz_batch(int batch, const nkh_batch_solve *solves, int maxit,
   NK_CHECK(g_init, NKH_ERR_STATE, "nkd_hmholtz_batch: library not initialised");
   NK_CHECK(solves && res && maxit >= 0, NKH_ERR_ARG, "nkd_hmholtz_batch: bad arguments");
   const nkh_params &p = prm ? *prm : kDefaultParams;
-  NK_CHECK(p.precision == 0, NKH_ERR_UNSUPPORTED, "nkd_hmholtz_batch: precision=%d",
-           p.precision);
+  NK_CHECK(p.precision == 0 || p.precision == 2, NKH_ERR_UNSUPPORTED,
+           "nkd_hmholtz_batch: precision=%d (0 FP64, 2 mixed)", p.precision);
   int nb = 0;
   std::vector<int> meshes;
   NK_TRY(batch_meshes(batch, &meshes));

@@ -338,6 +338,217 @@ __global__ void __launch_bounds__(256)
   if (threadIdx.x == 0) part[blockIdx.x] = fmax(fmax(sb[0], sb[1]), fmax(sb[2], sb[3]));
 }
 
+// ---- mixed precision: FP32 iteration, FP64 solution and true residual ----
+// The Jacobi PCG iterates in single precision (r, p, w, the correction xs
+// accumulated since the last refresh, the preconditioner; FP32 operator),
+// with every inner product accumulated in FP64.  A refresh -- x += xs in
+// FP64, r = f - mask*QQ^T A_L x in FP64 (FP64 axhelm and assembly), the
+// iterated residual replaced by it -- runs when the iterated residual has
+// fallen by kRefresh since the last one, when it meets the tolerance (the
+// solve ends only if the FP64 residual meets it too) and when maxit is
+// reached.  The search direction survives a refresh (reliable updates:
+// Sleijpen & van der Vorst 1996; Clark et al. 2010 for single/double CG).
+struct McgState {
+  CgState cg;    // the iteration (read back by the host as a CgState)
+  CgState gate;  // gate.done == 0 while a refresh is due (gates its kernels)
+  double rmax;   // largest iterated residual norm since the last refresh
+  int final;     // 1: iterated residual met the tolerance; 2: maxit reached
+  int nrefresh;
+};
+constexpr double kRefresh = 0.1;
+
+// xs += alpha p ; r -= alpha w (FP32) ; FP64 partials of dm*r*r and bm*r*r
+template <bool VEC, bool UPD>
+__global__ void __launch_bounds__(256)
+    k_mcg_update_dots(float *__restrict__ xs, float *__restrict__ r,
+                      const float *__restrict__ p, const float *__restrict__ w,
+                      const float *__restrict__ dm, const float *__restrict__ bm, int64_t n,
+                      const CgState *__restrict__ st, double *__restrict__ part1,
+                      double *__restrict__ part2) {
+  if (st->done) return;
+  const float alpha = (float)st->alpha;
+  double s1 = 0.0, s2 = 0.0;
+  const int64_t tid = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  int64_t done_upto = 0;
+  if constexpr (VEC) {
+    const int64_t n4 = n >> 2;
+    float4 *x4 = reinterpret_cast<float4 *>(xs);
+    float4 *r4 = reinterpret_cast<float4 *>(r);
+    const float4 *p4 = reinterpret_cast<const float4 *>(p);
+    const float4 *w4 = reinterpret_cast<const float4 *>(w);
+    const float4 *d4 = reinterpret_cast<const float4 *>(dm);
+    const float4 *b4 = reinterpret_cast<const float4 *>(bm);
+    for (int64_t i = tid; i < n4; i += stride) {
+      float4 ri = r4[i];
+      if constexpr (UPD) {
+        float4 xi = x4[i];
+        const float4 pi = p4[i], wi = w4[i];
+        xi.x += alpha * pi.x;
+        xi.y += alpha * pi.y;
+        xi.z += alpha * pi.z;
+        xi.w += alpha * pi.w;
+        ri.x -= alpha * wi.x;
+        ri.y -= alpha * wi.y;
+        ri.z -= alpha * wi.z;
+        ri.w -= alpha * wi.w;
+        x4[i] = xi;
+        r4[i] = ri;
+      }
+      const float4 d = d4[i], b = b4[i];
+      const double q0 = (double)ri.x * ri.x, q1 = (double)ri.y * ri.y,
+                   q2 = (double)ri.z * ri.z, q3 = (double)ri.w * ri.w;
+      s1 += d.x * q0 + d.y * q1 + d.z * q2 + d.w * q3;
+      s2 += b.x * q0 + b.y * q1 + b.z * q2 + b.w * q3;
+    }
+    done_upto = n4 * 4;
+  }
+  for (int64_t i = done_upto + tid; i < n; i += stride) {
+    float ri = r[i];
+    if constexpr (UPD) {
+      xs[i] = xs[i] + alpha * p[i];
+      ri = ri - alpha * w[i];
+      r[i] = ri;
+    }
+    const double q = (double)ri * ri;
+    s1 += dm[i] * q;
+    s2 += bm[i] * q;
+  }
+  block_store2(s1, s2, part1, part2);
+}
+
+// as k_cg_scalars1, but a refresh of the residual is requested instead of
+// ending the solve or when the iterated residual fell by kRefresh
+__global__ void __launch_bounds__(kScalThreads)
+    k_mcg_scalars1(McgState *__restrict__ ms, const double *__restrict__ part1,
+                   const double *__restrict__ part2, double *__restrict__ hist) {
+  CgState *st = &ms->cg;
+  if (st->done) return;
+  const double s1 = block_sum_array(part1, kUpdBlocks);
+  const double s2 = block_sum_array(part2, kUpdBlocks);
+  if (threadIdx.x != 0) return;
+  const int iter = st->iter + 1;
+  st->iter = iter;
+  if (iter > st->maxit) {  // loop exhausted: fold the correction into x
+    ms->final = 2;
+    ms->gate.done = 0;
+    return;
+  }
+  st->rtz2 = st->rtz1;
+  st->rtz1 = s1;
+  const double rbn2 = sqrt(s2 / st->vol);
+  st->rbn2 = rbn2;
+  if (iter == 1) {
+    st->rbn0 = rbn2;
+    ms->rmax = rbn2;
+  }
+  if (st->param22 < 0) st->tol = fabs(st->param22) * st->rbn0;
+  if (st->tin < 0) st->tol = fabs(st->tin) * st->rbn0;
+  if (hist) {
+    hist[5 * (iter - 1) + 0] = rbn2;
+    hist[5 * (iter - 1) + 1] = s1;
+  }
+  if (rbn2 <= st->tol && (iter > 1 || st->istep <= 5)) {
+    ms->final = 1;
+    ms->gate.done = 0;
+    return;
+  }
+  if (rbn2 < kRefresh * ms->rmax) {
+    ms->final = 0;
+    ms->gate.done = 0;
+    return;
+  }
+  if (rbn2 > ms->rmax) ms->rmax = rbn2;
+  st->beta = (iter == 1) ? 0.0 : st->rtz1 / st->rtz2;
+}
+
+// refresh, part 1: x += xs (FP64), xs = 0
+__global__ void k_mcg_fold(double *__restrict__ x, float *__restrict__ xs, int64_t n,
+                           const CgState *__restrict__ gate) {
+  if (gate->done) return;
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    x[i] = x[i] + (double)xs[i];
+    xs[i] = 0.0f;
+  }
+}
+
+// refresh, part 2 (after ax = mask*QQ^T A_L x): r = f - ax in FP64, stored
+// rounded; partials of dm*r32^2 (the iteration's rtz) and bm*r^2 (FP64 norm)
+__global__ void __launch_bounds__(256)
+    k_mcg_resid(float *__restrict__ r32, const double *__restrict__ f,
+                const double *__restrict__ ax, const float *__restrict__ dm,
+                const float *__restrict__ bm, int64_t n, const CgState *__restrict__ gate,
+                double *__restrict__ part1, double *__restrict__ part2) {
+  if (gate->done) return;
+  double s1 = 0.0, s2 = 0.0;
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    const double ri = f[i] - ax[i];
+    const float rf = (float)ri;
+    r32[i] = rf;
+    s1 += dm[i] * ((double)rf * rf);
+    s2 += bm[i] * (ri * ri);
+  }
+  block_store2(s1, s2, part1, part2);
+}
+
+// refresh, part 3: the true residual's norm decides the exit; otherwise the
+// iteration continues from it with the same search direction
+__global__ void __launch_bounds__(kScalThreads)
+    k_mcg_scalars1b(McgState *__restrict__ ms, const double *__restrict__ part1,
+                    const double *__restrict__ part2, double *__restrict__ hist) {
+  if (ms->gate.done) return;
+  const double s1 = block_sum_array(part1, kUpdBlocks);
+  const double s2 = block_sum_array(part2, kUpdBlocks);
+  if (threadIdx.x != 0) return;
+  CgState *st = &ms->cg;
+  ms->gate.done = 1;
+  ms->nrefresh += 1;
+  const double rbn2 = sqrt(s2 / st->vol);
+  st->rbn2 = rbn2;
+  if (ms->final == 2) {
+    st->done = 1;
+    st->niter = st->maxit;
+    return;
+  }
+  const int iter = st->iter;
+  st->rtz1 = s1;
+  if (hist) {
+    hist[5 * (iter - 1) + 0] = rbn2;
+    hist[5 * (iter - 1) + 1] = s1;
+  }
+  if (ms->final == 1 && rbn2 <= st->tol) {
+    st->done = 1;
+    st->converged = 1;
+    st->niter = iter - 1;
+    return;
+  }
+  ms->final = 0;
+  ms->rmax = rbn2;
+  st->beta = (iter == 1) ? 0.0 : st->rtz1 / st->rtz2;
+}
+
+// x = 0, xs = 0, p = 0, r = f (rounded), d, d*mult and binv*mult (or mult)
+// in FP32
+__global__ void k_mcg_init(double *__restrict__ x, float *__restrict__ xs,
+                           float *__restrict__ p, float *__restrict__ r,
+                           float *__restrict__ d32, float *__restrict__ dm,
+                           float *__restrict__ bm, const double *__restrict__ f,
+                           const double *__restrict__ d, const double *__restrict__ mult,
+                           const double *__restrict__ binv, int plain_norm, int64_t n) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    x[i] = 0.0;
+    xs[i] = 0.0f;
+    p[i] = 0.0f;
+    r[i] = (float)f[i];
+    d32[i] = (float)d[i];
+    dm[i] = (float)(d[i] * mult[i]);
+    bm[i] = (float)(plain_norm ? mult[i] : binv[i] * mult[i]);
+  }
+}
+
 int grid_for(int64_t n) {
   int64_t g = (n + 255) / 256;
   if (g > 8192) g = 8192;
@@ -362,14 +573,15 @@ int reduce_host(const Mesh &m, const double *a, const double *b, const double *c
 }  // namespace
 
 int ensure_small(Mesh &m) {
-  if (!m.st) NK_HIP(hipMalloc(&m.st, sizeof(CgState)));
+  // room for the mixed-precision solve's state (a CgState first)
+  if (!m.st) NK_HIP(hipMalloc(&m.st, sizeof(McgState)));
   if (!m.red) NK_HIP(hipMalloc(&m.red, sizeof(double) * kRedSize));
   return NKH_OK;
 }
 
 int ensure_workspace(Mesh &m) {
   NK_TRY(ensure_small(m));
-  const size_t need = (size_t)6 * m.n;
+  const size_t need = cg_ws_doubles(m.n);
   if (m.ws && m.ws_doubles >= need) return NKH_OK;
   if (m.ws) (void)hipFree(m.ws);
   m.ws = nullptr;
@@ -605,12 +817,155 @@ struct CgRun {
   }
 };
 
-int run_cggo(Mesh &m, double *x, const double *f, Coef h1, Coef h2, const double *mask,
-             const double *mult, double tin, int maxit, const nkh_params &prm,
-             nkh_result *res, double *history_host) {
+// The mixed-precision Jacobi PCG (see McgState), enqueued like CgRun.
+// Work space (cg_ws_doubles): ax (n doubles, FP64 refresh and set-up
+// scratch), then seven FP32 vectors r, p, w, xs, d, d*mult, bm at 256-byte
+// aligned offsets.  Needs uniform h1/h2 and the fused iteration's conditions.
+struct McgRun {
+  Mesh *m = nullptr;
+  double *x = nullptr, *ax = nullptr;
+  const double *f = nullptr;
+  float *r = nullptr, *p = nullptr, *w = nullptr, *xs = nullptr, *d = nullptr, *dm = nullptr,
+        *bm = nullptr;
+  double *red = nullptr;
+  McgState *ms = nullptr;
+  CgState *st = nullptr;  // &ms->cg
+  const uint8_t *flag = nullptr;
+  Coef h1, h2;
+  bool trivial = false, vec_ok = false, fused = true;
+  int niter = 0, launched = 0, ax_grid_parts = 0;
+  double *hist = nullptr;
+
+  int begin(Mesh &mesh, double *ws, double *red_, void *st_, double *x_, const double *f_,
+            Coef h1_, Coef h2_, const double *mask, const double *mult, double tin, int maxit,
+            const nkh_params &prm, bool want_hist, hipStream_t s) {
+    m = &mesh;
+    const int64_t n = mesh.n;
+    NK_CHECK(h1_.uniform() && h2_.uniform(), NKH_ERR_UNSUPPORTED,
+             "mixed-precision cggo: h1 and h2 must be uniform");
+    x = x_;
+    f = f_;
+    ax = ws;
+    const int64_t nn = (n + 63) / 64 * 64;
+    float *base = reinterpret_cast<float *>(ws + (n + 31) / 32 * 32);
+    r = base;
+    p = base + nn;
+    w = base + 2 * nn;
+    xs = base + 3 * nn;
+    d = base + 4 * nn;
+    dm = base + 5 * nn;
+    bm = base + 6 * nn;
+    red = red_;
+    ms = static_cast<McgState *>(st_);
+    st = &ms->cg;
+    h1 = h1_;
+    h2 = h2_;
+    double tol = std::fabs(tin);
+    if (prm.restol != 0.0) tol = prm.restol;
+    if (tin < 0) tol = std::fabs(tin);
+    niter = maxit < kMaxCg ? maxit : kMaxCg;
+    NK_TRY(run_setprec(mesh, ax, h1, h2, s));  // d (FP64) in the ax slot
+    hipLaunchKernelGGL(k_mcg_init, dim3(grid_for(n)), dim3(256), 0, s, x, xs, p, r, d, dm, bm, f,
+                       ax, mult, mesh.binv, prm.param18 == 1 ? 1 : 0, n);
+    NK_HIP(hipGetLastError());
+    double fmax;
+    NK_TRY(reduce_host(mesh, f, nullptr, nullptr, n, 1, &fmax, s));
+    trivial = fmax == 0.0;
+    if (trivial) return NKH_OK;
+    double skmin;
+    NK_TRY(reduce_host(mesh, mask, nullptr, nullptr, n, 3, &skmin, s));
+    NK_CHECK(!(skmin > 0 && h2.s == 0.0), NKH_ERR_UNSUPPORTED,
+             "cggo: pure-Neumann mean correction (ifmcor) is not implemented");
+    bool ok = false;
+    NK_TRY(fused_conditions(mesh, mask, mult, f, &ok, s));
+    NK_CHECK(ok, NKH_ERR_UNSUPPORTED,
+             "mixed-precision cggo: needs ids zero at element interiors, 0/1 masks and "
+             "mult summing to 1 per group");
+    flag = mesh.gs.flag;
+    NK_TRY(ensure_geom32(mesh, s));
+    ax_grid_parts = axhelm_cg_grid(mesh);
+    auto al16 = [](const void *q) { return (reinterpret_cast<uintptr_t>(q) & 15) == 0; };
+    vec_ok = al16(r) && al16(p) && al16(w) && al16(xs) && al16(dm) && al16(bm);
+    McgState h{};
+    h.cg.rtz1 = 1.0;
+    h.cg.tol = tol;
+    h.cg.tin = tin;
+    h.cg.vol = mesh.vol;
+    h.cg.param22 = prm.param22;
+    h.cg.maxit = niter;
+    h.cg.istep = prm.istep;
+    h.cg.param18 = prm.param18;
+    h.gate.done = 1;
+    NK_HIP(hipMemcpyAsync(ms, &h, sizeof(McgState), hipMemcpyHostToDevice, s));
+    hist = nullptr;
+    if (want_hist) {
+      hist = red + kRedHist;
+      hipLaunchKernelGGL(k_fill, dim3(grid_for(5 * kMaxCg)), dim3(256), 0, s, hist, 0.0,
+                         (int64_t)5 * kMaxCg);
+    }
+    NK_HIP(hipGetLastError());
+    return NKH_OK;
+  }
+
+  int enqueue(hipStream_t s, hipEvent_t *ev = nullptr) {
+    const int64_t n = m->n;
+    double *part1 = red + kRedPart1, *part2 = red + kRedPart2, *rho1 = red + kRedRho1,
+           *rho2 = red + kRedRho2;
+    auto kern = vec_ok ? (launched == 0 ? k_mcg_update_dots<true, false>
+                                        : k_mcg_update_dots<true, true>)
+                       : (launched == 0 ? k_mcg_update_dots<false, false>
+                                        : k_mcg_update_dots<false, true>);
+    hipLaunchKernelGGL(kern, dim3(kUpdBlocks), dim3(256), 0, s, xs, r, p, w, dm, bm, n, st, part1,
+                       part2);
+    hipLaunchKernelGGL(k_mcg_scalars1, dim3(1), dim3(kScalThreads), 0, s, ms, part1, part2, hist);
+    // the refresh (its kernels return at once while the gate is closed)
+    const CgState *gate = &ms->gate;
+    hipLaunchKernelGGL(k_mcg_fold, dim3(grid_for(n)), dim3(256), 0, s, x, xs, n, gate);
+    NK_TRY(launch_axhelm_gated(*m, ax, x, h1, h2, gate, s));
+    NK_TRY(gs_assemble(m->gs, ax, gate, s, flag));
+    hipLaunchKernelGGL(k_mcg_resid, dim3(kUpdBlocks), dim3(256), 0, s, r, f, ax, dm, bm, n, gate,
+                       part1, part2);
+    hipLaunchKernelGGL(k_mcg_scalars1b, dim3(1), dim3(kScalThreads), 0, s, ms, part1, part2,
+                       hist);
+    if (ev) NK_HIP(hipEventRecord(ev[0], s));
+    NK_TRY(launch_axhelm32_jac(*m, w, p, r, d, (float)h1.s, (float)h2.s, st, rho1, s));
+    if (ev) NK_HIP(hipEventRecord(ev[1], s));
+    NK_TRY(gs_assemble(m->gs, w, st, s, flag));
+    if (ev) NK_HIP(hipEventRecord(ev[2], s));
+    hipLaunchKernelGGL(k_cg_scalars2, dim3(1), dim3(kScalThreads), 0, s, st, rho1,
+                       ax_grid_parts, rho2, 0, hist);
+    NK_HIP(hipGetLastError());
+    ++launched;
+    return NKH_OK;
+  }
+
+  void result(const CgState &hs, nkh_result *res) const {
+    nkh_result out{};
+    if (!trivial) {
+      out.niter = hs.niter;
+      out.converged = hs.converged;
+      out.rbn0 = hs.rbn0;
+      out.rbn2 = hs.rbn2;
+      out.tol = hs.tol;
+    }
+    if (res) *res = out;
+  }
+};
+
+size_t cg_ws_doubles(int64_t n) {
+  const size_t mixed = (size_t)((n + 31) / 32 * 32) + (size_t)(7 * ((n + 63) / 64 * 64) + 1) / 2;
+  return std::max((size_t)6 * (size_t)n, mixed);
+}
+
+namespace {
+
+template <class Run>
+int drive_cggo(Mesh &m, double *x, const double *f, Coef h1, Coef h2, const double *mask,
+               const double *mult, double tin, int maxit, const nkh_params &prm,
+               nkh_result *res, double *history_host) {
   hipStream_t s = cur_stream();
   NK_TRY(ensure_workspace(m));
-  CgRun run;
+  Run run;
   NK_TRY(run.begin(m, m.ws, m.red, m.st, x, f, h1, h2, mask, mult, tin, maxit, prm,
                    history_host != nullptr, s));
   if (run.trivial) {
@@ -670,6 +1025,16 @@ int run_cggo(Mesh &m, double *x, const double *f, Coef h1, Coef h2, const double
     NK_HIP(hipStreamSynchronize(s));
   }
   return NKH_OK;
+}
+
+}  // namespace
+
+int run_cggo(Mesh &m, double *x, const double *f, Coef h1, Coef h2, const double *mask,
+             const double *mult, double tin, int maxit, const nkh_params &prm,
+             nkh_result *res, double *history_host) {
+  if (prm.precision == 2)
+    return drive_cggo<McgRun>(m, x, f, h1, h2, mask, mult, tin, maxit, prm, res, history_host);
+  return drive_cggo<CgRun>(m, x, f, h1, h2, mask, mult, tin, maxit, prm, res, history_host);
 }
 
 // ---------------------------------------------------------------------------
@@ -739,9 +1104,9 @@ int batch_create(int nb, const int *meshes, int *id) {
     CgState *c = nullptr;
     uint8_t *fl = nullptr;
     hipStream_t q = nullptr;
-    if (hipMalloc(&w, sizeof(double) * 6 * (size_t)m->n) != hipSuccess ||
+    if (hipMalloc(&w, sizeof(double) * cg_ws_doubles(m->n)) != hipSuccess ||
         hipMalloc(&r, sizeof(double) * kRedSize) != hipSuccess ||
-        hipMalloc(&c, sizeof(CgState)) != hipSuccess ||
+        hipMalloc(&c, sizeof(McgState)) != hipSuccess ||
         hipMalloc(&fl, (size_t)std::max<int64_t>(m->gs.ngroups, 1)) != hipSuccess ||
         hipStreamCreateWithFlags(&q, hipStreamNonBlocking) != hipSuccess) {
       (void)hipGetLastError();
@@ -793,15 +1158,15 @@ int batch_destroy(int id) {
   return NKH_OK;
 }
 
-int run_cggo_batch(int id, const double *const *x, const double *const *f, const Coef *h1,
-                   const Coef *h2, const double *const *mask, const double *const *mult,
-                   const double *tin, int maxit, const nkh_params &prm, nkh_result *res,
-                   int *launches_out) {
-  NK_CHECK(id >= 0 && id < (int)g_batches.size() && g_batches[(size_t)id], NKH_ERR_ARG,
-           "batch: bad handle %d", id);
-  CgBatch &B = *g_batches[(size_t)id];
+namespace {
+
+template <class Run>
+int batch_drive(CgBatch &B, const double *const *x, const double *const *f, const Coef *h1,
+                const Coef *h2, const double *const *mask, const double *const *mult,
+                const double *tin, int maxit, const nkh_params &prm, nkh_result *res,
+                int *launches_out) {
   hipStream_t s = cur_stream();
-  std::vector<CgRun> runs((size_t)B.nb);
+  std::vector<Run> runs((size_t)B.nb);
   std::vector<int> active;
   for (int k = 0; k < B.nb; ++k) {
     Mesh *m = get_mesh(B.mesh[(size_t)k]);
@@ -891,6 +1256,20 @@ int run_cggo_batch(int id, const double *const *x, const double *const *f, const
     if (runs[(size_t)k].trivial) runs[(size_t)k].result(CgState{}, &res[k]);
   if (launches_out) *launches_out = launches;
   return NKH_OK;
+}
+
+}  // namespace
+
+int run_cggo_batch(int id, const double *const *x, const double *const *f, const Coef *h1,
+                   const Coef *h2, const double *const *mask, const double *const *mult,
+                   const double *tin, int maxit, const nkh_params &prm, nkh_result *res,
+                   int *launches_out) {
+  NK_CHECK(id >= 0 && id < (int)g_batches.size() && g_batches[(size_t)id], NKH_ERR_ARG,
+           "batch: bad handle %d", id);
+  CgBatch &B = *g_batches[(size_t)id];
+  if (prm.precision == 2)
+    return batch_drive<McgRun>(B, x, f, h1, h2, mask, mult, tin, maxit, prm, res, launches_out);
+  return batch_drive<CgRun>(B, x, f, h1, h2, mask, mult, tin, maxit, prm, res, launches_out);
 }
 
 }  // namespace nekamd

@@ -39,6 +39,12 @@ __device__ __forceinline__ double block_sum(double v, double *sbuf) {
   return sbuf[0];
 }
 
+// Streaming accesses of the element kernels: every field word is read or
+// written once per apply, so loads and stores carry the nontemporal hint
+// (measured on MI355X: +9-17 % on a plain copy).
+__device__ __forceinline__ double ldnt(const double *p) { return __builtin_nontemporal_load(p); }
+__device__ __forceinline__ void stnt(double v, double *p) { __builtin_nontemporal_store(v, p); }
+
 // ---------------------------------------------------------------------------
 // axhelm: hmholtz.f:188-218 (3-D deformed branch) and :225 (h2 term).
 //   tmp1 = h1*(g1*ur + g4*us + g5*ut)   (same order as col3/addcol3 there)
@@ -50,7 +56,8 @@ __device__ __forceinline__ double block_sum(double v, double *sbuf) {
 // rho = (Q Q^T w, p)_mult for a continuous, masked p -- is written to one
 // slot per block when rho_part is given.  MODE 2 (flexible CG): the same
 // with p = beta*p + z (add2s1, hmholtz.f:2243), z passed as r, beta read
-// from *betap.
+// from *betap.  MODE 3: MODE 0 gated by a state (returns while st->done),
+// for applies that a captured iteration runs only on some passes.
 // ---------------------------------------------------------------------------
 template <int NP, int MODE, bool H1F, int H2M>
 __global__ void __launch_bounds__(NP * NP * epb<NP>())
@@ -75,6 +82,9 @@ __global__ void __launch_bounds__(NP * NP * epb<NP>())
     beta = st->beta;
   }
   if constexpr (MODE == 2) beta = *betap;
+  if constexpr (MODE == 3) {
+    if (st->done) return;
+  }
   const int tid = threadIdx.x;
   const int le = tid / NP2;
   const int ij = tid - le * NP2;
@@ -94,13 +104,14 @@ __global__ void __launch_bounds__(NP * NP * epb<NP>())
       if (valid) {
         const int64_t l = base + k * NP2;
         if constexpr (MODE == 1) {
-          v = beta * uin[l] + dinv[l] * r[l];  // add2s1: p = beta*p + z, z = r*d
-          pout[l] = v;
+          // add2s1: p = beta*p + z, z = r*d
+          v = beta * ldnt(uin + l) + ldnt(dinv + l) * ldnt(r + l);
+          stnt(v, pout + l);
         } else if constexpr (MODE == 2) {
-          v = beta * uin[l] + r[l];
-          pout[l] = v;
+          v = beta * ldnt(uin + l) + ldnt(r + l);
+          stnt(v, pout + l);
         } else {
-          v = uin[l];
+          v = ldnt(uin + l);
         }
       }
       r_u[k] = v;
@@ -120,9 +131,10 @@ __global__ void __launch_bounds__(NP * NP * epb<NP>())
       double wr = 0.0, ws = 0.0, wt = 0.0;
       if (valid) {
         const int64_t l = base + k * NP2;
-        const double G1 = g[l], G2 = g[gstride + l], G3 = g[2 * gstride + l];
-        const double G4 = g[3 * gstride + l], G5 = g[4 * gstride + l],
-                     G6 = g[5 * gstride + l];
+        const double G1 = ldnt(g + l), G2 = ldnt(g + gstride + l),
+                     G3 = ldnt(g + 2 * gstride + l);
+        const double G4 = ldnt(g + 3 * gstride + l), G5 = ldnt(g + 4 * gstride + l),
+                     G6 = ldnt(g + 5 * gstride + l);
         wr = ur * G1;
         ws = us * G2;
         wt = ut * G3;
@@ -132,7 +144,7 @@ __global__ void __launch_bounds__(NP * NP * epb<NP>())
         ws = ws + ut * G6;
         wt = wt + ur * G5;
         wt = wt + us * G6;
-        const double h1 = H1F ? h1f[l] : h1s;
+        const double h1 = H1F ? ldnt(h1f + l) : h1s;
         wr *= h1;
         ws *= h1;
         wt *= h1;
@@ -154,14 +166,14 @@ __global__ void __launch_bounds__(NP * NP * epb<NP>())
       if (valid) {
         const int64_t l = base + k * NP2;
         double a = ar + as + r_au[k];
-        if constexpr (H2M == 1) a = a + (h2s * bm1[l]) * r_u[k];
-        if constexpr (H2M == 2) a = a + (h2f[l] * bm1[l]) * r_u[k];
-        au[l] = a;
-        if constexpr (MODE >= 1) rho += a * r_u[k];  // p . (A_L p), element-local
+        if constexpr (H2M == 1) a = a + (h2s * ldnt(bm1 + l)) * r_u[k];
+        if constexpr (H2M == 2) a = a + (ldnt(h2f + l) * ldnt(bm1 + l)) * r_u[k];
+        stnt(a, au + l);
+        if constexpr (MODE == 1 || MODE == 2) rho += a * r_u[k];  // p . (A_L p), element-local
       }
     }
   }
-  if constexpr (MODE >= 1) {
+  if constexpr (MODE == 1 || MODE == 2) {
     if (rho_part != nullptr) {
       const double t = block_sum(rho, s_red);
       if (threadIdx.x == 0) rho_part[blockIdx.x] = t;
@@ -196,6 +208,9 @@ __global__ void __launch_bounds__(NP * NP * epb<NP>())
     beta = st->beta;
   }
   if constexpr (MODE == 2) beta = *betap;
+  if constexpr (MODE == 3) {
+    if (st->done) return;
+  }
   const int tid = threadIdx.x;
   const int le = tid / NP2;
   const int ij = tid - le * NP2;
@@ -213,13 +228,13 @@ __global__ void __launch_bounds__(NP * NP * epb<NP>())
       if (valid) {
         const int64_t l = base + k * NP2;
         if constexpr (MODE == 1) {
-          v = beta * uin[l] + dinv[l] * r[l];
-          pout[l] = v;
+          v = beta * ldnt(uin + l) + ldnt(dinv + l) * ldnt(r + l);
+          stnt(v, pout + l);
         } else if constexpr (MODE == 2) {
-          v = beta * uin[l] + r[l];
-          pout[l] = v;
+          v = beta * ldnt(uin + l) + ldnt(r + l);
+          stnt(v, pout + l);
         } else {
-          v = uin[l];
+          v = ldnt(uin + l);
         }
       }
       r_u[k] = v;
@@ -243,9 +258,10 @@ __global__ void __launch_bounds__(NP * NP * epb<NP>())
       double wr = 0.0, ws = 0.0, wt = 0.0;
       if (valid) {
         const int64_t l = base + k * NP2;
-        const double G1 = g[l], G2 = g[gstride + l], G3 = g[2 * gstride + l];
-        const double G4 = g[3 * gstride + l], G5 = g[4 * gstride + l],
-                     G6 = g[5 * gstride + l];
+        const double G1 = ldnt(g + l), G2 = ldnt(g + gstride + l),
+                     G3 = ldnt(g + 2 * gstride + l);
+        const double G4 = ldnt(g + 3 * gstride + l), G5 = ldnt(g + 4 * gstride + l),
+                     G6 = ldnt(g + 5 * gstride + l);
         wr = ur * G1;
         ws = us * G2;
         wt = ut * G3;
@@ -255,7 +271,7 @@ __global__ void __launch_bounds__(NP * NP * epb<NP>())
         ws = ws + ut * G6;
         wt = wt + ur * G5;
         wt = wt + us * G6;
-        const double h1 = H1F ? h1f[l] : h1s;
+        const double h1 = H1F ? ldnt(h1f + l) : h1s;
         wr *= h1;
         ws *= h1;
         wt *= h1;
@@ -283,13 +299,14 @@ __global__ void __launch_bounds__(NP * NP * epb<NP>())
     for (int k = 0; k < NP; ++k) {
       const int64_t l = base + k * NP2;
       double a = r_o[k] + r_au[k];
-      if constexpr (H2M == 1) a = a + (h2s * (valid ? bm1[l] : 0.0)) * r_u[k];
-      if constexpr (H2M == 2) a = a + ((valid ? h2f[l] : 0.0) * (valid ? bm1[l] : 0.0)) * r_u[k];
-      if (valid) au[l] = a;
-      if constexpr (MODE >= 1) rho += a * r_u[k];  // a = r_u = 0 off the mesh
+      if constexpr (H2M == 1) a = a + (h2s * (valid ? ldnt(bm1 + l) : 0.0)) * r_u[k];
+      if constexpr (H2M == 2)
+        a = a + ((valid ? ldnt(h2f + l) : 0.0) * (valid ? ldnt(bm1 + l) : 0.0)) * r_u[k];
+      if (valid) stnt(a, au + l);
+      if constexpr (MODE == 1 || MODE == 2) rho += a * r_u[k];  // a = r_u = 0 off the mesh
     }
   }
-  if constexpr (MODE >= 1) {
+  if constexpr (MODE == 1 || MODE == 2) {
     if (rho_part != nullptr) {
       const double t = block_sum(rho, s_red);
       if (threadIdx.x == 0) rho_part[blockIdx.x] = t;

@@ -285,8 +285,9 @@ int gs_assemble(const GsMap &gs, double *w, const CgState *st, hipStream_t s,
   return gs_assemble_t(gs, w, st, s, flag);
 }
 
-int gs_assemble(const GsMap &gs, float *w, const CgState *st, hipStream_t s) {
-  return gs_assemble_t(gs, w, st, s, (const uint8_t *)nullptr);
+int gs_assemble(const GsMap &gs, float *w, const CgState *st, hipStream_t s,
+                const uint8_t *flag) {
+  return gs_assemble_t(gs, w, st, s, flag);
 }
 
 template <typename E, typename Z, typename W>

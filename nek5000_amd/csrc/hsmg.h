@@ -71,13 +71,10 @@ struct Mg {
   // GMRES work: V (kGmresM+1 fields), Z (kGmresM fields), x, r, w, partials
   double *gV = nullptr, *gZ = nullptr, *gx = nullptr, *gr = nullptr, *gw = nullptr;
   double *gpart = nullptr, *gsm = nullptr;
-  // mixed-precision solver: single-precision inner vectors, the operator's
-  // geometric factors in single precision (rebuilt when the mesh geometry
-  // changes), partial sums and scalars
+  // mixed-precision solver: single-precision inner vectors, partial sums
+  // and scalars (the FP32 geometric factors belong to the mesh)
   float *xr = nullptr, *xp = nullptr, *xz = nullptr, *xw = nullptr, *xx = nullptr,
         *xwt = nullptr;
-  float *g32 = nullptr, *bm1_32 = nullptr;
-  int g32_version = -1;
   double *xpart = nullptr;
   void *xst = nullptr;
 };

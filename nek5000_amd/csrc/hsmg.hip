@@ -768,6 +768,9 @@ int run_flex_cg(Mesh &m, double *res, Coef h1, Coef h2, const double *wt, const 
   const double *mgmask = mg.lev[mg.lmax].mask;
   double *r = mg.fr, *p = mg.fp, *z = mg.fz, *w = mg.fw;
   double *pa = mg.fpart, *pb = mg.fpart + kFcgBlocks;
+  // the operator's kernel choice for this order (w is scratch until the
+  // first apply of the iteration)
+  NK_TRY(axhelm_autotune(m, w, res, h1, h2, s));
   // hmholtz.f:2205-2208: tolps from chktcg1 (pmask, vmult), capped by param(21)
   double tolps = 0.0;
   NK_TRY(run_chktcg1(m, &tolps, res, h1, h2, mask, wt));

@@ -515,8 +515,7 @@ void mg_free(Mesh &mesh) {
                   (void *)mg->fflag, (void *)mg->gV, (void *)mg->gZ, (void *)mg->gx,
                   (void *)mg->gr, (void *)mg->gw, (void *)mg->gpart, (void *)mg->gsm,
                   (void *)mg->xr, (void *)mg->xp, (void *)mg->xz, (void *)mg->xw,
-                  (void *)mg->xx, (void *)mg->xwt, (void *)mg->g32, (void *)mg->bm1_32,
-                  (void *)mg->xpart, mg->xst})
+                  (void *)mg->xx, (void *)mg->xwt, (void *)mg->xpart, mg->xst})
     if (p) (void)hipFree(p);
   if (mg->ev_fork) (void)hipEventDestroy(mg->ev_fork);
   if (mg->ev_join) (void)hipEventDestroy(mg->ev_join);

@@ -181,23 +181,30 @@ __global__ void __launch_bounds__(256)
   part1(a, part);
 }
 
-// Single-precision axhelm of the inner flexible CG: p = beta*p + z (beta
-// from the state, FP64 -> FP32), w = A_L p with the FP32 factors (h1, h2
-// uniform), partial sums of p.w in FP64 per block.  Same element slab as the
-// FP64 kernel.
-template <int NP, bool H2>
+}  // namespace
+
+// Single-precision axhelm of the inner Krylov loops: p = beta*p + z
+// (flexible CG) or, JAC, p = d*r + beta*p (Jacobi CG, z = r; returns while
+// *donep), beta read in FP64 and rounded; w = A_L p with the FP32 factors
+// (h1, h2 uniform), partial sums of p.w in FP64 per block.  Same element
+// slab as the FP64 kernel.
+template <int NP, bool H2, bool JAC>
 __global__ void __launch_bounds__(NP * NP * epb<NP>())
     k_axhelm32(DMatF<NP> Dk, const double *__restrict__ Dg, float *__restrict__ w,
-               float *__restrict__ p, const float *__restrict__ z, const float *__restrict__ g,
-               int64_t gstride, float h1, float h2, const float *__restrict__ bm1, int64_t nel,
-               const MxState *__restrict__ st, double *__restrict__ den_part) {
+               float *__restrict__ p, const float *__restrict__ z, const float *__restrict__ dd,
+               const float *__restrict__ g, int64_t gstride, float h1, float h2,
+               const float *__restrict__ bm1, int64_t nel, const double *__restrict__ betap,
+               const int *__restrict__ donep, double *__restrict__ den_part) {
   constexpr int NP2 = NP * NP, NP3 = NP2 * NP, EPB = epb<NP>();
   __shared__ float s_D[NP][NP + 1];
   __shared__ float s_u[EPB][NP3];
   __shared__ float s_wr[EPB][NP3];
   __shared__ float s_ws[EPB][NP3];
   __shared__ double s_red[NP2 * EPB];
-  const float beta = (float)st->beta;
+  if constexpr (JAC) {
+    if (*donep) return;
+  }
+  const float beta = (float)*betap;
   const int tid = threadIdx.x;
   const int le = tid / NP2;
   const int ij = tid - le * NP2;
@@ -215,7 +222,8 @@ __global__ void __launch_bounds__(NP * NP * epb<NP>())
       float v = 0.0f;
       if (valid) {
         const int64_t l = base + k * NP2;
-        v = beta * p[l] + z[l];
+        if constexpr (JAC) v = dd[l] * z[l] + beta * p[l];
+        else v = beta * p[l] + z[l];
         p[l] = v;
       }
       r_u[k] = v;
@@ -268,29 +276,66 @@ __global__ void __launch_bounds__(NP * NP * epb<NP>())
   if (threadIdx.x == 0) den_part[blockIdx.x] = t;
 }
 
-template <int NP>
-int axhelm32_dispatch(const Mesh &m, const Mg &mg, float *w, float *p, const float *z, float h1,
-                      float h2, const MxState *st, double *den_part, hipStream_t s) {
+template <int NP, bool JAC>
+static int axhelm32_dispatch(const Mesh &m, float *w, float *p, const float *z, const float *dd,
+                             float h1, float h2, const double *betap, const int *donep,
+                             double *den_part, hipStream_t s) {
   DMatF<NP> Dm;
   for (int t = 0; t < NP * NP; ++t) Dm.d[t] = (float)m.D[t];
   const int grid = axhelm_cg_grid(m);
   const dim3 block(NP * NP * epb<NP>());
   if (h2 != 0.0f)
-    hipLaunchKernelGGL((k_axhelm32<NP, true>), dim3(grid), block, 0, s, Dm, m.Dd, w, p, z,
-                       mg.g32, m.n, h1, h2, mg.bm1_32, m.nel, st, den_part);
+    hipLaunchKernelGGL((k_axhelm32<NP, true, JAC>), dim3(grid), block, 0, s, Dm, m.Dd, w, p, z,
+                       dd, m.g32, m.n, h1, h2, m.bm1_32, m.nel, betap, donep, den_part);
   else
-    hipLaunchKernelGGL((k_axhelm32<NP, false>), dim3(grid), block, 0, s, Dm, m.Dd, w, p, z,
-                       mg.g32, m.n, h1, h2, mg.bm1_32, m.nel, st, den_part);
+    hipLaunchKernelGGL((k_axhelm32<NP, false, JAC>), dim3(grid), block, 0, s, Dm, m.Dd, w, p, z,
+                       dd, m.g32, m.n, h1, h2, m.bm1_32, m.nel, betap, donep, den_part);
   NK_HIP(hipGetLastError());
   return NKH_OK;
 }
 
-int launch_axhelm32(const Mesh &m, const Mg &mg, float *w, float *p, const float *z, float h1,
-                    float h2, const MxState *st, double *den_part, hipStream_t s) {
-#define CALL(NP) axhelm32_dispatch<NP>(m, mg, w, p, z, h1, h2, st, den_part, s)
+int launch_axhelm32_jac(const Mesh &m, float *w, float *p, const float *r, const float *d,
+                        float h1, float h2, const CgState *st, double *rho_part,
+                        hipStream_t s) {
+#define CALL(NP) \
+  axhelm32_dispatch<NP, true>(m, w, p, r, d, h1, h2, &st->beta, &st->done, rho_part, s)
   NK_NP_SWITCH(m.np, CALL)
 #undef CALL
 }
+
+__global__ void k_to_f32(float *__restrict__ d, const double *__restrict__ s, int64_t n) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x)
+    d[i] = (float)s[i];
+}
+
+int ensure_geom32(Mesh &m, hipStream_t s) {
+  const int64_t n = m.n;
+  if (!m.g32) {
+    if (hipMalloc((void **)&m.g32, sizeof(float) * 6 * (size_t)n) != hipSuccess ||
+        hipMalloc((void **)&m.bm1_32, sizeof(float) * (size_t)n) != hipSuccess) {
+      (void)hipGetLastError();
+      if (m.g32) (void)hipFree(m.g32);
+      m.g32 = nullptr;
+      m.bm1_32 = nullptr;
+      set_error("FP32 operator: out of device memory (%lld floats)", (long long)(7 * n));
+      return NKH_ERR_NOMEM;
+    }
+    m.g32_version = -1;
+  }
+  if (m.g32_version != m.geom_version) {
+    const int64_t g6 = 6 * n;
+    const int grid6 = (int)std::min<int64_t>((g6 + 255) / 256, 65536);
+    const int grid1 = (int)std::min<int64_t>((n + 255) / 256, 65536);
+    hipLaunchKernelGGL(k_to_f32, dim3(grid6), dim3(256), 0, s, m.g32, m.g, g6);
+    hipLaunchKernelGGL(k_to_f32, dim3(grid1), dim3(256), 0, s, m.bm1_32, m.bm1, n);
+    NK_HIP(hipGetLastError());
+    m.g32_version = m.geom_version;
+  }
+  return NKH_OK;
+}
+
+namespace {
 
 int falloc(float **p, int64_t n) {
   if (hipMalloc((void **)p, sizeof(float) * (size_t)n) != hipSuccess) {
@@ -299,6 +344,14 @@ int falloc(float **p, int64_t n) {
     return NKH_ERR_NOMEM;
   }
   return NKH_OK;
+}
+
+int launch_axhelm32_flex(const Mesh &m, float *w, float *p, const float *z, float h1, float h2,
+                         const MxState *st, double *den_part, hipStream_t s) {
+#define CALL(NP) \
+  axhelm32_dispatch<NP, false>(m, w, p, z, nullptr, h1, h2, &st->beta, nullptr, den_part, s)
+  NK_NP_SWITCH(m.np, CALL)
+#undef CALL
 }
 
 }  // namespace
@@ -336,20 +389,13 @@ int run_flex_cg_mixed(Mesh &m, double *res, Coef h1, Coef h2, const double *wt,
   }
   if (!mg.xr) {
     for (float **p : {&mg.xr, &mg.xp, &mg.xz, &mg.xw, &mg.xx, &mg.xwt}) NK_TRY(falloc(p, n));
-    NK_TRY(falloc(&mg.g32, 6 * n));
-    NK_TRY(falloc(&mg.bm1_32, n));
     NK_TRY(mg_dalloc(&mg.xpart, 3 * kMxBlocks + kAxBlocks));
     NK_HIP(hipMalloc(&mg.xst, sizeof(MxState)));
   }
   if (!mg.fr)
     for (double **p : {&mg.fr, &mg.fp, &mg.fz, &mg.fw}) NK_TRY(mg_dalloc(p, n));
   const dim3 G(kMxBlocks), B(256);
-  if (mg.g32_version != m.geom_version) {
-    hipLaunchKernelGGL(k_mx_to_f32, dim3(mg_grid(6 * n)), B, 0, s, mg.g32, m.g, 6 * n);
-    hipLaunchKernelGGL(k_mx_to_f32, dim3(mg_grid(n)), B, 0, s, mg.bm1_32, m.bm1, n);
-    NK_HIP(hipGetLastError());
-    mg.g32_version = m.geom_version;
-  }
+  NK_TRY(ensure_geom32(m, s));
   hipLaunchKernelGGL(k_mx_to_f32, dim3(mg_grid(n)), B, 0, s, mg.xwt, wt, n);
   MxState *st = (MxState *)mg.xst;
   double *pa = mg.xpart, *pb = mg.xpart + kMxBlocks, *pc = mg.xpart + 2 * kMxBlocks;
@@ -389,7 +435,7 @@ int run_flex_cg_mixed(Mesh &m, double *res, Coef h1, Coef h2, const double *wt,
       NK_TRY(mg_solve32(m, mg.xz, mg.xr, s, prm.param40));
       hipLaunchKernelGGL(k_mx_dots, G, B, 0, s, mg.xz, mg.xr, mg.xw, mg.xwt, n, pa, pb);
       hipLaunchKernelGGL(k_mx_beta, dim3(1), B, 0, s, st, pa, pb);
-      NK_TRY(launch_axhelm32(m, mg, mg.xw, mg.xp, mg.xz, h1f, h2f, st, den_part, s));
+      NK_TRY(launch_axhelm32_flex(m, mg.xw, mg.xp, mg.xz, h1f, h2f, st, den_part, s));
       NK_TRY(gs_assemble(m.gs, mg.xw, nullptr, s));
       hipLaunchKernelGGL(k_mx_alpha, dim3(1), B, 0, s, st, den_part, nden);
       hipLaunchKernelGGL(k_mx_update, G, B, 0, s, mg.xx, mg.xr, mg.xp, mg.xw, mg.xwt, st, n, pa);

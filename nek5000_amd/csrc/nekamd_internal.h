@@ -118,6 +118,10 @@ struct Mesh {
   CgState *st = nullptr;       // device
   double *red = nullptr;       // device, partial sums
   Mg *mg = nullptr;            // H1 multigrid hierarchy (nkh_mg_setup)
+  // FP32 copies of g and bm1 for the single-precision operator (built on
+  // first use, rebuilt when geom_version moves)
+  float *g32 = nullptr, *bm1_32 = nullptr;
+  int g32_version = -1;
 };
 
 // CTIMER-style accumulators (seconds, calls), see nkh_timers_*.
@@ -139,6 +143,7 @@ inline int axhelm_cg_grid(const Mesh &m) {
 Mesh *get_mesh(int id);
 hipStream_t cur_stream();
 int ensure_workspace(Mesh &m);  // solve vectors + state + partial sums
+size_t cg_ws_doubles(int64_t n);  // work space of one Jacobi-PCG solve (doubles)
 int ensure_small(Mesh &m);      // state + partial sums only
 
 // ---- kernels.hip ----
@@ -155,7 +160,20 @@ int launch_axhelm(const Mesh &m, double *au, const double *u, Coef h1, Coef h2,
 // the plain apply's grid (0: one block per element group); tuned on the
 // first apply of the order on a mesh of >= 2^20 nodes unless set first.
 int axhelm_tuning(int np, int *variant, int *grid, int *tuned);
+// Tune the order of m now if that is due (au is overwritten, u only read).
+int axhelm_autotune(const Mesh &m, double *au, const double *u, Coef h1, Coef h2,
+                    hipStream_t s);
 int axhelm_set_tuning(int np, int variant, int grid);
+// MODE 0 apply that returns while gate->done (replayed in captured loops).
+int launch_axhelm_gated(const Mesh &m, double *au, const double *u, Coef h1, Coef h2,
+                        const CgState *gate, hipStream_t s);
+// FP32 operator (mixed.hip): g32/bm1_32 current for the mesh's geometry.
+int ensure_geom32(Mesh &m, hipStream_t s);
+// FP32 Jacobi-CG apply: p = d*r + beta*p (beta = st->beta), w = A_L p with
+// uniform h1/h2, FP64 partial sums of p.w per block; returns while st->done.
+int launch_axhelm32_jac(const Mesh &m, float *w, float *p, const float *r, const float *d,
+                        float h1, float h2, const CgState *st, double *rho_part,
+                        hipStream_t s);
 int launch_geom(Mesh &m, const double *x, const double *y, const double *z, hipStream_t s);
 int launch_setprec(const Mesh &m, double *d, Coef h1, Coef h2, hipStream_t s);
 // axhelm fused into the CG iteration: p = d*r + beta*p; w = A p (h1/h2 as
@@ -184,7 +202,8 @@ int gs_op(const GsMap &gs, double *u, int op, hipStream_t s);
 // flag: group flags to use instead of gs.flag (a solve with its own mask)
 int gs_assemble(const GsMap &gs, double *w, const CgState *st, hipStream_t s,
                 const uint8_t *flag = nullptr);
-int gs_assemble(const GsMap &gs, float *w, const CgState *st, hipStream_t s);
+int gs_assemble(const GsMap &gs, float *w, const CgState *st, hipStream_t s,
+                const uint8_t *flag = nullptr);
 // Per id group (single members included): S = sum of e; z_i <- mean over
 // the members of S*w_i + z_i.  The fine-level end of the multigrid V-cycle.
 int gs_fin(const GsMap &gs, double *z, const double *e, const double *w, hipStream_t s);
