@@ -424,8 +424,10 @@ def main():
     log(f"[bench] free unique DOF = {free_dof:.0f}")
 
     def params_for(crs, precision=0):
-        return (solver.default_params(param42=2, param21=-args.tol, param40=crs,
-                                      precision=precision) if hsmg else None)
+        if hsmg:
+            return solver.default_params(param42=2, param21=-args.tol, param40=crs,
+                                         precision=precision)
+        return solver.default_params(precision=precision)
 
     prm = params_for(args.crs, args.precision)
 
@@ -473,14 +475,17 @@ def main():
     t_ax = sec.value / max(calls.value, 1)
     lib.nkh_timers_read(b"dsum", ctypes.byref(sec), ctypes.byref(calls))
     t_gs = sec.value / max(calls.value, 1)
+    # the kernel the autotuner chose for this order in the warm-up solve
+    variant, _, _ = solver.axhelm_tuning(args.lx1)
+    kname = "k_axhelm_pl" if variant == 1 else "k_axhelm"
     if hsmg:
         # axhelm of ax (gmres.f:284) with the p update fused: reads p, z,
         # g1..g6, bm1; writes p, w
         bpn = 8.0 * 11
-        ax_kernel = f"k_axhelm<{args.lx1},2,false,1> (flex-CG axhelm, p = beta p + z fused)"
+        ax_kernel = f"{kname}<{args.lx1},2,false,1> (flex-CG axhelm, p = beta p + z fused)"
     else:
         bpn = fused_axhelm_bytes_per_node(args.lx1)
-        ax_kernel = f"k_axhelm<{args.lx1},1,false,1> (CG-fused axhelm)"
+        ax_kernel = f"{kname}<{args.lx1},1,false,1> (CG-fused axhelm)"
     ax_bytes = bpn * box.n
     # (the mixed solver's FP32 operator records no axhelm events: no roofline
     # entry for a --precision 2 main line)
@@ -489,7 +494,7 @@ def main():
     value = free_dof * total_iters / elapsed / 1e9
     mg_info = F.h1mg_info() if hsmg else None
     variants = []
-    if hsmg and not args.no_variants:
+    if not args.no_variants:
         # beside the main line (not its value): the other coarse solver, and
         # the mixed-precision smoother; one warm-up and two timed solves each
         def measure(label, p):
@@ -511,15 +516,48 @@ def main():
             log(f"[bench] {v}")
             variants.append(v)
 
-        other = 1 - args.crs
-        measure("coarse: " + ("one AMG V-cycle (param40=1)" if other == 1 else
-                              "AMG-PCG to 1e-13"), params_for(other))
-        measure("mixed: FP32 fine-level smoother, FP64 Krylov and coarse",
-                params_for(args.crs, 1))
-        measure("mixed + one-V-cycle coarse", params_for(1, 1))
-        measure("mixed: FP32 inner flexible CG (operator, fine-level V-cycle, vectors) in FP64 "
-                "defect correction, FP64 coarse", params_for(args.crs, 2))
-        measure("mixed FP32 inner solve + one-V-cycle coarse", params_for(1, 2))
+        if hsmg:
+            other = 1 - args.crs
+            measure("coarse: " + ("one AMG V-cycle (param40=1)" if other == 1 else
+                                  "AMG-PCG to 1e-13"), params_for(other))
+            measure("mixed: FP32 fine-level smoother, FP64 Krylov and coarse",
+                    params_for(args.crs, 1))
+            measure("mixed + one-V-cycle coarse", params_for(1, 1))
+            measure("mixed: FP32 inner flexible CG (operator, fine-level V-cycle, vectors) in "
+                    "FP64 defect correction, FP64 coarse", params_for(args.crs, 2))
+            measure("mixed FP32 inner solve + one-V-cycle coarse", params_for(1, 2))
+        else:
+            measure("mixed: FP32 Jacobi-PCG iteration (operator, vectors), FP64 dots and "
+                    "residual refreshes", params_for(0, 2))
+
+    host_io = None
+    if not args.no_variants:
+        # the same solve through the host-array entry point (nkh_hmholtz:
+        # rhs, h1, h2, mask, mult in and u, rhs out over PCIe, pageable
+        # numpy memory): the PCIe-inclusive rate, never the bench value
+        arrs = {k: np.ascontiguousarray(t.cpu().numpy()) for k, t in
+                (("rhs", rhs0), ("h1", h1), ("h2", h2), ("mask", F.mask), ("mult", F.mult))}
+        u_h = np.zeros_like(arrs["rhs"])
+        res = _lib.NkhResult()
+
+        def host_solve():
+            r_h = arrs["rhs"].copy()
+            t = time.perf_counter()
+            _lib.check(lib.nkh_hmholtz(F.handle, b"PRES" if hsmg else b"TEMP",
+                                       u_h.ctypes.data, r_h.ctypes.data, arrs["h1"].ctypes.data,
+                                       arrs["h2"].ctypes.data, arrs["mask"].ctypes.data,
+                                       arrs["mult"].ctypes.data, -args.tol, args.maxit, 1,
+                                       ctypes.byref(prm), ctypes.byref(res), None), "nkh_hmholtz")
+            return time.perf_counter() - t
+
+        host_solve()
+        th = host_solve()
+        host_io = {"entry": "nkh_hmholtz (host arrays: 5 fields in, 2 out)",
+                   "seconds_per_solve": th, "iterations": res.niter,
+                   "value": free_dof * res.niter / th / 1e9,
+                   "vs_device_arrays": (elapsed / args.steps) / th}
+        log(f"[bench] host arrays {host_io}")
+        del arrs, u_h
 
     traffic = None
     if hsmg and args.nel == 64 and args.lx1 == 8:
@@ -562,6 +600,7 @@ def main():
             "axhelm_ms": t_ax * 1e3, "gs_ms": t_gs * 1e3,
             "ms_per_iteration": elapsed / total_iters * 1e3,
             "copy_GBs": copy_gbs,
+            "host_arrays": host_io,
         },
         "roofline": {
             "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",

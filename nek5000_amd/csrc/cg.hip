@@ -193,26 +193,28 @@ __global__ void __launch_bounds__(256)
   const int64_t stride = (int64_t)gridDim.x * blockDim.x;
   int64_t done_upto = 0;
   if constexpr (VEC) {
+    // node pairs as native 2-vectors, streamed with nontemporal hints
+    typedef double v2 __attribute__((ext_vector_type(2)));
     const int64_t n2 = n >> 1;
-    double2 *x2 = reinterpret_cast<double2 *>(x);
-    double2 *r2 = reinterpret_cast<double2 *>(r);
-    const double2 *p2 = reinterpret_cast<const double2 *>(p);
-    const double2 *w2 = reinterpret_cast<const double2 *>(w);
-    const double2 *dm2 = reinterpret_cast<const double2 *>(dm);
-    const double2 *bm2 = reinterpret_cast<const double2 *>(bm);
+    v2 *x2 = reinterpret_cast<v2 *>(x);
+    v2 *r2 = reinterpret_cast<v2 *>(r);
+    const v2 *p2 = reinterpret_cast<const v2 *>(p);
+    const v2 *w2 = reinterpret_cast<const v2 *>(w);
+    const v2 *dm2 = reinterpret_cast<const v2 *>(dm);
+    const v2 *bm2 = reinterpret_cast<const v2 *>(bm);
     for (int64_t i = tid; i < n2; i += stride) {
-      double2 ri = r2[i];
+      v2 ri = __builtin_nontemporal_load(r2 + i);
       if constexpr (upd) {
-        double2 xi = x2[i];
-        const double2 pi = p2[i], wi = w2[i];
+        v2 xi = __builtin_nontemporal_load(x2 + i);
+        const v2 pi = __builtin_nontemporal_load(p2 + i), wi = __builtin_nontemporal_load(w2 + i);
         xi.x = xi.x + alpha * pi.x;
         xi.y = xi.y + alpha * pi.y;
         ri.x = ri.x + (-alpha) * wi.x;
         ri.y = ri.y + (-alpha) * wi.y;
-        x2[i] = xi;
-        r2[i] = ri;
+        __builtin_nontemporal_store(xi, x2 + i);
+        __builtin_nontemporal_store(ri, r2 + i);
       }
-      const double2 d = dm2[i], b = bm2[i];
+      const v2 d = __builtin_nontemporal_load(dm2 + i), b = __builtin_nontemporal_load(bm2 + i);
       const double rr0 = ri.x * ri.x, rr1 = ri.y * ri.y;
       s1 += d.x * rr0;
       s2 += b.x * rr0;
@@ -356,6 +358,9 @@ struct McgState {
   int nrefresh;
 };
 constexpr double kRefresh = 0.1;
+// grid of the refresh kernels: they run every pass of a captured loop and
+// mostly return at once, at a cost that grows with the number of blocks
+constexpr int kRefreshBlocks = 2048;
 
 // xs += alpha p ; r -= alpha w (FP32) ; FP64 partials of dm*r*r and bm*r*r
 template <bool VEC, bool UPD>
@@ -372,18 +377,19 @@ __global__ void __launch_bounds__(256)
   const int64_t stride = (int64_t)gridDim.x * blockDim.x;
   int64_t done_upto = 0;
   if constexpr (VEC) {
+    typedef float v4 __attribute__((ext_vector_type(4)));
     const int64_t n4 = n >> 2;
-    float4 *x4 = reinterpret_cast<float4 *>(xs);
-    float4 *r4 = reinterpret_cast<float4 *>(r);
-    const float4 *p4 = reinterpret_cast<const float4 *>(p);
-    const float4 *w4 = reinterpret_cast<const float4 *>(w);
-    const float4 *d4 = reinterpret_cast<const float4 *>(dm);
-    const float4 *b4 = reinterpret_cast<const float4 *>(bm);
+    v4 *x4 = reinterpret_cast<v4 *>(xs);
+    v4 *r4 = reinterpret_cast<v4 *>(r);
+    const v4 *p4 = reinterpret_cast<const v4 *>(p);
+    const v4 *w4 = reinterpret_cast<const v4 *>(w);
+    const v4 *d4 = reinterpret_cast<const v4 *>(dm);
+    const v4 *b4 = reinterpret_cast<const v4 *>(bm);
     for (int64_t i = tid; i < n4; i += stride) {
-      float4 ri = r4[i];
+      v4 ri = __builtin_nontemporal_load(r4 + i);
       if constexpr (UPD) {
-        float4 xi = x4[i];
-        const float4 pi = p4[i], wi = w4[i];
+        v4 xi = __builtin_nontemporal_load(x4 + i);
+        const v4 pi = __builtin_nontemporal_load(p4 + i), wi = __builtin_nontemporal_load(w4 + i);
         xi.x += alpha * pi.x;
         xi.y += alpha * pi.y;
         xi.z += alpha * pi.z;
@@ -392,10 +398,10 @@ __global__ void __launch_bounds__(256)
         ri.y -= alpha * wi.y;
         ri.z -= alpha * wi.z;
         ri.w -= alpha * wi.w;
-        x4[i] = xi;
-        r4[i] = ri;
+        __builtin_nontemporal_store(xi, x4 + i);
+        __builtin_nontemporal_store(ri, r4 + i);
       }
-      const float4 d = d4[i], b = b4[i];
+      const v4 d = __builtin_nontemporal_load(d4 + i), b = __builtin_nontemporal_load(b4 + i);
       const double q0 = (double)ri.x * ri.x, q1 = (double)ri.y * ri.y,
                    q2 = (double)ri.z * ri.z, q3 = (double)ri.w * ri.w;
       s1 += d.x * q0 + d.y * q1 + d.z * q2 + d.w * q3;
@@ -499,8 +505,8 @@ __global__ void __launch_bounds__(kScalThreads)
     k_mcg_scalars1b(McgState *__restrict__ ms, const double *__restrict__ part1,
                     const double *__restrict__ part2, double *__restrict__ hist) {
   if (ms->gate.done) return;
-  const double s1 = block_sum_array(part1, kUpdBlocks);
-  const double s2 = block_sum_array(part2, kUpdBlocks);
+  const double s1 = block_sum_array(part1, kRefreshBlocks);
+  const double s2 = block_sum_array(part2, kRefreshBlocks);
   if (threadIdx.x != 0) return;
   CgState *st = &ms->cg;
   ms->gate.done = 1;
@@ -920,11 +926,11 @@ struct McgRun {
     hipLaunchKernelGGL(k_mcg_scalars1, dim3(1), dim3(kScalThreads), 0, s, ms, part1, part2, hist);
     // the refresh (its kernels return at once while the gate is closed)
     const CgState *gate = &ms->gate;
-    hipLaunchKernelGGL(k_mcg_fold, dim3(grid_for(n)), dim3(256), 0, s, x, xs, n, gate);
+    hipLaunchKernelGGL(k_mcg_fold, dim3(kRefreshBlocks), dim3(256), 0, s, x, xs, n, gate);
     NK_TRY(launch_axhelm_gated(*m, ax, x, h1, h2, gate, s));
-    NK_TRY(gs_assemble(m->gs, ax, gate, s, flag));
-    hipLaunchKernelGGL(k_mcg_resid, dim3(kUpdBlocks), dim3(256), 0, s, r, f, ax, dm, bm, n, gate,
-                       part1, part2);
+    NK_TRY(gs_assemble(m->gs, ax, gate, s, flag, kRefreshBlocks));
+    hipLaunchKernelGGL(k_mcg_resid, dim3(kRefreshBlocks), dim3(256), 0, s, r, f, ax, dm, bm, n,
+                       gate, part1, part2);
     hipLaunchKernelGGL(k_mcg_scalars1b, dim3(1), dim3(kScalThreads), 0, s, ms, part1, part2,
                        hist);
     if (ev) NK_HIP(hipEventRecord(ev[0], s));

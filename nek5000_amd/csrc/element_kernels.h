@@ -335,7 +335,11 @@ static int axhelm_dispatch(const Mesh &m, double *au, const double *u, Coef h1, 
   const int64_t ngroups = (m.nel + EPB - 1) / EPB;
   const AxhelmChoice ch = g_axhelm_choice[NP];
   int grid;
-  if (MODE >= 1) {
+  if (MODE == 3) {
+    // gated: most launches find the gate closed, and an empty launch costs
+    // in proportion to its blocks
+    grid = (int)(ngroups < 4096 ? ngroups : 4096);
+  } else if (MODE >= 1) {
     grid = axhelm_cg_grid(m);
   } else {
     const int64_t cap = ch.grid > 0 ? ch.grid : ((int64_t)1 << 20);

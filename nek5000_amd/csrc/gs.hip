@@ -270,9 +270,9 @@ int gs_prepare(GsMap &gs, const double *mask, const double *mult, double *red, b
 
 template <typename T>
 static int gs_assemble_t(const GsMap &gs, T *w, const CgState *st, hipStream_t s,
-                         const uint8_t *flag) {
+                         const uint8_t *flag, int nblocks = 0) {
   if (gs.ngroups == 0) return NKH_OK;
-  const int nb = kGsBlocks;
+  const int nb = nblocks > 0 ? nblocks : kGsBlocks;
   const int64_t chunk = (gs.ngroups + nb - 1) / nb;
   hipLaunchKernelGGL(k_gs_assemble<T>, dim3(nb), dim3(256), 0, s, w, gs.off, gs.idx,
                      flag ? flag : gs.flag, gs.ngroups, chunk, st);
@@ -281,8 +281,8 @@ static int gs_assemble_t(const GsMap &gs, T *w, const CgState *st, hipStream_t s
 }
 
 int gs_assemble(const GsMap &gs, double *w, const CgState *st, hipStream_t s,
-                const uint8_t *flag) {
-  return gs_assemble_t(gs, w, st, s, flag);
+                const uint8_t *flag, int nblocks) {
+  return gs_assemble_t(gs, w, st, s, flag, nblocks);
 }
 
 int gs_assemble(const GsMap &gs, float *w, const CgState *st, hipStream_t s,

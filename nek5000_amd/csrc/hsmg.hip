@@ -730,11 +730,15 @@ __global__ void __launch_bounds__(256)
                  int64_t n, double *part) {
   const double alpha = st->alpha;
   double a = 0.0;
+  // streaming: every word is touched once, hence the nontemporal hints
   NK_GRID_LOOP(i, n) {
-    res[i] = res[i] + alpha * p[i];
-    const double ri = r[i] - alpha * w[i];
-    r[i] = ri;
-    a = a + ri * ri * wt[i];
+    __builtin_nontemporal_store(__builtin_nontemporal_load(res + i) +
+                                    alpha * __builtin_nontemporal_load(p + i),
+                                res + i);
+    const double ri =
+        __builtin_nontemporal_load(r + i) - alpha * __builtin_nontemporal_load(w + i);
+    __builtin_nontemporal_store(ri, r + i);
+    a = a + ri * ri * __builtin_nontemporal_load(wt + i);
   }
   block_part(a, part);
 }

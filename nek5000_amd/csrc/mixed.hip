@@ -188,6 +188,8 @@ __global__ void __launch_bounds__(256)
 // *donep), beta read in FP64 and rounded; w = A_L p with the FP32 factors
 // (h1, h2 uniform), partial sums of p.w in FP64 per block.  Same element
 // slab as the FP64 kernel.
+__device__ __forceinline__ float ldnt32(const float *q) { return __builtin_nontemporal_load(q); }
+
 template <int NP, bool H2, bool JAC>
 __global__ void __launch_bounds__(NP * NP * epb<NP>())
     k_axhelm32(DMatF<NP> Dk, const double *__restrict__ Dg, float *__restrict__ w,
@@ -222,9 +224,9 @@ __global__ void __launch_bounds__(NP * NP * epb<NP>())
       float v = 0.0f;
       if (valid) {
         const int64_t l = base + k * NP2;
-        if constexpr (JAC) v = dd[l] * z[l] + beta * p[l];
-        else v = beta * p[l] + z[l];
-        p[l] = v;
+        if constexpr (JAC) v = ldnt32(dd + l) * ldnt32(z + l) + beta * ldnt32(p + l);
+        else v = beta * ldnt32(p + l) + ldnt32(z + l);
+        __builtin_nontemporal_store(v, p + l);
       }
       r_u[k] = v;
       s_u[le][k * NP2 + ij] = v;
@@ -243,8 +245,10 @@ __global__ void __launch_bounds__(NP * NP * epb<NP>())
       float wr = 0.0f, ws = 0.0f, wt = 0.0f;
       if (valid) {
         const int64_t l = base + k * NP2;
-        const float G1 = g[l], G2 = g[gstride + l], G3 = g[2 * gstride + l];
-        const float G4 = g[3 * gstride + l], G5 = g[4 * gstride + l], G6 = g[5 * gstride + l];
+        const float G1 = ldnt32(g + l), G2 = ldnt32(g + gstride + l),
+                    G3 = ldnt32(g + 2 * gstride + l);
+        const float G4 = ldnt32(g + 3 * gstride + l), G5 = ldnt32(g + 4 * gstride + l),
+                    G6 = ldnt32(g + 5 * gstride + l);
         wr = h1 * (ur * G1 + us * G4 + ut * G5);
         ws = h1 * (ur * G4 + us * G2 + ut * G6);
         wt = h1 * (ur * G5 + us * G6 + ut * G3);
@@ -266,8 +270,8 @@ __global__ void __launch_bounds__(NP * NP * epb<NP>())
       if (valid) {
         const int64_t l = base + k * NP2;
         float a = ar + as + r_au[k];
-        if constexpr (H2) a = a + (h2 * bm1[l]) * r_u[k];
-        w[l] = a;
+        if constexpr (H2) a = a + (h2 * ldnt32(bm1 + l)) * r_u[k];
+        __builtin_nontemporal_store(a, w + l);
         den += (double)a * (double)r_u[k];
       }
     }

@@ -200,8 +200,9 @@ int gs_op(const GsMap &gs, double *u, int op, hipStream_t s);
 // CG assembly: w <- mask * dssum(w) using the flags of gs_prepare; skipped
 // when st is given and st->done.
 // flag: group flags to use instead of gs.flag (a solve with its own mask)
+// nblocks: grid (a multiple of 8; 0 -> kGsBlocks)
 int gs_assemble(const GsMap &gs, double *w, const CgState *st, hipStream_t s,
-                const uint8_t *flag = nullptr);
+                const uint8_t *flag = nullptr, int nblocks = 0);
 int gs_assemble(const GsMap &gs, float *w, const CgState *st, hipStream_t s,
                 const uint8_t *flag = nullptr);
 // Per id group (single members included): S = sum of e; z_i <- mean over
