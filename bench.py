@@ -112,14 +112,36 @@ def cpu_baseline(nel1d, lx1, deform, target_seconds, solver="hsmg"):
 
 def order_sweep():
     """Polynomial-order sweep (jittered boxes, E^(1/3) ~ 368/N so every order
-    carries about 4e7 unique DOF; h1 = h2 = 1; f ~ U(-1,1)): per order one
-    timed FP64 axhelm apply (5 launches averaged) and 100 fixed Jacobi-PCG
-    iterations of hmholtz.  Prints one JSON line with the table."""
+    carries about 5e7 unique DOF; h1 = h2 = 1; f ~ U(-1,1)): per order the
+    autotuner's choice (made on the first apply), the CG-fused axhelm
+    kernel's time per kernel variant (HIP events of the library's timers
+    over 20 Jacobi-PCG iterations each), and 100 fixed Jacobi-PCG iterations
+    of hmholtz with the FP64 operator (its axhelm kernel time and HBM
+    fraction at 12 words per node) and with the FP32 one (mixed precision).
+    Prints one JSON line with the table."""
+    import ctypes
+
     import torch
-    from nek5000_amd import solver
+    from nek5000_amd import _lib, solver
     from nek5000_amd.mesh import BoxMesh
 
     solver.init(0)
+    lib = _lib.load()
+
+    def timed_pcg(F, u, rhs, one, iters, params=None):
+        """wall seconds, and the axhelm kernel's mean ms from the timers"""
+        lib.nkh_timers_reset()
+        lib.nkh_timers_enable(1)
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        out = F.hmholtz("TEMP", u, rhs.clone(), one, one, F.mask, F.mult, -1e-30, iters,
+                        params=params)
+        torch.cuda.synchronize()
+        wall = time.perf_counter() - t
+        lib.nkh_timers_enable(0)
+        sec, calls = ctypes.c_double(), ctypes.c_int64()
+        lib.nkh_timers_read(b"axhm", ctypes.byref(sec), ctypes.byref(calls))
+        return wall, sec.value / max(calls.value, 1) * 1e3, out
     rows = []
     for N in range(3, 16):
         e1 = {3: 110, 15: 24}.get(N, int(round(368.0 / N)))
@@ -138,35 +160,30 @@ def order_sweep():
         torch.cuda.synchronize()
         chosen = solver.axhelm_tuning(N + 1)
 
-        def time_apply():
-            F.axhelm(au, rhs, one, one)
-            torch.cuda.synchronize()
-            t = time.perf_counter()
-            for _ in range(5):
-                F.axhelm(au, rhs, one, one)
-            torch.cuda.synchronize()
-            return (time.perf_counter() - t) / 5 * 1e3
-
+        u = torch.zeros_like(rhs)
         per_variant = {}
         for v in (0, 1):
             solver.set_axhelm_tuning(N + 1, v, 0)
-            per_variant[f"variant{v}_ms"] = time_apply()
+            timed_pcg(F, u, rhs, one, 5)
+            per_variant[f"variant{v}_cg_ms"] = timed_pcg(F, u, rhs, one, 20)[1]
         solver.set_axhelm_tuning(N + 1, chosen[0], chosen[1])
-        ax_ms = time_apply()
-        # u, au, g1..g6, bm1 (h2 uniform nonzero): 9 words per local node
-        ax_gbs = 9 * 8 * box.n / (ax_ms * 1e-3) / 1e9
-        u = torch.zeros_like(rhs)
-        F.hmholtz("TEMP", u, rhs.clone(), one, one, F.mask, F.mult, -1e-30, 100)
-        torch.cuda.synchronize()
-        t = time.perf_counter()
-        out = F.hmholtz("TEMP", u, rhs.clone(), one, one, F.mask, F.mult, -1e-30, 100)
-        torch.cuda.synchronize()
-        pcg_s = time.perf_counter() - t
+        timed_pcg(F, u, rhs, one, 100)
+        pcg_s, ax_ms, out = timed_pcg(F, u, rhs, one, 100)
+        # CG-fused axhelm (h2 uniform nonzero): r, d, p, g1..g6, bm1 in;
+        # p, w out -- 12 words per local node
+        ax_gbs = fused_axhelm_bytes_per_node(N + 1) * box.n / (ax_ms * 1e-3) / 1e9
+        # the same 100 iterations with the FP32 operator (mixed Jacobi PCG)
+        p32 = solver.default_params(precision=2)
+        timed_pcg(F, u, rhs, one, 100, p32)
+        pcg32_s, _, out32 = timed_pcg(F, u, rhs, one, 100, p32)
         row = dict(N=N, E=box.nel, local_nodes=box.n, free_dof=free, setup_s=round(setup, 1),
                    axhelm_variant=chosen[0], axhelm_grid=chosen[1], **per_variant,
-                   axhelm_ms=ax_ms, axhelm_GBs=ax_gbs, axhelm_hbm_frac=ax_gbs / HBM_PEAK_GBS,
+                   axhelm_cg_ms=ax_ms, axhelm_cg_GBs=ax_gbs,
+                   axhelm_cg_hbm_frac=ax_gbs / HBM_PEAK_GBS,
                    pcg_iterations=out["niter"], pcg_ms=pcg_s * 1e3,
-                   pcg_GDOF_iter_s=free * out["niter"] / pcg_s / 1e9)
+                   pcg_GDOF_iter_s=free * out["niter"] / pcg_s / 1e9,
+                   pcg32_ms=pcg32_s * 1e3,
+                   pcg32_GDOF_iter_s=free * out32["niter"] / pcg32_s / 1e9)
         log(f"[sweep] {row}")
         rows.append(row)
         F.destroy()

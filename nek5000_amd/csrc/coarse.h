@@ -74,8 +74,9 @@ struct CoarseSolver {
   // A: CSR of the assembled problem (rows sorted, diagonal present).
   int setup(const std::vector<int32_t> &rowptr, const std::vector<int32_t> &col,
             const std::vector<double> &val, hipStream_t s);
-  // x = A^-1 b to the relative residual rtol; host waits on stream s.
-  int solve(double *xout, const double *b, hipStream_t s);
+  // x = A^-1 b to the relative residual rtol (or rtol_use when > 0); host
+  // waits on stream s.
+  int solve(double *xout, const double *b, hipStream_t s, double rtol_use = 0.0);
   // x = M b, one AMG V-cycle (no host wait)
   int vcycle_only(double *xout, const double *b, hipStream_t s);
   void vcycle(const double *rhs, double *out, double *part_out, hipStream_t s,

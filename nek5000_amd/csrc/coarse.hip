@@ -749,7 +749,7 @@ int CoarseSolver::vcycle_only(double *xout, const double *b, hipStream_t s) {
   return NKH_OK;
 }
 
-int CoarseSolver::solve(double *xout, const double *b, hipStream_t s) {
+int CoarseSolver::solve(double *xout, const double *b, hipStream_t s, double rtol_use) {
   if (n == 0) return NKH_OK;
   const dim3 B(256);
   if (nlevels_amg == 0) {  // small problem: the dense inverse is the solve
@@ -765,7 +765,8 @@ int CoarseSolver::solve(double *xout, const double *b, hipStream_t s) {
   const EllDev &A0 = lev[0].A;
   // residual of iteration k in rb[k & 1]
   hipLaunchKernelGGL(k_pcg_init, dim3(nb0), B, 0, s, x, rb[0], p[0], b, n, p_rr);
-  hipLaunchKernelGGL(k_pcg_start, dim3(1), B, 0, s, st, p_rr, nb0, rtol, maxit);
+  hipLaunchKernelGGL(k_pcg_start, dim3(1), B, 0, s, st, p_rr, nb0,
+                     rtol_use > 0.0 ? rtol_use : rtol, maxit);
   vcycle(rb[0], z, p_rz0, s);
   NK_HIP(hipGetLastError());
   CrsPcg hs{};

@@ -443,7 +443,10 @@ static int run_tnsr(VT *v, int nv, const UT *u, int nu, const double *w, const d
 // Coarse solve (hsmg.f:1321-1354 -> crs_solve, fcrs.c:77-85): assemble by
 // unknown, solve, scatter back (Dirichlet vertices 0).
 // crs_mode 1 applies one AMG V-cycle instead (the reference's param(40) = 1
-// coarse solver is likewise one pass of its AMG hierarchy).
+// coarse solver is likewise one pass of its AMG hierarchy); crs_mode 2 solves
+// to kInnerCoarseRtol (the V-cycles of the mixed solver's inner solves, which
+// themselves stop at 1e-4 of the outer residual).
+constexpr double kInnerCoarseRtol = 1e-6;
 static int coarse_solve(Mg &mg, double *e1, const double *r1, int crs_mode, hipStream_t s) {
   const int n = mg.ncrs;
   if (n > 0) {
@@ -451,6 +454,7 @@ static int coarse_solve(Mg &mg, double *e1, const double *r1, int crs_mode, hipS
                        mg.gidx, n);
     NK_HIP(hipGetLastError());
     if (crs_mode == 1) NK_TRY(mg.crs.vcycle_only(mg.cx, mg.cb, s));
+    else if (crs_mode == 2) NK_TRY(mg.crs.solve(mg.cx, mg.cb, s, kInnerCoarseRtol));
     else NK_TRY(mg.crs.solve(mg.cx, mg.cb, s));
   }
   hipLaunchKernelGGL(k_crs_scatter, dim3(mg_grid(8 * mg.nel)), dim3(256), 0, s, e1, mg.cx,
