@@ -245,7 +245,7 @@ int mg_setup(Mesh &m, const int *fbc, const int64_t *vertex, const double *x, co
 void mg_free(Mesh &m);
 // z = M rhs for rhs already masked with the fine multigrid mask
 // crs_mode: 0 coarse problem solved to a 1e-13 relative residual, 1 one AMG
-// V-cycle, 2 solved to 1e-6 (inner solves of the mixed solver)
+// V-cycle, 2 solved to 1e-4 (inner solves of the mixed solver)
 // precision: 0 FP64; 1 the fine-level smoother (local FDM solves and overlap
 // sum) in FP32, everything else FP64
 int mg_solve(Mesh &m, double *z, const double *rhs, hipStream_t s, int crs_mode = 0,
