@@ -159,6 +159,7 @@ def order_sweep():
         F.axhelm(au, rhs, one, one)  # the first apply of the order tunes it
         torch.cuda.synchronize()
         chosen = solver.axhelm_tuning(N + 1)
+        chosen_fused = solver.axhelm_tuning_fused(N + 1)
 
         u = torch.zeros_like(rhs)
         per_variant = {}
@@ -166,7 +167,7 @@ def order_sweep():
             solver.set_axhelm_tuning(N + 1, v, 0)
             timed_pcg(F, u, rhs, one, 5)
             per_variant[f"variant{v}_cg_ms"] = timed_pcg(F, u, rhs, one, 20)[1]
-        solver.set_axhelm_tuning(N + 1, chosen[0], chosen[1])
+        solver.set_axhelm_tuning(N + 1, chosen_fused[0], chosen[1])
         timed_pcg(F, u, rhs, one, 100)
         pcg_s, ax_ms, out = timed_pcg(F, u, rhs, one, 100)
         # CG-fused axhelm (h2 uniform nonzero): r, d, p, g1..g6, bm1 in;
@@ -177,7 +178,8 @@ def order_sweep():
         timed_pcg(F, u, rhs, one, 100, p32)
         pcg32_s, _, out32 = timed_pcg(F, u, rhs, one, 100, p32)
         row = dict(N=N, E=box.nel, local_nodes=box.n, free_dof=free, setup_s=round(setup, 1),
-                   axhelm_variant=chosen[0], axhelm_grid=chosen[1], **per_variant,
+                   axhelm_variant=chosen[0], axhelm_grid=chosen[1],
+                   axhelm_cg_variant=chosen_fused[0], **per_variant,
                    axhelm_cg_ms=ax_ms, axhelm_cg_GBs=ax_gbs,
                    axhelm_cg_hbm_frac=ax_gbs / HBM_PEAK_GBS,
                    pcg_iterations=out["niter"], pcg_ms=pcg_s * 1e3,
@@ -492,9 +494,10 @@ def main():
     t_ax = sec.value / max(calls.value, 1)
     lib.nkh_timers_read(b"dsum", ctypes.byref(sec), ctypes.byref(calls))
     t_gs = sec.value / max(calls.value, 1)
-    # the kernel the autotuner chose for this order in the warm-up solve
-    variant, _, _ = solver.axhelm_tuning(args.lx1)
-    kname = "k_axhelm_pl" if variant == 1 else "k_axhelm"
+    # the kernel the autotuner chose for this order's fused apply in the
+    # warm-up solve
+    v_cg, v_flex = solver.axhelm_tuning_fused(args.lx1)
+    kname = "k_axhelm_pl" if (v_flex if hsmg else v_cg) == 1 else "k_axhelm"
     if hsmg:
         # axhelm of ax (gmres.f:284) with the p update fused: reads p, z,
         # g1..g6, bm1; writes p, w
@@ -543,6 +546,15 @@ def main():
             measure("mixed: FP32 inner flexible CG (operator, fine-level V-cycle, vectors) in "
                     "FP64 defect correction, FP64 coarse", params_for(args.crs, 2))
             measure("mixed FP32 inner solve + one-V-cycle coarse", params_for(1, 2))
+            # the matrix-core trial of the FP32 local solves (bf16 operands,
+            # FP32 accumulation); the default stays the vector kernel
+            solver.set_fdm_variant(2)
+            measure("mixed FP32 inner solve, FDM contractions in bf16 on the matrix cores",
+                    params_for(args.crs, 2))
+            solver.set_fdm_variant(1)
+            measure("mixed FP32 inner solve, FDM contractions in FP32 on the matrix cores",
+                    params_for(args.crs, 2))
+            solver.set_fdm_variant(0)
         else:
             measure("mixed: FP32 Jacobi-PCG iteration (operator, vectors), FP64 dots and "
                     "residual refreshes", params_for(0, 2))

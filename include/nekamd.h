@@ -162,7 +162,18 @@ int nkd_axhelm(int mesh, double *au, const double *u, const double *h1, const do
    a mesh of >= 2^20 nodes times both variants at two grids and keeps the
    fastest (*tuned = 1 afterwards). */
 int nkh_axhelm_tuning(int lx1, int *variant, int *grid, int *tuned);
+/* sets the plain, CG-fused and flexible-CG-fused applies to variant */
 int nkh_axhelm_set_tuning(int lx1, int variant, int grid);
+/* the kernels of the CG-fused (cggo) and flexible-CG-fused applies, which
+   the autotuner times separately on scratch vectors */
+int nkh_axhelm_tuning_fused(int lx1, int *variant_cg, int *variant_flex);
+/* Kernel of the single-precision local FDM solves (precision 1 and 2; no
+   counterpart in the reference, whose hsmg_do_fast has one form), for Schwarz
+   blocks of up to 16 points: 0 vector FMA (default), 1 the six contractions
+   as FP32 matrix-core products (v_mfma_f32_16x16x4_f32), 2 as bf16
+   matrix-core products with FP32 accumulation.  Process-wide. */
+int nkh_fdm_set_variant(int variant);
+int nkh_fdm_variant(int *variant);
 /* dssum.f:33-98 DSSUM: u_i <- sum over nodes sharing glo_num(i). */
 int nkd_dssum(int mesh, double *u);
 /* dssum.f:100-161 DSOP: op '+', '*', 'm' (min), 'M' (max). */

@@ -14,31 +14,66 @@ static int axhelm_apply(const Mesh &m, double *au, const double *u, Coef h1, Coe
 // Autotuner: on the first apply of an order on a mesh of at least 2^20
 // local nodes (and not inside a graph capture), time both kernels with two
 // grid sizes on the call's own data -- the plain apply is element-local, so
-// every candidate writes the same au -- and keep the fastest for that order
-// (also for the CG-fused kernels, whose grid stays fixed).
+// every candidate writes the same au -- and keep the fastest for that order;
+// then time both kernels of the CG-fused and flexible-CG-fused applies
+// (their grid is fixed) on scratch vectors and keep the faster of each.
 static int tune_axhelm(const Mesh &m, double *au, const double *u, Coef h1, Coef h2,
                        hipStream_t s) {
   AxhelmChoice &ch = g_axhelm_choice[m.np];
-  const AxhelmChoice cands[4] = {{0, 16384, false}, {0, 0, false}, {1, 16384, false},
-                                 {1, 0, false}};
   hipEvent_t e0, e1;
   NK_HIP(hipEventCreate(&e0));
   NK_HIP(hipEventCreate(&e1));
-  float best = 1e30f;
-  AxhelmChoice pick = ch;
-  for (const AxhelmChoice &c : cands) {
-    ch = c;
-    NK_TRY(axhelm_apply(m, au, u, h1, h2, s));  // warm-up
+  auto timed = [&](auto &&launch, float *ms) -> int {
+    NK_TRY(launch());  // warm-up
     NK_HIP(hipEventRecord(e0, s));
-    NK_TRY(axhelm_apply(m, au, u, h1, h2, s));
+    NK_TRY(launch());
     NK_HIP(hipEventRecord(e1, s));
     NK_HIP(hipEventSynchronize(e1));
+    NK_HIP(hipEventElapsedTime(ms, e0, e1));
+    return NKH_OK;
+  };
+  const int cand_var[4] = {0, 0, 1, 1}, cand_grid[4] = {16384, 0, 16384, 0};
+  float best = 1e30f;
+  AxhelmChoice pick = ch;
+  for (int c = 0; c < 4; ++c) {
+    ch.variant = cand_var[c];
+    ch.grid = cand_grid[c];
     float ms = 0.f;
-    NK_HIP(hipEventElapsedTime(&ms, e0, e1));
+    NK_TRY(timed([&] { return axhelm_apply(m, au, u, h1, h2, s); }, &ms));
     if (ms < best) {
       best = ms;
-      pick = c;
+      pick.variant = ch.variant;
+      pick.grid = ch.grid;
     }
+  }
+  ch = pick;
+  // the fused applies on scratch: p, r, d, w and a zeroed state (beta 0)
+  const int64_t n = m.n;
+  const size_t part = (size_t)kAxBlocks;
+  double *buf = nullptr;
+  const size_t bytes = sizeof(double) * (4 * (size_t)n + part) + sizeof(CgState);
+  if (hipMalloc(&buf, bytes) == hipSuccess) {
+    double *p = buf, *r = buf + n, *d = buf + 2 * n, *w = buf + 3 * n, *rho = buf + 4 * n;
+    CgState *st = reinterpret_cast<CgState *>(buf + 4 * n + part);
+    NK_HIP(hipMemsetAsync(buf, 0, bytes, s));
+    float ms[2] = {0.f, 0.f};
+    for (int v = 0; v < 2; ++v) {
+      ch.variant_cg = v;
+      NK_TRY(timed([&] { return launch_axhelm_cg(m, w, p, r, d, h1, h2, r, st, rho, s); },
+                   &ms[v]));
+    }
+    pick.variant_cg = ms[1] < ms[0] ? 1 : 0;
+    for (int v = 0; v < 2; ++v) {
+      ch.variant_flex = v;
+      NK_TRY(timed([&] { return launch_axhelm_flex(m, w, p, r, h1, h2, &st->beta, rho, s); },
+                   &ms[v]));
+    }
+    pick.variant_flex = ms[1] < ms[0] ? 1 : 0;
+    NK_HIP(hipStreamSynchronize(s));
+    (void)hipFree(buf);
+  } else {
+    (void)hipGetLastError();  // no room for scratch: the fused applies follow the plain one
+    pick.variant_cg = pick.variant_flex = pick.variant;
   }
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
@@ -76,7 +111,15 @@ int axhelm_tuning(int np, int *variant, int *grid, int *tuned) {
 int axhelm_set_tuning(int np, int variant, int grid) {
   NK_CHECK(np >= 2 && np <= kMaxNP && (variant == 0 || variant == 1) && grid >= 0,
            NKH_ERR_ARG, "axhelm tuning: lx1=%d variant=%d grid=%d", np, variant, grid);
-  g_axhelm_choice[np] = AxhelmChoice{variant, grid, true};
+  g_axhelm_choice[np] = AxhelmChoice{variant, grid, variant, variant, true};
+  return NKH_OK;
+}
+
+int axhelm_tuning_fused(int np, int *variant_cg, int *variant_flex) {
+  NK_CHECK(np >= 2 && np <= kMaxNP, NKH_ERR_ARG, "axhelm tuning: lx1=%d", np);
+  const AxhelmChoice &c = g_axhelm_choice[np];
+  if (variant_cg) *variant_cg = c.variant_cg;
+  if (variant_flex) *variant_flex = c.variant_flex;
   return NKH_OK;
 }
 

@@ -732,6 +732,18 @@ int nkh_axhelm_set_tuning(int lx1, int variant, int grid) {
   return axhelm_set_tuning(lx1, variant, grid);
 }
 
+int nkh_axhelm_tuning_fused(int lx1, int *variant_cg, int *variant_flex) {
+  return axhelm_tuning_fused(lx1, variant_cg, variant_flex);
+}
+
+int nkh_fdm_set_variant(int variant) { return fdm_set_variant(variant); }
+
+int nkh_fdm_variant(int *variant) {
+  NK_CHECK(variant, NKH_ERR_ARG, "nkh_fdm_variant: null pointer");
+  *variant = fdm_variant();
+  return NKH_OK;
+}
+
 int nkd_dssum(int id, double *u) {
   Mesh *m = get_mesh(id);
   NK_CHECK(g_init && m && m->has_gs, NKH_ERR_STATE, "nkd_dssum: mesh %d not ready", id);

@@ -314,12 +314,15 @@ __global__ void __launch_bounds__(NP * NP * epb<NP>())
   }
 }
 
-// Per polynomial order: which of the two kernels and (for the plain apply)
-// how many blocks, chosen by the autotuner of launch_axhelm on the first
-// large apply of that order.  variant 0: element staged in LDS, 1: planes.
+// Per polynomial order: which of the two kernels (0: element staged in LDS,
+// 1: t-planes) for the plain apply (and its gated form), the CG-fused and
+// the flexible-CG-fused apply, and how many blocks for the plain apply;
+// chosen by the autotuner on the first large apply of that order.
 struct AxhelmChoice {
-  int variant = 0;
-  int grid = 0;  // plain apply: blocks (0 -> one per element group, up to 2^20)
+  int variant = 0;       // plain apply (MODE 0, MODE 3)
+  int grid = 0;          // plain apply: blocks (0 -> one per element group, up to 2^20)
+  int variant_cg = 0;    // MODE 1
+  int variant_flex = 0;  // MODE 2
   bool tuned = false;
 };
 inline AxhelmChoice g_axhelm_choice[kMaxNP + 1];
@@ -350,8 +353,9 @@ static int axhelm_dispatch(const Mesh &m, double *au, const double *u, Coef h1, 
   // h2 mode: 0 none (reference ifh2 false), 1 uniform, 2 field
   const int h2m = h2.uniform() ? (h2.s != 0.0 ? 1 : 0) : 2;
   const bool h1f = !h1.uniform();
+  const int var = MODE == 1 ? ch.variant_cg : (MODE == 2 ? ch.variant_flex : ch.variant);
 #define NK_LAUNCH(H1F_, H2M_)                                                              \
-  if (ch.variant == 1)                                                                      \
+  if (var == 1)                                                                             \
     hipLaunchKernelGGL((k_axhelm_pl<NP, MODE, H1F_, H2M_>), dim3(grid), block, 0, s, Dm,    \
                        m.Dd, au, u, m.g, m.n, h1.f, h1.s, h2.f, h2.s, m.bm1, m.nel, pout, r, \
                        dinv, mult, st, rho_part, betap);                                    \

@@ -163,7 +163,8 @@ int axhelm_tuning(int np, int *variant, int *grid, int *tuned);
 // Tune the order of m now if that is due (au is overwritten, u only read).
 int axhelm_autotune(const Mesh &m, double *au, const double *u, Coef h1, Coef h2,
                     hipStream_t s);
-int axhelm_set_tuning(int np, int variant, int grid);
+int axhelm_set_tuning(int np, int variant, int grid);  // all three applies
+int axhelm_tuning_fused(int np, int *variant_cg, int *variant_flex);
 // MODE 0 apply that returns while gate->done (replayed in captured loops).
 int launch_axhelm_gated(const Mesh &m, double *au, const double *u, Coef h1, Coef h2,
                         const CgState *gate, hipStream_t s);
@@ -253,6 +254,11 @@ int mg_solve(Mesh &m, double *z, const double *rhs, hipStream_t s, int crs_mode 
 // the same V-cycle with the fine level in single precision throughout
 // (input r and output z float; levels 2.. and the coarse solve FP64)
 int mg_solve32(Mesh &m, float *z, const float *r, hipStream_t s, int crs_mode);
+// Kernel of the single-precision local FDM solves (precision 1 and 2), for
+// blocks of up to 16 points: 0 vector FMA (default), 1 FP32 matrix-core
+// products, 2 bf16 matrix-core products with FP32 accumulation.
+int fdm_set_variant(int v);
+int fdm_variant();
 int mg_mask_fine(Mesh &m, double *r, hipStream_t s);
 int mg_info(const Mesh &m, int *lmax, int *nh, int *ncrs, double *crs_avg_iters);
 int run_flex_cg(Mesh &m, double *res, Coef h1, Coef h2, const double *wt, const double *mask,

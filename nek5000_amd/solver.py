@@ -77,10 +77,31 @@ def axhelm_tuning(lx1):
     return v.value, g.value, bool(t.value)
 
 
+def axhelm_tuning_fused(lx1):
+    """(CG-fused variant, flexible-CG-fused variant) for order lx1."""
+    a, b = ctypes.c_int(), ctypes.c_int()
+    check(init().nkh_axhelm_tuning_fused(int(lx1), ctypes.byref(a), ctypes.byref(b)),
+          "nkh_axhelm_tuning_fused")
+    return a.value, b.value
+
+
 def set_axhelm_tuning(lx1, variant, grid=0):
-    """Fix the axhelm kernel choice for order lx1 (no autotuning afterwards)."""
+    """Fix the axhelm kernel choice for order lx1 -- the plain, CG-fused and
+    flexible-CG-fused applies alike -- (no autotuning afterwards)."""
     check(init().nkh_axhelm_set_tuning(int(lx1), int(variant), int(grid)),
           "nkh_axhelm_set_tuning")
+
+
+def fdm_variant():
+    """Kernel of the single-precision local FDM solves: 0 vector FMA, 1 FP32
+    matrix cores, 2 bf16 matrix cores (FP32 accumulation)."""
+    v = ctypes.c_int()
+    check(init().nkh_fdm_variant(ctypes.byref(v)), "nkh_fdm_variant")
+    return v.value
+
+
+def set_fdm_variant(variant):
+    check(init().nkh_fdm_set_variant(int(variant)), "nkh_fdm_set_variant")
 
 
 def default_params(istep=1, param18=0, param22=1e-7, restol=1e-7, precision=0, param42=0,

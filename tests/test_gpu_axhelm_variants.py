@@ -126,8 +126,9 @@ def test_autotuner_picks_and_keeps_results(gpu):
     au = F.zeros()
     F.axhelm(au, u, h1, h2)
     variant, grid, tuned = gpu.axhelm_tuning(lx1)
-    say(f"[autotune] chosen variant={variant} grid={grid} tuned={tuned}")
-    assert tuned and variant in (0, 1) and grid >= 0
+    v_cg, v_flex = gpu.axhelm_tuning_fused(lx1)
+    say(f"[autotune] chosen variant={variant} grid={grid} tuned={tuned}, fused {v_cg} {v_flex}")
+    assert tuned and variant in (0, 1) and grid >= 0 and v_cg in (0, 1) and v_flex in (0, 1)
     a_tuned = host(au)
     for v in (0, 1):
         gpu.set_axhelm_tuning(lx1, v, 0)
@@ -135,6 +136,7 @@ def test_autotuner_picks_and_keeps_results(gpu):
         F.axhelm(au2, u, h1, h2)
         assert np.array_equal(host(au2), a_tuned)
     gpu.set_axhelm_tuning(lx1, variant, grid)
+    assert gpu.axhelm_tuning_fused(lx1) == (variant, variant)
     F.destroy()
 
 

@@ -339,3 +339,42 @@ def test_flex_cg_mixed_needs_uniform_coefficients(gpu):
         F.hmholtz("PRES", u, F.to_dev(P.bm1), F.zeros() + 1.0, F.zeros(), F.mask, F.mult, -1e-8,
                   50, params=gpu.default_params(param42=0, param21=-1e-8, precision=2))
     F.destroy()
+
+
+@pytest.mark.parametrize("case", [MG_CASES[0], MG_CASES[1]])
+def test_fp32_smoother_matrix_core_variants(gpu, case):
+    """The single-precision local solves with their contractions on the
+    matrix cores (FP32 operands, or bf16 operands with FP32 accumulation):
+    the FP32-smoother and mixed solves still meet the FP64 tolerance in the
+    true residual, within a few iterations of the oracle's FP64 count."""
+    P, mg, F = build(gpu, case)
+    h1 = np.ones(P.shape)
+    h2 = np.full(P.shape, 10.0)
+    b = P.bm1 * P.uniform_field(71)
+    rhs = P.field.dssum(b) * P.mask
+    tol = 1e-10
+    uo, ito, _, div0 = mg.flex_cg(rhs, h1, h2, 300, -tol)
+    assert gpu.fdm_variant() == 0
+    try:
+        for variant in (1, 2):
+            gpu.set_fdm_variant(variant)
+            for precision in (1, 2):
+                u = F.zeros()
+                out = F.hmholtz("PRES", u, F.to_dev(b), F.to_dev(h1), F.to_dev(h2), F.mask,
+                                F.mult, -tol, 300,
+                                params=gpu.default_params(param42=2, param21=-tol,
+                                                          precision=precision))
+                ug = host(u)
+                r = rhs - P.field.dssum(P.field.axhelm(ug, h1, h2)) * P.mask
+                rn = float(np.sqrt(np.sum(r * r * P.mult) / P.field.vol))
+                say(f"[fdm variant {variant}] {case} precision={precision}: {out['niter']} "
+                    f"iterations (oracle {ito}), true residual {rn / div0:.2e}")
+                assert out["converged"]
+                assert out["niter"] <= ito + 8
+                assert rn < tol * div0 * 1.0001
+                assert l2rel(ug, uo) < 1e-7
+    finally:
+        gpu.set_fdm_variant(0)
+    with pytest.raises(gpu.NekAmdError):
+        gpu.set_fdm_variant(3)
+    F.destroy()
