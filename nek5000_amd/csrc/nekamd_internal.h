@@ -16,7 +16,7 @@ constexpr int kMaxNP = 16;
 // every reduction sums the same partials in the same order (deterministic).
 constexpr int kRedBlocks = 2048;
 // Partial-sum slots of the CG-fused axhelm (one per block of its fixed grid).
-constexpr int kAxBlocks = 16384;
+constexpr int kAxBlocks = 65536;
 // Blocks of the CG assembly kernel (a multiple of the 8 XCDs).
 constexpr int kGsBlocks = 8192;
 constexpr int kMaxCgIter = 900;  // hmholtz.f:636 maxcg
