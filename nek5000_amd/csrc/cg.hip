@@ -68,6 +68,14 @@ __device__ __forceinline__ void block_store2(double a, double b, double *pa, dou
   }
 }
 
+// Grid of the update-and-dots kernels: about 2048 nodes per block, at most
+// kUpdBlocks -- a launch costs about 10 ns per block, which dominates small
+// fields at a fixed large grid.  One partial sum per block.
+inline int upd_blocks(int64_t n) {
+  int64_t b = (n + 2047) / 2048;
+  return (int)(b < 64 ? 64 : (b > kUpdBlocks ? kUpdBlocks : b));
+}
+
 // ---- generic reductions (setup) ----------------------------------------
 // kind 0: sum a*b*c (c may be null -> 1, b may be null -> 1)
 // kind 1: max |a|   kind 2: max a   kind 3: min a
@@ -239,10 +247,10 @@ __global__ void __launch_bounds__(256)
 
 __global__ void __launch_bounds__(kScalThreads)
     k_cg_scalars1(CgState *__restrict__ st, const double *__restrict__ part1,
-                  const double *__restrict__ part2, double *__restrict__ hist) {
+                  const double *__restrict__ part2, int nparts, double *__restrict__ hist) {
   if (st->done) return;
-  const double s1 = block_sum_array(part1, kUpdBlocks);
-  const double s2 = block_sum_array(part2, kUpdBlocks);
+  const double s1 = block_sum_array(part1, nparts);
+  const double s2 = block_sum_array(part2, nparts);
   if (threadIdx.x != 0) return;
   st->pending = 0;
   const int iter = st->iter + 1;
@@ -360,7 +368,14 @@ struct McgState {
 constexpr double kRefresh = 0.1;
 // grid of the refresh kernels: they run every pass of a captured loop and
 // mostly return at once, at a cost that grows with the number of blocks
+// (about 10 ns each), so it scales with the field: 64Ki nodes per block,
+// 64 to 2048 blocks (a multiple of 8, as the assembly's XCD mapping needs)
 constexpr int kRefreshBlocks = 2048;
+inline int refresh_blocks(int64_t n) {
+  int64_t b = (n + 65535) / 65536;
+  b = (b + 7) / 8 * 8;
+  return (int)(b < 64 ? 64 : (b > kRefreshBlocks ? kRefreshBlocks : b));
+}
 
 // xs += alpha p ; r -= alpha w (FP32) ; FP64 partials of dm*r*r and bm*r*r
 template <bool VEC, bool UPD>
@@ -427,11 +442,11 @@ __global__ void __launch_bounds__(256)
 // ending the solve or when the iterated residual fell by kRefresh
 __global__ void __launch_bounds__(kScalThreads)
     k_mcg_scalars1(McgState *__restrict__ ms, const double *__restrict__ part1,
-                   const double *__restrict__ part2, double *__restrict__ hist) {
+                   const double *__restrict__ part2, int nparts, double *__restrict__ hist) {
   CgState *st = &ms->cg;
   if (st->done) return;
-  const double s1 = block_sum_array(part1, kUpdBlocks);
-  const double s2 = block_sum_array(part2, kUpdBlocks);
+  const double s1 = block_sum_array(part1, nparts);
+  const double s2 = block_sum_array(part2, nparts);
   if (threadIdx.x != 0) return;
   const int iter = st->iter + 1;
   st->iter = iter;
@@ -503,10 +518,10 @@ __global__ void __launch_bounds__(256)
 // iteration continues from it with the same search direction
 __global__ void __launch_bounds__(kScalThreads)
     k_mcg_scalars1b(McgState *__restrict__ ms, const double *__restrict__ part1,
-                    const double *__restrict__ part2, double *__restrict__ hist) {
+                    const double *__restrict__ part2, int nparts, double *__restrict__ hist) {
   if (ms->gate.done) return;
-  const double s1 = block_sum_array(part1, kRefreshBlocks);
-  const double s2 = block_sum_array(part2, kRefreshBlocks);
+  const double s1 = block_sum_array(part1, nparts);
+  const double s2 = block_sum_array(part2, nparts);
   if (threadIdx.x != 0) return;
   CgState *st = &ms->cg;
   ms->gate.done = 1;
@@ -785,9 +800,10 @@ struct CgRun {
                                         : k_cg_update_dots<true, true>)
                        : (launched == 0 ? k_cg_update_dots<false, false>
                                         : k_cg_update_dots<false, true>);
-    hipLaunchKernelGGL(kern, dim3(kUpdBlocks), dim3(256), 0, s, x, r, p, w, dm, bm, n, st, part1,
-                       part2);
-    hipLaunchKernelGGL(k_cg_scalars1, dim3(1), dim3(kScalThreads), 0, s, st, part1, part2, hist);
+    const int ub = upd_blocks(n);
+    hipLaunchKernelGGL(kern, dim3(ub), dim3(256), 0, s, x, r, p, w, dm, bm, n, st, part1, part2);
+    hipLaunchKernelGGL(k_cg_scalars1, dim3(1), dim3(kScalThreads), 0, s, st, part1, part2, ub,
+                       hist);
     if (fused) {
       if (ev) NK_HIP(hipEventRecord(ev[0], s));
       NK_TRY(launch_axhelm_cg(*m, w, p, r, d, h1, h2, mult, st, rho1, s));
@@ -921,17 +937,19 @@ struct McgRun {
                                         : k_mcg_update_dots<true, true>)
                        : (launched == 0 ? k_mcg_update_dots<false, false>
                                         : k_mcg_update_dots<false, true>);
-    hipLaunchKernelGGL(kern, dim3(kUpdBlocks), dim3(256), 0, s, xs, r, p, w, dm, bm, n, st, part1,
-                       part2);
-    hipLaunchKernelGGL(k_mcg_scalars1, dim3(1), dim3(kScalThreads), 0, s, ms, part1, part2, hist);
+    const int ub = upd_blocks(n);
+    hipLaunchKernelGGL(kern, dim3(ub), dim3(256), 0, s, xs, r, p, w, dm, bm, n, st, part1, part2);
+    hipLaunchKernelGGL(k_mcg_scalars1, dim3(1), dim3(kScalThreads), 0, s, ms, part1, part2, ub,
+                       hist);
     // the refresh (its kernels return at once while the gate is closed)
     const CgState *gate = &ms->gate;
-    hipLaunchKernelGGL(k_mcg_fold, dim3(kRefreshBlocks), dim3(256), 0, s, x, xs, n, gate);
+    const int rb = refresh_blocks(n);
+    hipLaunchKernelGGL(k_mcg_fold, dim3(rb), dim3(256), 0, s, x, xs, n, gate);
     NK_TRY(launch_axhelm_gated(*m, ax, x, h1, h2, gate, s));
-    NK_TRY(gs_assemble(m->gs, ax, gate, s, flag, kRefreshBlocks));
-    hipLaunchKernelGGL(k_mcg_resid, dim3(kRefreshBlocks), dim3(256), 0, s, r, f, ax, dm, bm, n,
-                       gate, part1, part2);
-    hipLaunchKernelGGL(k_mcg_scalars1b, dim3(1), dim3(kScalThreads), 0, s, ms, part1, part2,
+    NK_TRY(gs_assemble(m->gs, ax, gate, s, flag, rb));
+    hipLaunchKernelGGL(k_mcg_resid, dim3(rb), dim3(256), 0, s, r, f, ax, dm, bm, n, gate, part1,
+                       part2);
+    hipLaunchKernelGGL(k_mcg_scalars1b, dim3(1), dim3(kScalThreads), 0, s, ms, part1, part2, rb,
                        hist);
     if (ev) NK_HIP(hipEventRecord(ev[0], s));
     NK_TRY(launch_axhelm32_jac(*m, w, p, r, d, (float)h1.s, (float)h2.s, st, rho1, s));

@@ -340,8 +340,10 @@ static int axhelm_dispatch(const Mesh &m, double *au, const double *u, Coef h1, 
   int grid;
   if (MODE == 3) {
     // gated: most launches find the gate closed, and an empty launch costs
-    // in proportion to its blocks
-    grid = (int)(ngroups < 4096 ? ngroups : 4096);
+    // in proportion to its blocks: 16 element groups per block, 64 to 4096
+    int64_t g = ngroups / 16;
+    g = g < 64 ? 64 : (g > 4096 ? 4096 : g);
+    grid = (int)(ngroups < g ? ngroups : g);
   } else if (MODE >= 1) {
     grid = axhelm_cg_grid(m);
   } else {

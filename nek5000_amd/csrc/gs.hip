@@ -268,11 +268,20 @@ int gs_prepare(GsMap &gs, const double *mask, const double *mult, double *red, b
   return NKH_OK;
 }
 
+// Blocks of the chunked group kernels: about 512 groups per block, 64 to
+// kGsBlocks, a multiple of 8 (their XCD mapping); small maps would otherwise
+// pay mostly for dispatching kGsBlocks blocks.  Results do not depend on it.
+static int gs_blocks(int64_t ngroups) {
+  int64_t b = (ngroups + 511) / 512;
+  b = (b + 7) / 8 * 8;
+  return (int)(b < 64 ? 64 : (b > kGsBlocks ? kGsBlocks : b));
+}
+
 template <typename T>
 static int gs_assemble_t(const GsMap &gs, T *w, const CgState *st, hipStream_t s,
                          const uint8_t *flag, int nblocks = 0) {
   if (gs.ngroups == 0) return NKH_OK;
-  const int nb = nblocks > 0 ? nblocks : kGsBlocks;
+  const int nb = nblocks > 0 ? nblocks : gs_blocks(gs.ngroups);
   const int64_t chunk = (gs.ngroups + nb - 1) / nb;
   hipLaunchKernelGGL(k_gs_assemble<T>, dim3(nb), dim3(256), 0, s, w, gs.off, gs.idx,
                      flag ? flag : gs.flag, gs.ngroups, chunk, st);
@@ -293,7 +302,7 @@ int gs_assemble(const GsMap &gs, float *w, const CgState *st, hipStream_t s,
 template <typename E, typename Z, typename W>
 static int gs_fin_t(const GsMap &gs, Z *z, const E *e, const W *w, hipStream_t s) {
   if (gs.ngroups == 0) return NKH_OK;
-  const int nb = kGsBlocks;
+  const int nb = gs_blocks(gs.ngroups);
   const int64_t chunk = (gs.ngroups + nb - 1) / nb;
   hipLaunchKernelGGL((k_gs_fin<E, Z, W>), dim3(nb), dim3(256), 0, s, z, e, w, gs.off, gs.idx,
                      gs.ngroups, chunk);
