@@ -287,11 +287,12 @@ typedef struct {
 } nkh_batch_solve;
 /* hmholtz.f:2-69 + the Jacobi branch of cggo (:611-846) for every solve of
    the batch (non-PRES names): the set-up of each solve in order, then all
-   solves iterate together -- the steady-state iteration of the whole batch
-   (five launches per solve, each solve on its own stream) is captured once
-   in a hipGraph and replayed until every solve has stopped.  Iterates and
-   results equal those of nkd_hmholtz on each solve (prm->precision 0 or 2).  res: nb results;
-   *graph_launches (optional): iterations run for the slowest solve. */
+   solves iterate together -- eight steady-state iterations of the whole
+   batch (each solve's launches on its own stream) are captured once in a
+   hipGraph and replayed until every solve has stopped.  Iterates and
+   results equal those of nkd_hmholtz on each solve (prm->precision 0 or
+   2).  res: nb results; *graph_launches (optional): iterations enqueued
+   for the slowest solve. */
 int nkd_hmholtz_batch(int batch, const nkh_batch_solve *solves, int maxit,
                       const nkh_params *prm, nkh_result *res, int *graph_launches);
 

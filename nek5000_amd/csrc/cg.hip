@@ -1064,12 +1064,17 @@ int run_cggo(Mesh &m, double *x, const double *f, Coef h1, Coef h2, const double
 // ---------------------------------------------------------------------------
 // Batched Jacobi-PCG: several hmholtz solves (different meshes, h1/h2,
 // masks, sizes; the same lx1 or not) iterate together.  Each solve keeps its
-// own work space, state and gs group flags; the steady-state iteration of
-// all of them -- five launches per solve, each solve on its own stream,
-// forked from and joined to the caller's stream -- is captured once in a
-// hipGraph and replayed until every solve has stopped (a finished solve's
-// kernels return at once).  No memset is recorded in the graph.
+// own work space, state and gs group flags; kGraphIters steady-state
+// iterations of all of them -- each solve's launches on its own stream,
+// forked from and joined to a library stream once per replay -- are
+// captured once in a hipGraph and replayed until every solve has stopped (a
+// finished solve's kernels return at once).  No memset is recorded in the
+// graph.
 // ---------------------------------------------------------------------------
+// iterations per replay of the batch graph (the host reads the states after
+// each replay)
+constexpr int kGraphIters = 8;
+
 struct CgBatch {
   int nb = 0;
   std::vector<int> mesh;
@@ -1231,9 +1236,13 @@ int batch_drive(CgBatch &B, const double *const *x, const double *const *f, cons
     launches = 1;
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
+    // the graph: kGraphIters iterations of every solve, each solve on its own
+    // stream (solves do not wait for each other between iterations)
     NK_HIP(hipStreamBeginCapture(c, hipStreamCaptureModeThreadLocal));
-    const int rc_cap =
-        fork_join([&](int k, hipStream_t q) { return runs[(size_t)k].enqueue(q); });
+    const int rc_cap = fork_join([&](int k, hipStream_t q) {
+      for (int it = 0; it < kGraphIters; ++it) NK_TRY(runs[(size_t)k].enqueue(q));
+      return NKH_OK;
+    });
     const hipError_t end = hipStreamEndCapture(c, &graph);
     NK_TRY(rc_cap);
     NK_HIP(end);
@@ -1241,15 +1250,15 @@ int batch_drive(CgBatch &B, const double *const *x, const double *const *f, cons
     int niter_max = 0;
     for (int k : active) niter_max = std::max(niter_max, runs[(size_t)k].niter);
     std::vector<CgState> hs((size_t)B.nb);
-    int check_every = 1, rc = NKH_OK;
+    int rc = NKH_OK;
     while (true) {
       if (hipGraphLaunch(exec, c) != hipSuccess) {
         set_error("batch: graph launch failed");
         rc = NKH_ERR_HIP;
         break;
       }
-      ++launches;
-      if (launches % check_every == 0 || launches > niter_max) {
+      launches += kGraphIters;
+      {
         for (int k : active)
           (void)hipMemcpyAsync(&hs[(size_t)k], B.st[(size_t)k], sizeof(CgState),
                                hipMemcpyDeviceToHost, c);
@@ -1261,12 +1270,11 @@ int batch_drive(CgBatch &B, const double *const *x, const double *const *f, cons
         bool all = true;
         for (int k : active) all = all && hs[(size_t)k].done;
         if (all) break;
-        if (launches > niter_max + 1) {
+        if (launches > niter_max + kGraphIters) {
           set_error("batch: state machines did not terminate");
           rc = NKH_ERR_STATE;
           break;
         }
-        if (launches >= 8) check_every = 4;
       }
     }
     (void)hipGraphExecDestroy(exec);
