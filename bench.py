@@ -394,16 +394,17 @@ def main():
                          "difference check at E=32^3, N=7")
     ap.add_argument("--sweep", action="store_true",
                     help="instead of the bench line: polynomial-order sweep N=3..15 of the "
-                         "FP64 axhelm and 100 Jacobi-PCG iterations")
+                         "CG-fused axhelm (both kernel variants) and 100 Jacobi-PCG "
+                         "iterations with the FP64 and the FP32 operator")
     ap.add_argument("--solver", choices=("hsmg", "jacobi"), default="hsmg",
                     help="hsmg: hmholtz('PRES') -> hmh_flex_cg with h1mg (param42=2); "
                          "jacobi: hmholtz -> cggo")
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
-        print("bench.py measures one GPU: this build has no multi-GPU path "
-              "(elements would be partitioned across GPUs with a halo exchange; "
-              "see DESIGN.md).", file=sys.stderr)
+        print("bench.py measures one GPU: there is no multi-GPU path (elements "
+              "would be partitioned across GPUs with a halo exchange of the "
+              "interface nodes).", file=sys.stderr)
         sys.exit(2)
     if args.sweep:
         order_sweep()
