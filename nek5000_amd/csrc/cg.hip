@@ -40,17 +40,7 @@ __device__ __forceinline__ double wave_sum(double v) {
 // order.  Result valid on thread 0.
 constexpr int kScalThreads = 1024;
 __device__ double block_sum_array(const double *__restrict__ part, int count) {
-  __shared__ double sb[kScalThreads / 64];
-  double v = 0.0;
-  for (int q = threadIdx.x; q < count; q += blockDim.x) v += part[q];
-  v = wave_sum(v);
-  if ((threadIdx.x & 63) == 0) sb[threadIdx.x >> 6] = v;
-  __syncthreads();
-  double t = 0.0;
-  if (threadIdx.x == 0)
-    for (int q = 0; q < (int)(blockDim.x >> 6); ++q) t += sb[q];
-  __syncthreads();
-  return t;
+  return block_sum_partials(part, count);
 }
 
 __device__ __forceinline__ void block_store2(double a, double b, double *pa, double *pb) {

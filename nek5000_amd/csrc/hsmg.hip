@@ -898,15 +898,10 @@ __global__ void __launch_bounds__(256)
 }
 
 // den from the count rho partials of the fused axhelm; alpha
-__global__ void __launch_bounds__(256) k_fcg_alpha_n(FcgState *st, const double *part, int count) {
-  __shared__ double sf[4];
-  double v = 0.0;
-  for (int q = threadIdx.x; q < count; q += blockDim.x) v += part[q];
-  for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
-  if ((threadIdx.x & 63) == 0) sf[threadIdx.x >> 6] = v;
-  __syncthreads();
+// one block of 1024 threads (the fused axhelm leaves up to 65536 partials)
+__global__ void __launch_bounds__(1024) k_fcg_alpha_n(FcgState *st, const double *part, int count) {
+  const double t = block_sum_partials(part, count);
   if (threadIdx.x == 0) {
-    const double t = ((sf[0] + sf[1]) + sf[2]) + sf[3];
     st->den = t;
     st->alpha = st->rho1 / t;
   }
@@ -1015,7 +1010,7 @@ int run_flex_cg(Mesh &m, double *res, Coef h1, Coef h2, const double *wt, const 
       if (tm.on) (void)hipEventRecord(ev[1], s);
       if ((rc = gs_assemble(m.gs, w, nullptr, s)) != NKH_OK) break;
       if (tm.on) (void)hipEventRecord(ev[2], s);
-      hipLaunchKernelGGL(k_fcg_alpha_n, dim3(1), b, 0, s, mg.fst, rho_part, nrho);
+      hipLaunchKernelGGL(k_fcg_alpha_n, dim3(1), dim3(1024), 0, s, mg.fst, rho_part, nrho);
     } else {
       hipLaunchKernelGGL(k_fcg_p, dim3(mg_grid(n)), b, 0, s, p, z, mg.fst, n);
       if (tm.on) (void)hipEventRecord(ev[0], s);

@@ -113,8 +113,9 @@ __global__ void __launch_bounds__(256) k_mx_beta(MxState *st, const double *pa, 
   }
 }
 
-__global__ void __launch_bounds__(256) k_mx_alpha(MxState *st, const double *part, int count) {
-  const double t = fsum(part, count);
+// one block of 1024 threads (the fused axhelm leaves up to 65536 partials)
+__global__ void __launch_bounds__(1024) k_mx_alpha(MxState *st, const double *part, int count) {
+  const double t = block_sum_partials(part, count);
   if (threadIdx.x == 0) {
     st->den = t;
     st->alpha = st->rho1 / t;
@@ -126,9 +127,29 @@ __global__ void __launch_bounds__(256)
     k_mx_update(float *__restrict__ x, float *__restrict__ r, const float *__restrict__ p,
                 const float *__restrict__ w, const float *__restrict__ wt, const MxState *st,
                 int64_t n, double *part) {
+  // 4 nodes per step with nontemporal hints (the vectors are separate
+  // 256-byte aligned allocations), the tail node by node
+  typedef float v4 __attribute__((ext_vector_type(4)));
   const float alpha = (float)st->alpha;
   double a = 0.0;
-  NK_LOOP(i, n) {
+  const int64_t n4 = n >> 2;
+  v4 *x4 = reinterpret_cast<v4 *>(x);
+  v4 *r4 = reinterpret_cast<v4 *>(r);
+  const v4 *p4 = reinterpret_cast<const v4 *>(p);
+  const v4 *w4 = reinterpret_cast<const v4 *>(w);
+  const v4 *t4 = reinterpret_cast<const v4 *>(wt);
+  NK_LOOP(i, n4) {
+    __builtin_nontemporal_store(__builtin_nontemporal_load(x4 + i) +
+                                    alpha * __builtin_nontemporal_load(p4 + i),
+                                x4 + i);
+    const v4 ri = __builtin_nontemporal_load(r4 + i) - alpha * __builtin_nontemporal_load(w4 + i);
+    __builtin_nontemporal_store(ri, r4 + i);
+    const v4 t = __builtin_nontemporal_load(t4 + i);
+    a += (double)ri.x * (double)ri.x * (double)t.x + (double)ri.y * (double)ri.y * (double)t.y +
+         (double)ri.z * (double)ri.z * (double)t.z + (double)ri.w * (double)ri.w * (double)t.w;
+  }
+  for (int64_t i = 4 * n4 + blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x) {
     x[i] = x[i] + alpha * p[i];
     const float ri = r[i] - alpha * w[i];
     r[i] = ri;
@@ -443,7 +464,7 @@ int run_flex_cg_mixed(Mesh &m, double *res, Coef h1, Coef h2, const double *wt,
       hipLaunchKernelGGL(k_mx_beta, dim3(1), B, 0, s, st, pa, pb);
       NK_TRY(launch_axhelm32_flex(m, mg.xw, mg.xp, mg.xz, h1f, h2f, st, den_part, s));
       NK_TRY(gs_assemble(m.gs, mg.xw, nullptr, s));
-      hipLaunchKernelGGL(k_mx_alpha, dim3(1), B, 0, s, st, den_part, nden);
+      hipLaunchKernelGGL(k_mx_alpha, dim3(1), dim3(1024), 0, s, st, den_part, nden);
       hipLaunchKernelGGL(k_mx_update, G, B, 0, s, mg.xx, mg.xr, mg.xp, mg.xw, mg.xwt, st, n, pa);
       hipLaunchKernelGGL(k_mx_rnorm, dim3(1), B, 0, s, st, pa, m.vol);
       NK_HIP(hipGetLastError());

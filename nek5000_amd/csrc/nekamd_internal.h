@@ -135,6 +135,32 @@ Timers &timers();
 // Elements per block of the element kernels (blocks of at least 64 lanes)
 // and the fixed grid of the CG-fused axhelm (its number of rho partials).
 inline int elems_per_block(int np) { return np * np >= 64 ? 1 : 64 / (np * np); }
+// Sum of count partials by one block (blockDim.x a multiple of 64, at most
+// 1024) in a fixed order: four interleaved accumulators per thread, so that
+// several loads are in flight, then the waves in order.  Valid on every
+// thread.  Used by the single-block scalar kernels of the Krylov loops.
+__device__ inline double block_sum_partials(const double *__restrict__ part, int count) {
+  __shared__ double sw[16];
+  const int t = threadIdx.x, nt = blockDim.x;
+  double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
+  int q = t;
+  for (; q + 3 * nt < count; q += 4 * nt) {
+    a0 += part[q];
+    a1 += part[q + nt];
+    a2 += part[q + 2 * nt];
+    a3 += part[q + 3 * nt];
+  }
+  for (; q < count; q += nt) a0 += part[q];
+  double v = (a0 + a1) + (a2 + a3);
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
+  if ((t & 63) == 0) sw[t >> 6] = v;
+  __syncthreads();
+  double s = 0.0;
+  for (int w = 0; w < (nt >> 6); ++w) s += sw[w];
+  __syncthreads();
+  return s;
+}
+
 inline int axhelm_cg_grid(const Mesh &m) {
   const int64_t g = (m.nel + elems_per_block(m.np) - 1) / elems_per_block(m.np);
   return (int)(g < kAxBlocks ? g : kAxBlocks);
