@@ -284,8 +284,10 @@ def adjoint_run(nel1d=32, lx1=8):
 
 def batch_run(steps=2):
     """Five solves of one time step (three velocity components on a 48^3 box,
-    h1 = 1e-3, h2 = 11/(6e-3); a passive scalar on 24^3, h1 = 1e-3; one on a
-    40x40x20 x-periodic box, h1 = 1e-2; N = 7, f ~ U(-1,1) per unique node,
+    h1 = 1e-3; a passive scalar on 24^3, h1 = 1e-3; one on a 40x40x20
+    x-periodic box, h1 = 1e-2; every field with the BDF3 mass term of
+    dt = 1e-3, h2 = bd(1)/dt = 11/(6e-3), as sethlm (subs1.f:1013-1036) sets
+    it for any field of a transient run; N = 7, f ~ U(-1,1) per unique node,
     Jacobi PCG to 1e-8 relative, at most 900 iterations as cggo's maxcg):
     one after another through hmholtz in FP64 (the baseline), and batched
     (nkd_hmholtz_batch: the iteration of all five in one hipGraph) in FP64
@@ -296,9 +298,10 @@ def batch_run(steps=2):
     from nek5000_amd.mesh import BoxMesh
 
     solver.init(0)
-    specs = [("vel", BoxMesh(48, 48, 48, 8, deform=0.1), 1e-3, 11.0 / (6.0 * 1e-3)),
-             ("A", BoxMesh(24, 24, 24, 8, deform=0.1), 1e-3, 0.0),
-             ("B", BoxMesh(40, 40, 20, 8, periodic_x=True), 1e-2, 0.0)]
+    h2_bdf3 = 11.0 / (6.0 * 1e-3)
+    specs = [("vel", BoxMesh(48, 48, 48, 8, deform=0.1), 1e-3, h2_bdf3),
+             ("A", BoxMesh(24, 24, 24, 8, deform=0.1), 1e-3, h2_bdf3),
+             ("B", BoxMesh(40, 40, 20, 8, periodic_x=True), 1e-2, h2_bdf3)]
     fields, solves, dofs = [], [], []
     for name, box, h1v, h2v in specs:
         F = solver.box_field(box)
@@ -360,7 +363,8 @@ def batch_run(steps=2):
             float(torch.linalg.norm(a - b) / torch.linalg.norm(b))
             for a, b in zip(sols[tag], sols["sequential_fp64"]))
     line = dict(run="batch", workload="3 x velocity 48^3 + scalar 24^3 + scalar 40x40x20 "
-                                      "periodic, N=7, Jacobi PCG to 1e-8 relative (maxcg 900)",
+                                      "periodic, N=7, h2=bd(1)/dt=11/(6e-3) for all, Jacobi PCG "
+                                      "to 1e-8 relative (maxcg 900)",
                 local_nodes=[int(F.n) for F in fields], free_dof=dofs, variants=rows)
     print(json.dumps(line), flush=True)
     B.destroy()
@@ -599,6 +603,8 @@ def main():
         cpu = cpu_baseline(args.cpu_nel, args.lx1, 0.1, args.cpu_seconds, args.solver)
         log(f"[bench] cpu baseline {cpu}")
 
+    prec_name = {0: "fp64", 1: "fp64 (FP32 fine-level smoother)",
+                 2: "mixed (FP32 inner iteration, FP64 residuals)"}[args.precision]
     line = {
         "metric": METRIC,
         "value": value,
@@ -615,11 +621,11 @@ def main():
         "config": {
             "workload": (f"hmholtz E={args.nel}^3={box.nel} N={args.lx1 - 1} deformed box "
                          f"(0.1h sin map), h1=1 h2=10, f~U(-1,1), Dirichlet, "
-                         + (f"flexible CG + h1mg V-cycle (PRES, param42=2) FP64 to "
+                         + (f"flexible CG + h1mg V-cycle (PRES, param42=2) {prec_name} to "
                             f"{args.tol:g} relative" if hsmg else
-                            f"Jacobi-PCG FP64 to {args.tol:g} relative")),
+                            f"Jacobi-PCG {prec_name} to {args.tol:g} relative")),
             "nel": box.nel, "lx1": args.lx1,
-            "preconditioner": "h1mg" if hsmg else "jacobi", "precision": "fp64",
+            "preconditioner": "h1mg" if hsmg else "jacobi", "precision": prec_name,
             "multigrid": mg_info,
             "coarse_solver": (None if not hsmg else
                               ("one AMG V-cycle (param40=1)" if args.crs == 1 else

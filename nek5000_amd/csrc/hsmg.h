@@ -46,6 +46,7 @@ struct MgLevel {
   // single-precision copies for the mixed-precision smoother (fine level)
   float *fs32 = nullptr, *fl32 = nullptr, *wint32 = nullptr, *wgh32 = nullptr, *e32 = nullptr;
   float *mswt32 = nullptr;  // mask * Schwarz weight, single precision
+  float *rwt32 = nullptr;   // restriction weights, single precision
 };
 
 struct Mg {

@@ -284,9 +284,9 @@ int launch_sch_out(R *e, const R *wint, const R *wgh, int nh, int64_t nel, hipSt
 //           finer level): v = v * w + s.
 //   MODE 3 (prolongation into the fine level before gs_fin): v = s on
 //           element-boundary nodes, v = eb * w + s inside.
-template <int NV, int NU, int MODE, typename EB, typename UT, typename VT>
+template <int NV, int NU, int MODE, typename EB, typename UT, typename VT, typename WT>
 __global__ void k_tnsr(VT *__restrict__ v, const UT *__restrict__ u,
-                       const double *__restrict__ w, const double *__restrict__ A, int64_t nel,
+                       const WT *__restrict__ w, const double *__restrict__ A, int64_t nel,
                        const EB *__restrict__ eb, const double *__restrict__ om) {
   constexpr int NV3 = NV * NV * NV, NU3 = NU * NU * NU;
   __shared__ double sA[NV * NU];
@@ -298,7 +298,7 @@ __global__ void k_tnsr(VT *__restrict__ v, const UT *__restrict__ u,
     __syncthreads();
     for (int t = threadIdx.x; t < NU3; t += blockDim.x) {
       double x = double(u[(int64_t)e * NU3 + t]);
-      if constexpr (MODE == 1) x = x * w[(int64_t)e * NU3 + t];
+      if constexpr (MODE == 1) x = x * double(w[(int64_t)e * NU3 + t]);
       su[t] = x;
     }
     __syncthreads();
@@ -322,11 +322,11 @@ __global__ void k_tnsr(VT *__restrict__ v, const UT *__restrict__ u,
       for (int m = 0; m < NU; ++m) s += s2[ij + NV * NV * m] * sA[k + NV * m];
       const int64_t o = (int64_t)e * NV3 + t;
       if constexpr (MODE == 2) {
-        v[o] = VT(double(v[o]) * w[o] + s);
+        v[o] = VT(double(v[o]) * double(w[o]) + s);
       } else if constexpr (MODE == 3) {
         const int i = t % NV, j = (t / NV) % NV, k = t / (NV * NV);
         const bool bnd = (i == 0 || i == NV - 1 || j == 0 || j == NV - 1 || k == 0 || k == NV - 1);
-        v[o] = VT(bnd ? s : double(eb[o]) * w[o] + s);
+        v[o] = VT(bnd ? s : double(eb[o]) * double(w[o]) + s);
       } else {
         v[o] = VT(om ? s * om[o] : s);
       }
@@ -609,14 +609,15 @@ int mg_schwarz_local(Mg &mg, int l, const double *r, double *e, hipStream_t s) {
 
 // om (MODE 1 only, may be null): 0/1 mask applied to the output.  UT/VT:
 // storage of the input / output field (arithmetic in FP64).
-template <int MODE, typename EB = double, typename UT = double, typename VT = double>
-static int run_tnsr(VT *v, int nv, const UT *u, int nu, const double *w, const double *A,
+template <int MODE, typename EB = double, typename UT = double, typename VT = double,
+          typename WT = double>
+static int run_tnsr(VT *v, int nv, const UT *u, int nu, const WT *w, const double *A,
                     int64_t nel, hipStream_t s, const EB *eb = nullptr,
                     const double *om = nullptr) {
   const int grid = (int)std::min<int64_t>(nel, 1 << 20);
 #define NK_T(V, U)                                                                       \
   if (nv == V && nu == U) {                                                              \
-    hipLaunchKernelGGL((k_tnsr<V, U, MODE, EB, UT, VT>), dim3(grid),                    \
+    hipLaunchKernelGGL((k_tnsr<V, U, MODE, EB, UT, VT, WT>), dim3(grid),                \
                        dim3(V * V * V > 256 ? 256 : 64),                                  \
                        0, s, v, u, w, A, nel, eb, om);                                   \
     NK_HIP(hipGetLastError());                                                           \
@@ -681,14 +682,32 @@ static int mg_solve_t(Mesh &mesh, FT *z, const FT *r, hipStream_t s, int crs_mod
            "single-precision V-cycle needs global ids zero exactly at element interiors");
   const bool fp32 = (precision == 1 || f32io) && merged;
   if (fp32) NK_TRY(mg_prepare_fp32(mg, s));
-  // the restriction of the fine residual runs first, on s: beside the fine
-  // local solves it would be starved of the compute units they occupy
+  // The streaming head of the coarse chain runs first, on s: the
+  // restriction of the fine residual, its dssum and (when that level is 2)
+  // the restriction to level 1.  Beside the fine local solves these
+  // bandwidth-bound passes are starved of the compute units the solves
+  // occupy, and the whole coarse chain waits on them; once they are done
+  // the coarse solve, latency-bound, overlaps the fine level.
+  MgLevel &L1 = mg.lev[1], &L2 = mg.lev[2];
+  const double *rsrc = nullptr;  // restriction source below the fine level
+  bool l1_restricted = false;
   if (Lm - 1 >= 2) {
     MgLevel &C = mg.lev[Lm - 1];
     // mask * dssum(v) = dssum(mask * v): the level mask is 0/1 and equal at
     // every copy of a node (made so by a product gs at set-up)
-    NK_TRY((run_tnsr<1, double, FT, double>(C.r, C.nh, r, T.nh, T.rwt, C.Jt, nel, s,
-                                            (const double *)nullptr, C.mask)));
+    if constexpr (f32io)  // single-precision input: single-precision weights
+      NK_TRY((run_tnsr<1, double, FT, double, float>(C.r, C.nh, r, T.nh, T.rwt32, C.Jt, nel,
+                                                     s, (const double *)nullptr, C.mask)));
+    else
+      NK_TRY((run_tnsr<1, double, FT, double>(C.r, C.nh, r, T.nh, T.rwt, C.Jt, nel, s,
+                                              (const double *)nullptr, C.mask)));
+    NK_TRY(gs_op(C.gs, C.r, 0, s));
+    NK_HIP(hipEventRecord(mg.ev_lr[Lm - 1], s));
+    rsrc = C.r;
+    if (Lm - 1 == 2) {
+      NK_TRY(run_tnsr<1>(L1.r, L1.nh, C.r, L2.nh, L2.rwt, L1.Jt, nel, s));
+      l1_restricted = true;
+    }
   }
   NK_HIP(hipEventRecord(mg.ev_fork, s));
   NK_HIP(hipStreamWaitEvent(b, mg.ev_fork, 0));
@@ -702,27 +721,26 @@ static int mg_solve_t(Mesh &mesh, FT *z, const FT *r, hipStream_t s, int crs_mod
   // intermediate level on c (it only reads the masked residual), beside the
   // restriction to level 1 and the coarse solve on b
   hipStream_t c = mg.sc;
-  const double *rsrc = nullptr;  // restriction source below the fine level
   for (int l = Lm - 1; l >= 2; --l) {
     MgLevel &C = mg.lev[l], &F = mg.lev[l + 1];
-    if (l < Lm - 1)  // (the level below the fine one was restricted above)
+    if (l < Lm - 1) {  // (the level below the fine one was done on s above)
       NK_TRY(run_tnsr<1>(C.r, C.nh, rsrc, F.nh, F.rwt, C.Jt, nel, b, (const double *)nullptr,
                          C.mask));
-    NK_TRY(gs_op(C.gs, C.r, 0, b));
-    NK_HIP(hipEventRecord(mg.ev_lr[l], b));
+      NK_TRY(gs_op(C.gs, C.r, 0, b));
+      NK_HIP(hipEventRecord(mg.ev_lr[l], b));
+    }
     NK_HIP(hipStreamWaitEvent(c, mg.ev_lr[l], 0));
     NK_TRY(mg_schwarz_local(mg, l, C.r, C.e, c));
     NK_TRY(gs_op(C.gs, C.e, 0, c));
     NK_HIP(hipEventRecord(mg.ev_le[l], c));
     rsrc = C.r;
   }
-  MgLevel &L1 = mg.lev[1], &L2 = mg.lev[2];
   // the level-1 mask needs no pass of its own: the coarse gather reads only
   // unmasked vertices and the scatter writes 0 at masked ones (their slot is
   // -1), which is what masking r1 before and e1 after the solve would give
   if (Lm == 2)
     NK_TRY((run_tnsr<1, double, FT, double>(L1.r, L1.nh, r, L2.nh, L2.rwt, L1.Jt, nel, b)));
-  else
+  else if (!l1_restricted)
     NK_TRY(run_tnsr<1>(L1.r, L1.nh, rsrc, L2.nh, L2.rwt, L1.Jt, nel, b));
   NK_TRY(coarse_solve(mg, L1.e, L1.r, crs_mode, b));
   for (int l = 2; l < Lm; ++l) {  // e_l = e_l * (mask*wt) + J e_{l-1}
@@ -735,9 +753,14 @@ static int mg_solve_t(Mesh &mesh, FT *z, const FT *r, hipStream_t s, int crs_mod
   // z = dssum(e)*(mask*wt) + J e_{lmax-1}, then dsavg (ic.f:1871-1897)
   MgLevel &C = mg.lev[Lm - 1];
   if (fp32) {
-    NK_TRY((run_tnsr<3, float, double, FT>(z, T.nh, C.e, C.nh, T.mswt, C.J, nel, s, T.e32)));
-    if constexpr (f32io) NK_TRY(gs_fin(mesh.gs, z, T.e32, T.mswt32, s));
-    else NK_TRY(gs_fin(mesh.gs, z, T.e32, T.mswt, s));
+    if constexpr (f32io) {
+      NK_TRY((run_tnsr<3, float, double, FT, float>(z, T.nh, C.e, C.nh, T.mswt32, C.J, nel, s,
+                                                    T.e32)));
+      NK_TRY(gs_fin(mesh.gs, z, T.e32, T.mswt32, s));
+    } else {
+      NK_TRY((run_tnsr<3, float, double, FT>(z, T.nh, C.e, C.nh, T.mswt, C.J, nel, s, T.e32)));
+      NK_TRY(gs_fin(mesh.gs, z, T.e32, T.mswt, s));
+    }
   } else if constexpr (!f32io) {
     if (merged) {
       NK_TRY(run_tnsr<3>(z, T.nh, C.e, C.nh, T.mswt, C.J, nel, s, T.e));

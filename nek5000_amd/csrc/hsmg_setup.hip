@@ -483,7 +483,9 @@ int mg_prepare_fp32(Mg &mg, hipStream_t s) {
   NK_TRY(falloc(&L.wgh32, ngh));
   NK_TRY(falloc(&L.e32, L.n));
   NK_TRY(falloc(&L.mswt32, L.n));
+  NK_TRY(falloc(&L.rwt32, L.n));
   hipLaunchKernelGGL(k_to_f32, dim3(mg_grid(L.n)), dim3(256), 0, s, L.mswt32, L.mswt, L.n);
+  hipLaunchKernelGGL(k_to_f32, dim3(mg_grid(L.n)), dim3(256), 0, s, L.rwt32, L.rwt, L.n);
   hipLaunchKernelGGL(k_to_f32, dim3(mg_grid(ns)), dim3(256), 0, s, L.fs32, L.fs, ns);
   hipLaunchKernelGGL(k_to_f32, dim3(mg_grid(nlam)), dim3(256), 0, s, L.fl32, L.fl, nlam);
   // ghost slots without a neighbour are never written: 0
@@ -497,7 +499,7 @@ static void free_level(MgLevel &L) {
     if (p) (void)hipFree(p);
   for (int32_t *p : {L.gin, L.gout})
     if (p) (void)hipFree(p);
-  for (float *p : {L.fs32, L.fl32, L.wint32, L.wgh32, L.e32, L.mswt32})
+  for (float *p : {L.fs32, L.fl32, L.wint32, L.wgh32, L.e32, L.mswt32, L.rwt32})
     if (p) (void)hipFree(p);
   if (L.own_gs) gs_free(L.gs);
 }
