@@ -457,8 +457,8 @@ int run_flex_cg_mixed(Mesh &m, double *res, Coef h1, Coef h2, const double *wt,
     hipLaunchKernelGGL(k_mx_state, dim3(1), dim3(1), 0, s, st, itol);
     NK_HIP(hipGetLastError());
     while (total < maxit) {
-      // the inner V-cycles' coarse problem: one V-cycle for param40 = 1,
-      // else solved to the inner solve's own tolerance, 1e-4
+      // the inner V-cycles' coarse problem: one AMG V-cycle for param40 = 1,
+      // else the coarse AMG-PCG to 0.1 relative (crs_mode 2)
       NK_TRY(mg_solve32(m, mg.xz, mg.xr, s, prm.param40 == 1 ? 1 : 2));
       hipLaunchKernelGGL(k_mx_dots, G, B, 0, s, mg.xz, mg.xr, mg.xw, mg.xwt, n, pa, pb);
       hipLaunchKernelGGL(k_mx_beta, dim3(1), B, 0, s, st, pa, pb);
