@@ -15,8 +15,10 @@
 //               land on this element's first interior layer (gout);
 //   gs(level) : e summed at shared nodes.
 // The weights (mask * Schwarz weight) are applied where the prolongation adds
-// the coarser correction.  The coarse levels run on a second stream, beside
-// the fine-level smoother, and join before the last prolongation.
+// the coarser correction.  The bandwidth-bound head of the coarse chain
+// (restrictions and the level-2 dssum) runs before the fine-level smoother;
+// the coarse solve and the level-2 Schwarz then run on two more streams,
+// beside it, and join before the last prolongation.
 #include <algorithm>
 #include <cmath>
 #include <cstring>
