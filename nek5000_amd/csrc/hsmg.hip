@@ -226,15 +226,15 @@ __global__ void __launch_bounds__(Fdm<NL>::kThreads)
 // Blocks of 64 walk elements; the element size N is a compile-time constant
 // so the node coordinates cost no divisions, and each lane issues all its
 // loads of an element before using them.
-template <typename R, int N>
+template <typename R, int N, typename EO = R>
 __global__ void __launch_bounds__(64)
-    k_sch_out(R *__restrict__ eo, const R *__restrict__ wint, const R *__restrict__ wgh,
+    k_sch_out(EO *__restrict__ eo, const R *__restrict__ wint, const R *__restrict__ wgh,
               int64_t nel) {
   constexpr int N2 = N * N, N3 = N2 * N, U = (N3 + 63) / 64;
   for (int64_t e = blockIdx.x; e < nel; e += gridDim.x) {
     const R *g = wgh + e * 6 * N2;
     const R *wi = wint + e * N3;
-    R *out = eo + e * N3;
+    EO *out = eo + e * N3;
     R v[U];
 #pragma unroll
     for (int u = 0; u < U; ++u) {
@@ -253,20 +253,20 @@ __global__ void __launch_bounds__(64)
       if (j == N - 2) a += g[3 * N2 + i + N * k];
       if (k == 1) a += g[4 * N2 + i + N * j];
       if (k == N - 2) a += g[5 * N2 + i + N * j];
-      out[q] = a;
+      out[q] = EO(a);
     }
   }
 }
 
-template <typename R>
-int launch_sch_out(R *e, const R *wint, const R *wgh, int nh, int64_t nel, hipStream_t s) {
+template <typename R, typename EO = R>
+int launch_sch_out(EO *e, const R *wint, const R *wgh, int nh, int64_t nel, hipStream_t s) {
   // a few elements per block: one small element per block is dominated by
   // block launch
   const int grid = (int)std::min<int64_t>(nel, 16384);
   switch (nh) {
 #define NK_O(N)                                                                             \
   case N:                                                                                   \
-    hipLaunchKernelGGL((k_sch_out<R, N>), dim3(grid), dim3(64), 0, s, e, wint, wgh, nel); \
+    hipLaunchKernelGGL((k_sch_out<R, N, EO>), dim3(grid), dim3(64), 0, s, e, wint, wgh, nel); \
     break;
     NK_O(3) NK_O(4) NK_O(5) NK_O(6) NK_O(7) NK_O(8) NK_O(9) NK_O(10) NK_O(11) NK_O(12)
     NK_O(13) NK_O(14) NK_O(15) NK_O(16)
@@ -552,8 +552,8 @@ int mg_overlap_out(Mg &mg, int l, double *e, hipStream_t s) {
 
 // Local solves + overlap sum in precision R: double uses the level's own
 // arrays, float the single-precision copies (fine level, mixed precision).
-template <typename R, typename RI = double>
-static int schwarz_local(Mg &mg, int l, const RI *r, R *e, hipStream_t s) {
+template <typename R, typename RI = double, typename EO = R>
+static int schwarz_local(Mg &mg, int l, const RI *r, EO *e, hipStream_t s) {
   MgLevel &L = mg.lev[l];
   R *wint, *wgh;
   const R *S, *lam;
@@ -592,7 +592,7 @@ static int schwarz_local(Mg &mg, int l, const RI *r, R *e, hipStream_t s) {
       return NKH_ERR_UNSUPPORTED;
   }
   NK_HIP(hipGetLastError());
-  NK_TRY(launch_sch_out<R>(e, wint, wgh, L.nh, mg.nel, s));
+  NK_TRY((launch_sch_out<R, EO>(e, wint, wgh, L.nh, mg.nel, s)));
   NK_HIP(hipGetLastError());
   return NKH_OK;
 }
@@ -737,7 +737,10 @@ static int mg_solve_t(Mesh &mesh, FT *z, const FT *r, hipStream_t s, int crs_mod
       NK_HIP(hipEventRecord(mg.ev_lr[l], b));
     }
     NK_HIP(hipStreamWaitEvent(c, mg.ev_lr[l], 0));
-    NK_TRY(mg_schwarz_local(mg, l, C.r, C.e, c));
+    if constexpr (f32io)  // inner solve of the mixed solver: FP32 local solves
+      NK_TRY((schwarz_local<float, double, double>(mg, l, C.r, C.e, c)));
+    else
+      NK_TRY(mg_schwarz_local(mg, l, C.r, C.e, c));
     NK_TRY(gs_op(C.gs, C.e, 0, c));
     NK_HIP(hipEventRecord(mg.ev_le[l], c));
     rsrc = C.r;

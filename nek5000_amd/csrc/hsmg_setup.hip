@@ -471,9 +471,9 @@ int falloc(float **p, int64_t n) {
 }
 }  // namespace
 
-int mg_prepare_fp32(Mg &mg, hipStream_t s) {
-  MgLevel &L = mg.lev[mg.lmax];
-  if (L.fs32) return NKH_OK;
+// single-precision smoother data of level l: S, eigenvalues, local solution
+// and ghost slots
+static int prepare_level_fp32(Mg &mg, MgLevel &L, hipStream_t s) {
   const int nl = L.nh + 2;
   const int64_t ns = (int64_t)nl * nl * 3 * mg.nel, nlam = (int64_t)(3 * nl + 1) * mg.nel;
   const int64_t ngh = (int64_t)6 * L.nh * L.nh * mg.nel;
@@ -481,15 +481,26 @@ int mg_prepare_fp32(Mg &mg, hipStream_t s) {
   NK_TRY(falloc(&L.fl32, nlam));
   NK_TRY(falloc(&L.wint32, L.n));
   NK_TRY(falloc(&L.wgh32, ngh));
+  hipLaunchKernelGGL(k_to_f32, dim3(mg_grid(ns)), dim3(256), 0, s, L.fs32, L.fs, ns);
+  hipLaunchKernelGGL(k_to_f32, dim3(mg_grid(nlam)), dim3(256), 0, s, L.fl32, L.fl, nlam);
+  // ghost slots without a neighbour are never written: 0
+  hipLaunchKernelGGL(k_fill_f, dim3(mg_grid(ngh)), dim3(256), 0, s, L.wgh32, 0.0f, ngh);
+  NK_HIP(hipGetLastError());
+  return NKH_OK;
+}
+
+int mg_prepare_fp32(Mg &mg, hipStream_t s) {
+  MgLevel &L = mg.lev[mg.lmax];
+  if (L.fs32) return NKH_OK;
+  // intermediate levels (the mixed solver's inner V-cycles smooth them in
+  // FP32 too)
+  for (int l = 2; l < mg.lmax; ++l) NK_TRY(prepare_level_fp32(mg, mg.lev[l], s));
+  NK_TRY(prepare_level_fp32(mg, L, s));
   NK_TRY(falloc(&L.e32, L.n));
   NK_TRY(falloc(&L.mswt32, L.n));
   NK_TRY(falloc(&L.rwt32, L.n));
   hipLaunchKernelGGL(k_to_f32, dim3(mg_grid(L.n)), dim3(256), 0, s, L.mswt32, L.mswt, L.n);
   hipLaunchKernelGGL(k_to_f32, dim3(mg_grid(L.n)), dim3(256), 0, s, L.rwt32, L.rwt, L.n);
-  hipLaunchKernelGGL(k_to_f32, dim3(mg_grid(ns)), dim3(256), 0, s, L.fs32, L.fs, ns);
-  hipLaunchKernelGGL(k_to_f32, dim3(mg_grid(nlam)), dim3(256), 0, s, L.fl32, L.fl, nlam);
-  // ghost slots without a neighbour are never written: 0
-  hipLaunchKernelGGL(k_fill_f, dim3(mg_grid(ngh)), dim3(256), 0, s, L.wgh32, 0.0f, ngh);
   NK_HIP(hipGetLastError());
   return NKH_OK;
 }
