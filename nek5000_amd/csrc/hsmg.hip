@@ -669,7 +669,8 @@ static int coarse_solve(Mg &mg, double *e1, const double *r1, int crs_mode, hipS
 // The V-cycle with the fine-level input r and output z stored as FT: double
 // (the drop-in; precision 1 runs the fine local solves in FP32 inside), or
 // float (the single-precision inner solve of the mixed solver: fine level
-// in FP32 throughout, levels 2.. and the coarse solve in FP64).
+// in FP32 throughout; the local solves of levels 2.. in FP32 between FP64
+// residuals and corrections; the coarse solve in FP64).
 template <typename FT>
 static int mg_solve_t(Mesh &mesh, FT *z, const FT *r, hipStream_t s, int crs_mode,
                       int precision) {
