@@ -218,6 +218,15 @@ typedef struct {
 
 static int64_t npow3(int n) { return (int64_t)n * n * n; }
 
+/* Planted mistakes for the tests that prove the pins can fail (never set
+   outside tests): bit 1 no dsavg, 2 no Schwarz weights, 4 no ghost-layer
+   exchange, 8 no overlap sum of the neighbours' ghost solutions, 16
+   plane_space weights w(j) read unshifted, 32 neighbour lengths replaced by
+   the element's own, 64 each Schwarz weight applied once (not as many times
+   as the weight loops visit the node). */
+static int g_mutation = 0;
+void nko_set_mutation(int m) { g_mutation = m; }
+
 /* hsmg.f:2272-2323 H1MG_SETUP_MG_NX */
 static void mg_levels(int lx1, int *lmax, int *nx) {
   static const int mgn2[10] = {1, 2, 2, 2, 2, 3, 3, 5, 5, 5};
@@ -268,7 +277,7 @@ static void plane_space(int n, int64_t nel, const double *w, const double *x,
     double s = 0, ws = 0;
     for (int k = 1; k <= nx; k++)
       for (int j = 1; j <= nx; j++) {
-        double wt = w[j - 1] * w[k - 1];
+        double wt = (g_mutation & 16) ? w[j] * w[k] : w[j - 1] * w[k - 1];
         double dx = X(x, n2, j, k, e) - X(x, 0, j, k, e), dy = X(y, n2, j, k, e) - X(y, 0, j, k, e),
                dz = X(z, n2, j, k, e) - X(z, 0, j, k, e);
         s = s + wt / (dx * dx + dy * dy + dz * dz);
@@ -280,7 +289,7 @@ static void plane_space(int n, int64_t nel, const double *w, const double *x,
     ws = 0;
     for (int k = 1; k <= nx; k++)
       for (int i = 1; i <= nx; i++) {
-        double wt = w[i - 1] * w[k - 1];
+        double wt = (g_mutation & 16) ? w[i] * w[k] : w[i - 1] * w[k - 1];
         double dx = X(x, i, n2, k, e) - X(x, i, 0, k, e), dy = X(y, i, n2, k, e) - X(y, i, 0, k, e),
                dz = X(z, i, n2, k, e) - X(z, i, 0, k, e);
         s = s + wt / (dx * dx + dy * dy + dz * dz);
@@ -292,7 +301,7 @@ static void plane_space(int n, int64_t nel, const double *w, const double *x,
     ws = 0;
     for (int j = 1; j <= nx; j++)
       for (int i = 1; i <= nx; i++) {
-        double wt = w[i - 1] * w[j - 1];
+        double wt = (g_mutation & 16) ? w[i] * w[j] : w[i - 1] * w[j - 1];
         double dx = X(x, i, j, n2, e) - X(x, i, j, 0, e), dy = X(y, i, j, n2, e) - X(y, i, j, 0, e),
                dz = X(z, i, j, n2, e) - X(z, i, j, 0, e);
         s = s + wt / (dx * dx + dy * dy + dz * dz);
@@ -331,6 +340,8 @@ static void swap_lengths(int n, int64_t nel, const double *w, const double *x, c
     lr[3 * e + 1] = LL(1, n - 1, 1, e) - lm[3 * e + 1];
     ll[3 * e + 2] = LL(1, 1, 0, e) - lm[3 * e + 2];
     lr[3 * e + 2] = LL(1, 1, n - 1, e) - lm[3 * e + 2];
+    if (g_mutation & 32)
+      for (int d = 0; d < 3; d++) ll[3 * e + d] = lr[3 * e + d] = lm[3 * e + d];
   }
 #undef LL
   free(L);
@@ -549,7 +560,7 @@ static void overlap_sum(nko_mg *mg, int l, double *W1, double *W2, double *e) {
   extrude(W1, 0, 0.0, W2, 0, 1.0, m, nel);
   nko_gs_op(mg->gsx[l], W2, 0);
   extrude(W2, 0, 1.0, W1, 0, -1.0, m, nel);
-  extrude(W2, 2, 1.0, W2, 0, 1.0, m, nel);
+  if (!(g_mutation & 8)) extrude(W2, 2, 1.0, W2, 0, 1.0, m, nel);
   toreg(e, W2, n, nel);
   nko_gs_op(mg->gs[l], e, 0);
 }
@@ -568,6 +579,14 @@ static void level_schwarz_wt(nko_mg *mg, int l) {
   for (int64_t q = 0; q < nt; q++) wt[q] = 1.0;
   const int lay[4] = {0, 1, n - 2, n - 1};
 #define Q(i, j, k) ((i) + n * ((j) + n * ((k) + (int64_t)n * e)))
+  if (g_mutation & 64) {
+    for (int64_t q = 0; q < nt; q++) wt[q] = 1.0 / cnt[q];
+    free(W1);
+    free(W2);
+    free(cnt);
+    mg->swt[l] = wt;
+    return;
+  }
   for (int64_t e = 0; e < nel; e++) {
     for (int k = 0; k < n; k++)
       for (int j = 0; j < n; j++)
@@ -809,15 +828,18 @@ static void schwarz(nko_mg *mg, int l, double *e, double *r, double sigma) {
   for (int64_t q = 0; q < nt; q++) r[q] = r[q] * mg->mask[l][q]; /* h1mg_mask */
   double *W1 = malloc(sizeof(double) * ns), *W2 = malloc(sizeof(double) * ns);
   toext(W1, r, n, nel);
-  extrude(W1, 0, 0.0, W1, 2, 1.0, m, nel);
-  nko_gs_op(mg->gsx[l], W1, 0);
-  extrude(W1, 0, 1.0, W1, 2, -1.0, m, nel);
+  if (!(g_mutation & 4)) {
+    extrude(W1, 0, 0.0, W1, 2, 1.0, m, nel);
+    nko_gs_op(mg->gsx[l], W1, 0);
+    extrude(W1, 0, 1.0, W1, 2, -1.0, m, nel);
+  }
   do_fast(W2, W1, mg->fs[l], mg->fd[l], nl, nel);
   overlap_sum(mg, l, W1, W2, e);
   for (int64_t q = 0; q < nt; q++) e[q] = e[q] * mg->mask[l][q];
   free(W1);
   free(W2);
-  for (int64_t q = 0; q < nt; q++) e[q] = e[q] * mg->swt[l][q];
+  if (!(g_mutation & 2))
+    for (int64_t q = 0; q < nt; q++) e[q] = e[q] * mg->swt[l][q];
   for (int64_t q = 0; q < nt; q++) e[q] = sigma * e[q];
 }
 
@@ -876,8 +898,10 @@ void nko_mg_solve(nko_mg *mg, double *z, double *rhs) {
   }
   /* ic.f:1871-1897 DSAVG */
   int64_t nt = npow3(mg->nh[L]) * nel;
-  nko_gs_op(mg->gs_fine, z, 0);
-  for (int64_t q = 0; q < nt; q++) z[q] = z[q] * mg->vmult[q];
+  if (!(g_mutation & 1)) {
+    nko_gs_op(mg->gs_fine, z, 0);
+    for (int64_t q = 0; q < nt; q++) z[q] = z[q] * mg->vmult[q];
+  }
   for (int l = 1; l <= L; l++) {
     free(e[l]);
     free(r[l]);

@@ -124,6 +124,7 @@ def lib():
         L.nko_project2.argtypes = [ctypes.POINTER(_Field), vp, vp, vp, vp, vp, vp]
         L.nko_num_threads.restype = c_int
         L.nko_set_num_threads.argtypes = [c_int]
+        L.nko_set_mutation.argtypes = [c_int]
         _lib = L
     return _lib
 
@@ -493,3 +494,18 @@ def num_threads():
 
 def set_num_threads(t):
     lib().nko_set_num_threads(int(t))
+
+
+class mutation:
+    """Context manager planting a mistake in the oracle's multigrid (the bits
+    of nko_set_mutation), for tests that show a pin can fail."""
+
+    def __init__(self, bits):
+        self.bits = int(bits)
+
+    def __enter__(self):
+        lib().nko_set_mutation(self.bits)
+        return self
+
+    def __exit__(self, *a):
+        lib().nko_set_mutation(0)

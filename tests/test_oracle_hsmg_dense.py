@@ -1,0 +1,116 @@
+"""Pins the oracle's h1mg_solve (hsmg.f:1855-1949) against the dense model of
+tests/dense_mg.py, which is built from the definition of every piece --
+lattice numbering instead of setvert3d, Schwarz patches from the box's
+geometry, local solves as inverses of the assembled extended operator,
+weights from patch counts, restriction as the transpose of interpolation,
+the Galerkin coarse matrix, dsavg -- and shares nothing with the oracle's
+multigrid code but the element operator (axhelm, pinned on its own).
+
+The meshes are small boxes with every element's local frame turned by a
+seeded proper rotation of the cube and the vertex ids relabelled, deformed
+or jittered, with Dirichlet and Neumann faces: every orientation of the
+face/edge/vertex sharing, every boundary case of the local solves, and at
+lx1 = 5 the middle-layer weight visited twice (nh = 3 on level 2).
+
+Each mistake planted in the oracle (oracle.mutation) makes the comparison
+fail: the pin can tell.
+"""
+import numpy as np
+import pytest
+
+from dense_mg import DenseMG, levels_nh
+from meshes import RotatedBox
+from nek5000_amd.mesh import BoxMesh
+from oracle import oracle as O
+from problems import Problem
+
+SYM_CODES = {"dirichlet": (1, 1, 1, 1, 1, 1), "mixed": (1, 1, 2, 2, 1, 2)}
+
+
+def build(nel, lx1, codes="dirichlet", seed=0, rotate=True, **kw):
+    box = BoxMesh(*nel, lx1, **kw)
+    rb = RotatedBox(box, seed=seed, rotate=rotate)
+    # box-frame symmetric-order codes: the given code on domain-boundary faces
+    prep = box.dirichlet_faces()
+    sym = prep[:, list(rb.SYM_TO_PREP)]
+    c = np.array(SYM_CODES[codes])[None, :] * sym
+    fbc = rb.face_codes(c.astype(np.int32))
+    P = Problem(rb, fbc=fbc)
+    mg = O.MG(P.field, rb.vertex, np.ascontiguousarray(fbc, np.int32), P.x, P.y, P.zc, P.w)
+    dm = DenseMG(rb, fbc, rb.xyz(), P.field.axhelm)
+    return rb, P, mg, dm
+
+
+CASES = [
+    dict(nel=(3, 3, 3), lx1=8, deform=0.1, seed=1),
+    dict(nel=(3, 2, 2), lx1=8, jitter=0.15, seed=2, codes="mixed"),
+    dict(nel=(3, 3, 2), lx1=5, jitter=0.1, seed=3),
+    dict(nel=(2, 3, 2), lx1=5, deform=0.1, seed=4, codes="mixed"),
+    dict(nel=(2, 2, 3), lx1=4, jitter=0.1, seed=5),
+    dict(nel=(3, 3, 3), lx1=6, deform=0.1, seed=6, rotate=False),
+]
+
+
+def compare(dm, mg, rng):
+    Lf = dm.lev[-1]
+    r = rng.standard_normal(Lf.N) * Lf.mask
+    z_dense = dm.solve(r)
+    z_or = mg.solve(np.ascontiguousarray(Lf.gather(r)))
+    return np.linalg.norm(z_or - Lf.gather(z_dense)) / np.linalg.norm(z_dense)
+
+
+@pytest.mark.parametrize("case", CASES)
+def test_h1mg_solve_equals_dense_model(case):
+    rb, P, mg, dm = build(**case)
+    assert mg.nh == dm.nhs == levels_nh(rb.lx1)
+    err = compare(dm, mg, np.random.default_rng(7))
+    assert err < 1e-11, err
+
+
+@pytest.mark.parametrize("case", CASES[:4])
+def test_swap_lengths_equal_plane_space_definition(case):
+    """fast3d.f:306-423 / 1542-1617: the oracle's element lengths and its
+    neighbour lengths equal the harmonic-mean definition evaluated on the
+    element's own GLL planes, the neighbour found from the box lattice."""
+    rb, P, mg, dm = build(**case)
+    nel = rb.nel
+    got = [np.ctypeslib.as_array(O.lib().nko_mg_lengths(mg.h, w), shape=(3 * nel,)).reshape(
+        nel, 3) for w in (0, 1, 2)]
+    names = ("ll", "lm", "lr")
+    ref = {"ll": dm.ll, "lm": dm.lm, "lr": dm.lr}
+    # faces without a neighbour: the reference's difference of its own
+    # length and itself, 0
+    for nm, g in zip(names, got):
+        np.testing.assert_allclose(g, ref[nm], rtol=1e-13, atol=1e-15, err_msg=nm)
+
+
+def test_plane_space_closed_form_on_affine_elements():
+    """On an undeformed box every plane distance is the element width: the
+    lengths are exactly (hx, hy, hz) in each element's frame."""
+    box = BoxMesh(3, 2, 2, 7, lengths=(1.5, 1.0, 0.5))
+    rb = RotatedBox(box, seed=9)
+    dm = DenseMG(rb, np.zeros((rb.nel, 6), np.int32), rb.xyz(), None, lengths_only=True)
+    for e in range(rb.nel):
+        for b in range(3):
+            g, _ = rb.axis_global(e, b)
+            assert abs(dm.lm[e, b] - box.h[g]) < 1e-14
+
+
+MUTATIONS = {1: "no dsavg", 2: "no Schwarz weights", 4: "no ghost exchange",
+             8: "no overlap sum", 16: "plane_space weights unshifted",
+             32: "neighbour lengths = own", 64: "weights applied once"}
+
+
+@pytest.mark.parametrize("bit", sorted(MUTATIONS))
+def test_planted_mistake_fails(bit):
+    """Each mistake planted in the oracle moves its V-cycle away from the
+    dense model by far more than the tolerance above, on one of two meshes
+    (lx1 = 5 rotated: nh = 3 on level 2, so the weights are discontinuous and
+    dsavg matters; lx1 = 8 jittered with Neumann faces)."""
+    worst = 0.0
+    for case in (CASES[2], CASES[1]):
+        with O.mutation(bit):
+            rb, P, mg, dm = build(**case)
+            err = compare(dm, mg, np.random.default_rng(7))
+        worst = max(worst, err)
+    assert worst > 1e-6, (MUTATIONS[bit], worst)
