@@ -22,17 +22,28 @@
  *                             symmetry, A*1 = 0.
  *   setvert3d               : geometric coincidence of GLL nodes; the
  *                             counting rules on edges and faces re-derived
- *                             geometrically.
+ *                             geometrically, on boxes with rotated element
+ *                             frames and permuted vertex ids and on the
+ *                             reference's own short-test meshes.
  *   gs (dssum/dsop)         : dense Q Q^T on tiny meshes; dssum(1)=multiplicity.
- *   setprec                 : diagonal of the brute-force assembled matrix.
- *   cggo                    : convergence to the analytic solution, residual
- *                             recomputed from scratch, PCG invariants.
+ *   setprec                 : diagonal of the brute-force assembled matrix
+ *                             (uniform coefficients); its h1-at-the-node and
+ *                             corner-only cross terms restated densely.
+ *   chktcg1                 : its three floors restated on global sums
+ *                             (Dirichlet, Neumann with and without h2).
+ *   cggo                    : iterate by iterate (rbn2, rtz1, rho, alpha,
+ *                             beta) against Jacobi PCG on the dense assembled
+ *                             global matrix with the same stop rule, norm and
+ *                             null-space correction; convergence to the
+ *                             analytic solution.
  */
 #include <math.h>
 #include <stdint.h>
 #include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
+
+extern int g_mutation; /* planted mistakes for tests (nek_oracle_hsmg.c) */
 #ifdef _OPENMP
 #include <omp.h>
 #endif
@@ -1121,6 +1132,7 @@ int nko_cggo(const nko_field *fl, double *x, const double *f, const double *h1,
     for (int64_t i = 0; i < n; i++) {
       s1 += z[i] * r[i] * fl->mult[i];
       if (prm->param18 == 1.0) s2 += r[i] * r[i] * fl->mult[i];
+      else if (g_mutation & 128) s2 += fl->mult[i] * r[i] * r[i];
       else s2 += fl->binv[i] * fl->mult[i] * r[i] * r[i]; /* vlsc32 :848-856 */
     }
     rtz1 = s1;

@@ -15,10 +15,14 @@
  * polynomials, A 1 = 0, B = GLL weights); generalev by the defining identities
  * A S = B S Lambda, S^T B S = I and scipy.linalg.eigh; the FDM local solve by
  * (S D S^T) A_ext = I on unmasked rows; the coarse matrix by brute-force
- * Galerkin assembly; the V-cycle and flexible CG by convergence and the
- * symmetry of the preconditioner.  The composition of weights, overlap and
- * masks inside h1mg_solve has no independent pin beyond these (parity of that
- * composition is unpinned against the reference, whose build needs gslib).
+ * Galerkin assembly; swap_lengths/plane_space by their definition evaluated
+ * on each element's GLL planes and the box lattice; the whole of h1mg_solve
+ * (weights, overlap exchange, masks, level transfers, coarse solve, dsavg)
+ * by a dense model built from the definition of each piece on boxes with
+ * rotated element frames (tests/dense_mg.py); hmh_flex_cg and hmh_gmres
+ * iterate by iterate against dense restatements preconditioned by that
+ * model (tests/dense_krylov.py).  Mistakes planted through nko_set_mutation
+ * make those comparisons fail.
  */
 #include <math.h>
 #include <stdint.h>
@@ -224,7 +228,7 @@ static int64_t npow3(int n) { return (int64_t)n * n * n; }
    plane_space weights w(j) read unshifted, 32 neighbour lengths replaced by
    the element's own, 64 each Schwarz weight applied once (not as many times
    as the weight loops visit the node). */
-static int g_mutation = 0;
+int g_mutation = 0;
 void nko_set_mutation(int m) { g_mutation = m; }
 
 /* hsmg.f:2272-2323 H1MG_SETUP_MG_NX */
@@ -953,7 +957,7 @@ int nko_flex_cg(const nko_field *f, nko_mg *mg, double *res, const double *h1,
     for (int64_t i = 0; i < n; i++) r1[i] = r1[i] - r[i];
     double rho0 = rho1;
     rho1 = glsc3(z, wt, r, n);
-    double rho2 = -glsc3(z, wt, r1, n);
+    double rho2 = (g_mutation & 256) ? glsc3(z, wt, r1, n) : -glsc3(z, wt, r1, n);
     double beta = rho2 / rho0;
     memcpy(r1, r, sizeof(double) * n);
     for (int64_t i = 0; i < n; i++) p[i] = beta * p[i] + z[i];
@@ -1036,7 +1040,7 @@ int nko_gmres(const nko_field *f, nko_mg *mg, double *res, const double *h1, con
         const double hij = HH(i - 1, j - 1);
         for (int64_t q = 0; q < n; q++) w[q] = w[q] + (-hij) * vi[q];
       }
-      for (int i = 1; i <= j - 1; i++) { /* previous Givens rotations */
+      for (int i = 1; i <= ((g_mutation & 512) ? 0 : j - 1); i++) { /* previous Givens rotations */
         temp = HH(i - 1, j - 1);
         HH(i - 1, j - 1) = c[i - 1] * temp + sn[i - 1] * HH(i, j - 1);
         HH(i, j - 1) = -sn[i - 1] * temp + c[i - 1] * HH(i, j - 1);

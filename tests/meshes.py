@@ -63,6 +63,7 @@ class RotatedBox(_HexMesh):
     def __init__(self, box: BoxMesh, seed=0, rotate=True, relabel=True):
         assert not box.periodic_x, "periodic boxes are not rotated here"
         self.box = box
+        self.periodic_x = False
         self.lx1 = box.lx1
         self.nel = box.nel
         self.h = box.h

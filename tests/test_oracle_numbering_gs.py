@@ -26,13 +26,24 @@ CASES = [
     dict(nel=(3, 3, 3), lx1=8),
     dict(nel=(2, 2, 3), lx1=3),
     dict(nel=(2, 2, 2), lx1=2),
+    # every element's frame turned by a seeded rotation of the cube and the
+    # vertex ids permuted: the smallest corner of a face lands anywhere, so
+    # every orientation branch of the edge and face numbering is taken
+    # (navier8.f:2113-2312, including ind(1) = 3, 4 at :2226-2238)
+    dict(nel=(3, 3, 2), lx1=5, rotate=1),
+    dict(nel=(3, 3, 3), lx1=6, rotate=2, jitter=0.1),
+    dict(nel=(2, 3, 2), lx1=4, rotate=3, relabel_only=True),
 ]
 
 
 def make(case):
     nel = case["nel"]
-    kw = {k: v for k, v in case.items() if k not in ("nel", "lx1")}
-    return BoxMesh(nel[0], nel[1], nel[2], case["lx1"], **kw)
+    kw = {k: v for k, v in case.items() if k not in ("nel", "lx1", "rotate", "relabel_only")}
+    b = BoxMesh(nel[0], nel[1], nel[2], case["lx1"], **kw)
+    if "rotate" in case:
+        from meshes import RotatedBox
+        b = RotatedBox(b, seed=case["rotate"], rotate=not case.get("relabel_only", False))
+    return b
 
 
 def key_points(b):
@@ -75,9 +86,19 @@ def test_product_numbering_bit_exact(case):
     assert np.array_equal(g1, g2)
 
 
-@pytest.mark.parametrize("case", [c for c in CASES if c["lx1"] > 2])
+def _re2(name, lx1):
+    import os
+    from nek5000_amd.mesh import Re2Mesh
+    return Re2Mesh(os.path.join(os.path.dirname(__file__), "golden", "re2", name), lx1)
+
+
+@pytest.mark.parametrize("case", [c for c in CASES if c["lx1"] > 2]
+                         + [dict(re2="ethier.re2", lx1=5), dict(re2="lpm_one_uniform.re2",
+                                                                 lx1=4)])
 def test_counting_rules(case):
-    b = make(case)
+    """(On the reference's own short-test meshes too: their element frames
+    and vertex ids are whatever genbox/prenek wrote.)"""
+    b = _re2(case["re2"], case["lx1"]) if "re2" in case else make(case)
     n = b.lx1
     glo, ngv = O.setvert3d(b.vertex, n)
     loc = glo.reshape(b.nel, n, n, n)   # [e, k, j, i]
