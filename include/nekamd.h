@@ -302,6 +302,21 @@ int nkd_cggo(int mesh, double *x, const double *f, const double *h1, const doubl
              const double *mask, const double *mult, double tin, int maxit,
              const nkh_params *prm, nkh_result *res, double *history);
 
+/* nkd_cggo without any device allocation: no counterpart in the reference.
+   For callers that cannot allocate (code under HIP graph capture, tight
+   serving loops, harnesses that forbid hipMalloc).  The same FP64 Jacobi PCG
+   (same kernels, bit-identical results) with the solve's vectors, partial
+   sums, state and gs group flags in a static device array of the library
+   (one per GPU, held until the process ends) instead of the mesh's lazily
+   allocated work space; the array holds 6 vectors of 2^19 doubles plus
+   partial sums, state and flags (about 26 MB).  Meshes of at most 2^19 =
+   524,288 local nodes (e.g. 1,024 elements at lx1 = 8) only, else
+   NKH_ERR_ARG; prm->precision must be 0.  Calls on one GPU share the array:
+   calls that overlap on one GPU race. */
+int nkd_cggo_noalloc(int mesh, double *x, const double *f, const double *h1, const double *h2,
+                     const double *mask, const double *mult, double tin, int maxit,
+                     const nkh_params *prm, nkh_result *res, double *history);
+
 /* ---- timers ----------------------------------------------------------- */
 
 /* Mirror of Nek's CTIMER counters (hmholtz.f:112-113 naxhm/taxhm, :26-31

@@ -97,6 +97,9 @@ SIGNATURES = {
     "nkd_hmholtz_batch": (c_int, [c_int, c_void_p, c_int, P(NkhParams), c_void_p, P(c_int)]),
     "nkd_cggo": (c_int, [c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
                          c_double, c_int, P(NkhParams), P(NkhResult), c_void_p]),
+    "nkd_cggo_noalloc": (c_int, [c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
+                                 c_void_p, c_double, c_int, P(NkhParams), P(NkhResult),
+                                 c_void_p]),
     "nkh_mg_setup": (c_int, [c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]),
     "nkd_mg_solve": (c_int, [c_int, c_void_p, c_void_p]),
     "nkd_hmh_flex_cg": (c_int, [c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_int,

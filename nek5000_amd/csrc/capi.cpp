@@ -497,6 +497,8 @@ int nkh_init(int device) {
            count);
   NK_HIP(hipSetDevice(device));
   NK_HIP(hipStreamCreateWithFlags(&g_lib_stream, hipStreamNonBlocking));
+  // device address of the allocation-free solve's static scratch
+  NK_TRY(noalloc_scratch(nullptr));
   g_stream = g_lib_stream;
   g_device = device;
   g_init = true;
@@ -830,6 +832,20 @@ int nkd_cggo(int id, double *x, const double *f, const double *h1, const double 
   Coef c1, c2;
   NK_TRY(classify_coefs(*m, h1, h2, &c1, &c2, g_stream));
   return run_cggo(*m, x, f, c1, c2, mask, mult, tin, maxit, p, res, history);
+}
+
+int nkd_cggo_noalloc(int id, double *x, const double *f, const double *h1, const double *h2,
+                     const double *mask, const double *mult, double tin, int maxit,
+                     const nkh_params *prm, nkh_result *res, double *history) {
+  Mesh *m = get_mesh(id);
+  NK_TRY(ready(m, id, true));
+  NK_CHECK(x && f && h1 && h2 && mask && mult, NKH_ERR_ARG, "nkd_cggo_noalloc: null array");
+  NK_CHECK(maxit >= 0, NKH_ERR_ARG, "nkd_cggo_noalloc: maxit=%d", maxit);
+  const nkh_params &p = prm ? *prm : kDefaultParams;
+  NK_CHECK(p.precision == 0, NKH_ERR_UNSUPPORTED,
+           "nkd_cggo_noalloc: precision=%d (the allocation-free solve is FP64 only)",
+           p.precision);
+  return run_cggo_noalloc(*m, x, f, h1, h2, mask, mult, tin, maxit, p, res, history);
 }
 
 int nkd_hmholtz(int id, const char *name, double *u, double *rhs, const double *h1,

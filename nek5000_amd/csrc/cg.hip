@@ -986,7 +986,7 @@ namespace {
 template <class Run>
 int drive_cggo(Mesh &m, double *x, const double *f, Coef h1, Coef h2, const double *mask,
                const double *mult, double tin, int maxit, const nkh_params &prm,
-               nkh_result *res, double *history_host) {
+               nkh_result *res, double *history_host, bool allow_timers = true) {
   hipStream_t s = cur_stream();
   NK_TRY(ensure_workspace(m));
   Run run;
@@ -999,7 +999,7 @@ int drive_cggo(Mesh &m, double *x, const double *f, Coef h1, Coef h2, const doub
   CgState hs{};
   int check_every = 1;
   // CTIMER mirror: event pairs around axhelm and gs of every launched iteration
-  const bool timed = timers().on;
+  const bool timed = allow_timers && timers().on;
   const int niter = run.niter;
   std::vector<hipEvent_t> ev;
   if (timed) {
@@ -1059,6 +1059,71 @@ int run_cggo(Mesh &m, double *x, const double *f, Coef h1, Coef h2, const double
   if (prm.precision == 2)
     return drive_cggo<McgRun>(m, x, f, h1, h2, mask, mult, tin, maxit, prm, res, history_host);
   return drive_cggo<CgRun>(m, x, f, h1, h2, mask, mult, tin, maxit, prm, res, history_host);
+}
+
+// ---------------------------------------------------------------------------
+// Allocation-free FP64 Jacobi PCG (nkd_cggo_noalloc), for callers that may
+// not allocate device memory.  The solve's vectors (r, w, p, d, d*mult,
+// bm), its partial sums and history, its device state and the gs group
+// flags of its mask/mult live in `scratch`, one file-scope device array per
+// GPU, sized for meshes of at most kNoAllocMaxN local nodes.  Host code
+// reaches the array only through the address hipGetSymbolAddress returns
+// for the current device (noalloc_scratch).  Every call on a GPU uses the
+// same array: calls that overlap on one GPU race.  Nothing in it needs to
+// start at zero (each solve writes its state, partial sums, flags and
+// vectors before reading them); the last call's contents stay behind.
+// ---------------------------------------------------------------------------
+constexpr size_t kNoAllocWs = 6 * (size_t)kNoAllocMaxN;                         // doubles
+constexpr size_t kNoAllocRed = ((size_t)kRedSize + 31) / 32 * 32;               // doubles
+constexpr size_t kNoAllocState = (sizeof(McgState) + 255) / 256 * 32;           // doubles
+constexpr size_t kNoAllocFlag = ((size_t)kNoAllocMaxN + 255) / 256 * 32;        // doubles
+constexpr size_t kNoAllocDoubles = kNoAllocWs + kNoAllocRed + kNoAllocState + kNoAllocFlag;
+static_assert(kNoAllocMaxN < ((int64_t)1 << 20),
+              "the axhelm autotuner (which allocates) runs on meshes of 2^20 nodes and up");
+
+static __device__ __align__(256) double scratch[kNoAllocDoubles];
+
+namespace {
+constexpr int kNoAllocDevices = 64;
+double *g_scratch_dev[kNoAllocDevices] = {};
+}  // namespace
+
+int noalloc_scratch(double **out) {
+  int dev = 0;
+  NK_HIP(hipGetDevice(&dev));
+  NK_CHECK(dev >= 0 && dev < kNoAllocDevices, NKH_ERR_ARG, "noalloc scratch: device %d", dev);
+  if (!g_scratch_dev[dev]) {
+    void *p = nullptr;
+    NK_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    NK_CHECK(p != nullptr, NKH_ERR_HIP, "noalloc scratch: no device address");
+    g_scratch_dev[dev] = static_cast<double *>(p);
+  }
+  if (out) *out = g_scratch_dev[dev];
+  return NKH_OK;
+}
+
+int run_cggo_noalloc(Mesh &m, double *x, const double *f, const double *h1, const double *h2,
+                     const double *mask, const double *mult, double tin, int maxit,
+                     const nkh_params &prm, nkh_result *res, double *history_host) {
+  NK_CHECK(m.n <= kNoAllocMaxN, NKH_ERR_ARG,
+           "nkd_cggo_noalloc: %lld local nodes exceed its static scratch (at most %lld)",
+           (long long)m.n, (long long)kNoAllocMaxN);
+  NK_CHECK(m.gs.ngroups <= kNoAllocMaxN, NKH_ERR_ARG, "nkd_cggo_noalloc: %lld gs groups",
+           (long long)m.gs.ngroups);
+  double *sp = nullptr;
+  NK_TRY(noalloc_scratch(&sp));
+  // a view of the mesh whose solve buffers are the scratch: the geometry
+  // and gather-scatter map are the mesh's own, nothing of it is changed
+  Mesh v = m;
+  v.ws = sp;
+  v.ws_doubles = kNoAllocWs;
+  v.red = sp + kNoAllocWs;
+  v.st = reinterpret_cast<CgState *>(sp + kNoAllocWs + kNoAllocRed);
+  v.gs.flag = reinterpret_cast<uint8_t *>(sp + kNoAllocWs + kNoAllocRed + kNoAllocState);
+  Coef c1, c2;
+  NK_TRY(classify_coefs(v, h1, h2, &c1, &c2, cur_stream()));
+  return drive_cggo<CgRun>(v, x, f, c1, c2, mask, mult, tin, maxit, prm, res, history_host,
+                           false);
 }
 
 // ---------------------------------------------------------------------------

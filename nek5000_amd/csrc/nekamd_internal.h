@@ -171,6 +171,13 @@ hipStream_t cur_stream();
 int ensure_workspace(Mesh &m);  // solve vectors + state + partial sums
 size_t cg_ws_doubles(int64_t n);  // work space of one Jacobi-PCG solve (doubles)
 int ensure_small(Mesh &m);      // state + partial sums only
+// Allocation-free Jacobi PCG: its buffers are a static device array sized for
+// at most kNoAllocMaxN local nodes (cg.hip).
+constexpr int64_t kNoAllocMaxN = (int64_t)1 << 19;
+int noalloc_scratch(double **out);  // this device's address of the array
+int run_cggo_noalloc(Mesh &m, double *x, const double *f, const double *h1, const double *h2,
+                     const double *mask, const double *mult, double tin, int maxit,
+                     const nkh_params &prm, nkh_result *res, double *history_host);
 
 // ---- kernels.hip ----
 // Coefficient description: field pointer or uniform scalar.

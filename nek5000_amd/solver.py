@@ -230,16 +230,19 @@ class Field:
                               hist.reshape(900, 5)[: max(res.niter + 1, 1)].copy())
         return out
 
-    def cggo(self, x, f, h1, h2, mask, mult, tin, maxit, params=None, history=False):
+    def cggo(self, x, f, h1, h2, mask, mult, tin, maxit, params=None, history=False,
+             noalloc=False):
+        """hmholtz.f:611 cggo; noalloc selects nkd_cggo_noalloc (the same solve
+        with its buffers in the library's static device array)."""
         prm = params if params is not None else default_params()
         res = NkhResult()
         hist = np.zeros(5 * 900) if history else None
         _sync_stream(self.lib)
-        check(self.lib.nkd_cggo(self.handle, _ptr(x), _ptr(f), _ptr(h1), _ptr(h2), _ptr(mask),
-                                _ptr(mult), float(tin), int(maxit), ctypes.byref(prm),
-                                ctypes.byref(res),
-                                ctypes.c_void_p(hist.ctypes.data) if history else None),
-              "nkd_cggo")
+        fn, name = ((self.lib.nkd_cggo_noalloc, "nkd_cggo_noalloc") if noalloc else
+                    (self.lib.nkd_cggo, "nkd_cggo"))
+        check(fn(self.handle, _ptr(x), _ptr(f), _ptr(h1), _ptr(h2), _ptr(mask), _ptr(mult),
+                 float(tin), int(maxit), ctypes.byref(prm), ctypes.byref(res),
+                 ctypes.c_void_p(hist.ctypes.data) if history else None), name)
         out = dict(niter=res.niter, converged=res.converged, rbn0=res.rbn0, rbn2=res.rbn2,
                    tol=res.tol)
         if history:
