@@ -139,6 +139,15 @@ int nkd_mesh_geom_from_xyz(int mesh, const double *xm1, const double *ym1, const
 /* Upload Nek's own /GMFACT/ g1m1..g6m1 and /MASS/ bm1 (host arrays). */
 int nkh_mesh_set_geom(int mesh, const double *g1, const double *g2, const double *g3,
                       const double *g4, const double *g5, const double *g6, const double *bm1);
+/* genxyz.f:351-470 SETDEF's IFDFRM for runs with param(59) = 0 (nel ints,
+   nonzero = deformed).  The reference's axhelm and setprec skip the cross
+   terms g4..g6 of an undeformed element (hmholtz.f:188-218: the fast path
+   and `if (ifdfrm(e))`; :440-468), and sfastax (hmholtz.f:353-364) has
+   overwritten its g4m1..g6m1 with g1/w..g3/w; the library then holds them
+   at zero, now and after later geometry uploads.  NULL (what param(59) != 0
+   and h1mg_setup's geom_reset give) treats every element as deformed again
+   for later uploads. */
+int nkh_mesh_set_deformed(int mesh, const int *ifdfrm);
 /* dssum.f:1-31 SETUPDS (fgslib_gs_setup) from Nek's glo_num (host), then
    coef.f:1500-1543 SETINVM (binvm1 = 1/dssum(bm1)) and volvm1 = sum(bm1). */
 int nkh_mesh_gs_setup(int mesh, const int64_t *glo_num);

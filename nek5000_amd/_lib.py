@@ -76,6 +76,7 @@ SIGNATURES = {
     "nkh_mesh_geom_from_xyz": (c_int, [c_int, c_void_p, c_void_p, c_void_p]),
     "nkd_mesh_geom_from_xyz": (c_int, [c_int, c_void_p, c_void_p, c_void_p]),
     "nkh_mesh_set_geom": (c_int, [c_int] + [c_void_p] * 7),
+    "nkh_mesh_set_deformed": (c_int, [c_int, c_void_p]),
     "nkh_mesh_gs_setup": (c_int, [c_int, c_void_p]),
     "nkh_mesh_info": (c_int, [c_int, P(c_int64), P(c_double), P(c_int64), P(c_int64), P(c_int)]),
     "nkh_mesh_ptrs": (c_int, [c_int, P(c_void_p), P(c_void_p)]),

@@ -509,7 +509,8 @@ static void mesh_release(Mesh *m) {
   mg_free(*m);
   gs_free(m->gs);
   for (void *p : {(void *)m->Dd, (void *)m->g, (void *)m->bm1, (void *)m->binv, (void *)m->ws,
-                  (void *)m->st, (void *)m->red, (void *)m->g32, (void *)m->bm1_32})
+                  (void *)m->st, (void *)m->red, (void *)m->g32, (void *)m->bm1_32,
+                  (void *)m->undeformed})
     if (p) (void)hipFree(p);
   delete m;
 }
@@ -636,6 +637,7 @@ int nkd_mesh_geom_from_xyz(int id, const double *x, const double *y, const doubl
   NK_CHECK(m && x && y && z, NKH_ERR_ARG, "nkd_mesh_geom_from_xyz: bad args");
   NK_TRY(alloc_geom(*m));
   NK_TRY(launch_geom(*m, x, y, z, g_stream));
+  NK_TRY(apply_undeformed(*m, g_stream));
   NK_HIP(hipStreamSynchronize(g_stream));
   m->has_geom = true;
   m->geom_version += 1;
@@ -672,9 +674,39 @@ int nkh_mesh_set_geom(int id, const double *g1, const double *g2, const double *
     NK_HIP(hipMemcpy(m->g + (size_t)f * m->n, src[f], sizeof(double) * m->n,
                      hipMemcpyHostToDevice));
   NK_HIP(hipMemcpy(m->bm1, bm1, sizeof(double) * m->n, hipMemcpyHostToDevice));
+  NK_TRY(apply_undeformed(*m, g_stream));
+  NK_HIP(hipStreamSynchronize(g_stream));
   m->has_geom = true;
   m->geom_version += 1;
   return finish_mass(*m);
+}
+
+int nkh_mesh_set_deformed(int id, const int *ifdfrm) {
+  Mesh *m = get_mesh(id);
+  NK_CHECK(m, NKH_ERR_ARG, "nkh_mesh_set_deformed: bad mesh %d", id);
+  if (!ifdfrm) {
+    if (m->undeformed) {
+      NK_HIP(hipStreamSynchronize(g_stream));
+      (void)hipFree(m->undeformed);
+      m->undeformed = nullptr;
+    }
+    return NKH_OK;  // factors already uploaded keep their cross terms
+  }
+  std::vector<uint8_t> f((size_t)m->nel);
+  bool any = false;
+  for (int64_t e = 0; e < m->nel; ++e) {
+    f[(size_t)e] = ifdfrm[e] == 0 ? 1 : 0;
+    any = any || f[(size_t)e];
+  }
+  if (!any) return nkh_mesh_set_deformed(id, nullptr);
+  if (!m->undeformed) NK_HIP(hipMalloc(&m->undeformed, (size_t)m->nel));
+  NK_HIP(hipMemcpy(m->undeformed, f.data(), (size_t)m->nel, hipMemcpyHostToDevice));
+  if (m->has_geom) {
+    NK_TRY(apply_undeformed(*m, g_stream));
+    NK_HIP(hipStreamSynchronize(g_stream));
+    m->geom_version += 1;
+  }
+  return NKH_OK;
 }
 
 int nkh_mesh_gs_setup(int id, const int64_t *glo_num) {

@@ -168,4 +168,30 @@ int launch_setprec(const Mesh &m, double *d, Coef h1, Coef h2, hipStream_t s) {
 #undef CALL
 }
 
+// Undeformed elements (genxyz.f:351-470 setdef; param(59) = 0): the
+// reference's axhelm and setprec skip their cross terms g4..g6
+// (hmholtz.f:188-218 fast path, `if (ifdfrm(e))`; :440-468), so they are
+// held as zeros here, whatever sfastax (hmholtz.f:353-364) left in Nek's
+// g4m1..g6m1.
+__global__ void k_zero_cross(double *__restrict__ g, const uint8_t *__restrict__ undeformed,
+                             int64_t npe, int64_t nel, int64_t n) {
+  const int64_t total = 3 * n;
+  for (int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; q < total;
+       q += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t l = q % n;
+    if (undeformed[l / npe]) g[3 * n + q] = 0.0;
+  }
+}
+
+int apply_undeformed(Mesh &m, hipStream_t s) {
+  if (!m.undeformed || !m.g) return NKH_OK;
+  const int64_t npe = (int64_t)m.np * m.np * m.np;
+  int64_t grid = (3 * m.n + 255) / 256;
+  if (grid > 65536) grid = 65536;
+  hipLaunchKernelGGL(k_zero_cross, dim3((int)grid), dim3(256), 0, s, m.g, m.undeformed, npe,
+                     m.nel, m.n);
+  NK_HIP(hipGetLastError());
+  return NKH_OK;
+}
+
 }  // namespace nekamd

@@ -122,6 +122,9 @@ struct Mesh {
   // first use, rebuilt when geom_version moves)
   float *g32 = nullptr, *bm1_32 = nullptr;
   int g32_version = -1;
+  // device, nel flags: 1 where the element is undeformed (IFDFRM false),
+  // whose cross terms g4..g6 are then held at zero; null: all deformed
+  uint8_t *undeformed = nullptr;
 };
 
 // CTIMER-style accumulators (seconds, calls), see nkh_timers_*.
@@ -210,6 +213,8 @@ int launch_axhelm32_jac(const Mesh &m, float *w, float *p, const float *r, const
                         hipStream_t s);
 int launch_geom(Mesh &m, const double *x, const double *y, const double *z, hipStream_t s);
 int launch_setprec(const Mesh &m, double *d, Coef h1, Coef h2, hipStream_t s);
+// zero g4..g6 of the elements flagged undeformed (no-op without flags)
+int apply_undeformed(Mesh &m, hipStream_t s);
 // axhelm fused into the CG iteration: p = d*r + beta*p; w = A p (h1/h2 as
 // above), rho partial over element-interior nodes when rho_part != nullptr.
 int launch_axhelm_cg(const Mesh &m, double *w, double *p, const double *r, const double *d,

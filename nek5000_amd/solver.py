@@ -159,6 +159,17 @@ class Field:
         check(self.lib.nkh_mesh_set_geom(self.handle, *[a.ctypes.data for a in arrs]),
               "nkh_mesh_set_geom")
 
+    def set_deformed(self, ifdfrm):
+        """genxyz.f setdef's IFDFRM (nel flags, 0 = undeformed: axhelm and
+        setprec drop that element's cross terms), or None for all deformed."""
+        if ifdfrm is None:
+            check(self.lib.nkh_mesh_set_deformed(self.handle, None), "nkh_mesh_set_deformed")
+            return
+        f = np.ascontiguousarray(ifdfrm, dtype=np.int32).reshape(-1)
+        assert f.size == self.nel
+        check(self.lib.nkh_mesh_set_deformed(self.handle, f.ctypes.data),
+              "nkh_mesh_set_deformed")
+
     def gs_setup(self, glo_num):
         glo = np.ascontiguousarray(glo_num, dtype=np.int64).reshape(-1)
         assert glo.size == self.n

@@ -232,3 +232,37 @@ def test_cggo_unfused_path(gpu):
     say(f"[cggo-unfused] oracle={n_o} gpu={out['niter']}")
     assert abs(out["niter"] - n_o) <= 1
     assert l2rel(host(x), x_o) < 1e-9
+
+
+def test_undeformed_elements_drop_cross_terms(gpu):
+    """param(59) = 0: elements flagged undeformed (genxyz.f setdef) keep the
+    g4m1..g6m1 that sfastax (hmholtz.f:353-364) wrote -- g1/w_x, g2/w_y,
+    g3/w_z -- and the reference's axhelm and setprec skip their cross terms.
+    On a box of rectangular elements the true cross terms are zero, so the
+    apply and the Jacobi diagonal equal the oracle's on the true factors;
+    half the elements are flagged (the others carry the true factors)."""
+    b = box((3, 2, 2), 8)
+    P = Problem(b)
+    F = gpu.Field(b.lx1, b.nel)
+    w = P.w
+    flags = np.arange(b.nel) % 2  # 0 = undeformed
+    g = [a.copy() for a in P.g]
+    und = flags == 0
+    g[3][und] = (P.g[0] / w[None, None, None, :])[und]
+    g[4][und] = (P.g[1] / w[None, None, :, None])[und]
+    g[5][und] = (P.g[2] / w[None, :, None, None])[und]
+    F.set_geom(g, P.bm1)
+    F.set_deformed(flags)
+    F.gs_setup(P.glo)
+    rng = np.random.default_rng(4)
+    u = rng.standard_normal(P.shape)
+    h1 = 0.5 + rng.random(P.shape)
+    h2 = np.full(P.shape, 2.0)
+    say("[undeformed] axhelm and setprec with sfastax factors on flagged elements")
+    au = F.zeros()
+    F.axhelm(au, F.to_dev(u), F.to_dev(h1), F.to_dev(h2))
+    assert rel(host(au), P.field.axhelm(u, h1, h2)) < 1e-13
+    d = F.zeros()
+    F.setprec(d, F.to_dev(h1), F.to_dev(h2))
+    assert rel(host(d), P.field.setprec(h1, h2)) < 1e-13
+    F.destroy()
