@@ -10,10 +10,15 @@ param(42) = 2), or with --solver jacobi by hmholtz -> Jacobi-PCG cggo.  One
 step = one full solve with all inputs resident in HBM (set-up excluded).
 
 value = (unique free DOF x CG iterations) / wall time, in GDOF*iter/s.
-roofline: the axhelm of the iteration with the search-direction update fused
-in, average duration from the library's HIP-event timers around it.
+roofline: the kernel of the iteration that takes longest (from the library's
+HIP-event timers on the stream it runs on, over the timed solves): the
+fine-level local FDM solves, the flexible-CG axhelm or the assembly, with
+its algorithmic bytes per launch; "kernels" lists all three.
 cpu_baseline: the oracle (C, OpenMP over all host threads) solving the same
 operator with the same solver on a bounded sample (E = 16^3).
+north_star: the mixed-precision solve's time to solution against FP64's,
+the five-solve batch (mixed, batched) against five FP64 solves one after
+another, and the adjoint gradient's finite-difference check.
 """
 import argparse
 import json
@@ -54,6 +59,29 @@ def fused_axhelm_bytes_per_node(lx1, h1_field=False, h2_kind="uniform"):
     if h1_field:
         words += 1
     return 8.0 * words
+
+
+def fdm_bytes_per_element(nh):
+    """Algorithmic HBM bytes per element of the fine-level local FDM solves
+    (k_sch_fdm): reads r (nh^3), the neighbours' first interior layers
+    (6 nh^2, gathered), the element's three eigenvector matrices (3 (nh+2)^2)
+    and eigenvalues (3 (nh+2) + 1), writes the local solution's interior
+    (nh^3) and its ghost faces (6 nh^2), all 8-byte words; plus the two
+    int32 ghost maps (gin, gout: 6 nh^2 each)."""
+    nl = nh + 2
+    words = 2 * nh ** 3 + 12 * nh ** 2 + 3 * nl ** 2 + 3 * nl + 1
+    return 8.0 * words + 4.0 * 12 * nh ** 2
+
+
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as fh:
+            for line in fh:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return None
 
 
 def measured_traffic(name):
@@ -105,7 +133,7 @@ def cpu_baseline(nel1d, lx1, deform, target_seconds, solver="hsmg"):
         el = time.perf_counter() - t
         what = f"hmholtz/cggo Jacobi-PCG, {niter} iterations"
     return dict(value=free * niter / el / 1e9, unit="GDOF*iter/s", cores=O.num_threads(),
-                kind="port",
+                kind="port", cpu=cpu_model(),
                 sample=(f"oracle (C restatement, OpenMP) on E={nel1d}^3 N={lx1-1} deformed, "
                         f"h1=1 h2=10: {what} ({free:.0f} free DOF) in {el:.1f} s"))
 
@@ -279,7 +307,8 @@ def adjoint_run(nel1d=32, lx1=8):
                 fd_directions=len(checks), fd_step=1e-6,
                 fd_max_rel_diff=max(c["rel_diff"] for c in checks), fd_seconds=fd_s,
                 finite_difference_checks=checks)
-    print(json.dumps(line), flush=True)
+    F.destroy()
+    return line
 
 
 def batch_run(steps=2):
@@ -366,8 +395,10 @@ def batch_run(steps=2):
                                       "periodic, N=7, h2=bd(1)/dt=11/(6e-3) for all, Jacobi PCG "
                                       "to 1e-8 relative (maxcg 900)",
                 local_nodes=[int(F.n) for F in fields], free_dof=dofs, variants=rows)
-    print(json.dumps(line), flush=True)
     B.destroy()
+    for F in {id(F): F for F in fields}.values():
+        F.destroy()
+    return line
 
 
 def main():
@@ -414,10 +445,10 @@ def main():
         order_sweep()
         return
     if args.adjoint:
-        adjoint_run()
+        print(json.dumps(adjoint_run()), flush=True)
         return
     if args.batch:
-        batch_run()
+        print(json.dumps(batch_run()), flush=True)
         return
 
     import torch
@@ -514,7 +545,26 @@ def main():
     ax_bytes = bpn * box.n
     # (the mixed solver's FP32 operator records no axhelm events: no roofline
     # entry for a --precision 2 main line)
-    achieved = ax_bytes / t_ax / 1e9 if t_ax > 0 else None
+    # per-kernel table from the library's HIP-event timers over the timed
+    # solves, each with its algorithmic bytes per launch
+    fb = 1.0 - ((args.lx1 - 2) / args.lx1) ** 3  # element-boundary fraction of the nodes
+    kernels = [dict(kernel=ax_kernel, ms=t_ax * 1e3, bytes_per_launch=ax_bytes),
+               dict(kernel="k_gs_assemble<double> (w = mask dssum(w))", ms=t_gs * 1e3,
+                    bytes_per_launch=box.n * fb * (8.0 * 2 + 4.0))]
+    if hsmg:
+        lib.nkh_timers_read(b"fdm", ctypes.byref(sec), ctypes.byref(calls))
+        t_fdm = sec.value / max(calls.value, 1)
+        lib.nkh_timers_read(b"mgv", ctypes.byref(sec), ctypes.byref(calls))
+        t_mgv = sec.value / max(calls.value, 1)
+        nh = args.lx1
+        kernels.append(dict(kernel=f"k_sch_fdm<{nh + 2},double> (fine-level local FDM solves)",
+                            ms=t_fdm * 1e3,
+                            bytes_per_launch=fdm_bytes_per_element(nh) * box.nel))
+    for k in kernels:
+        k["GBs"] = k["bytes_per_launch"] / (k["ms"] * 1e-3) / 1e9 if k["ms"] > 0 else None
+        k["hbm_frac"] = k["GBs"] / HBM_PEAK_GBS if k["GBs"] else None
+    dom = max(kernels, key=lambda k: k["ms"])
+    achieved = dom["GBs"]
     total_iters = sum(iters)
     value = free_dof * total_iters / elapsed / 1e9
     mg_info = F.h1mg_info() if hsmg else None
@@ -594,8 +644,43 @@ def main():
         del arrs, u_h
 
     traffic = None
-    if hsmg and args.nel == 64 and args.lx1 == 8:
-        traffic = measured_traffic("axhelm_flexcg_E262144_N7_pmc.md")
+    if hsmg and args.nel == 64 and args.lx1 == 8 and args.precision == 0:
+        traffic = measured_traffic("sch_fdm_E262144_N7_pmc.md" if "sch_fdm" in dom["kernel"]
+                                   else "axhelm_flexcg_E262144_N7_pmc.md")
+
+    north = None
+    if not args.no_variants and hsmg and args.precision == 0:
+        # the comparisons the north star states, recorded with the bench line
+        mixed = [v for v in variants if v["variant"].startswith("mixed: FP32 inner flexible CG")]
+        F.destroy()
+        del rhs0, rhss, h1, h2, bm1
+        torch.cuda.empty_cache()
+        log("[bench] north star: batch ...")
+        bl = batch_run()
+        bv = bl["variants"]
+        log("[bench] north star: adjoint ...")
+        al = adjoint_run()
+        north = {
+            "mixed_time_to_solution_vs_fp64": mixed[0]["time_to_solution_vs_fp64"] if mixed else None,
+            "mixed_value": mixed[0]["value"] if mixed else None,
+            "mixed_final_rel_residual": mixed[0]["final_rel_residual"] if mixed else None,
+            "target_mixed_ratio": 1.6,
+            "batch": {"workload": bl["workload"],
+                      "sequential_fp64_s": bv["sequential_fp64"]["seconds"],
+                      "batched_fp64_s": bv["batched_fp64"]["seconds"],
+                      "batched_mixed_s": bv["batched_mixed"]["seconds"],
+                      "batched_mixed_vs_sequential_fp64":
+                          bv["batched_mixed"]["speedup_vs_sequential_fp64"],
+                      "batched_mixed_GDOF_iter_s": bv["batched_mixed"]["GDOF_iter_s"],
+                      "sequential_fp64_GDOF_iter_s": bv["sequential_fp64"]["GDOF_iter_s"],
+                      "batched_mixed_max_rel_diff_vs_fp64":
+                          bv["batched_mixed"]["max_rel_diff_vs_fp64"],
+                      "converged": all(all(r["converged"]) for r in bv.values())},
+            "adjoint": {k: al[k] for k in ("workload", "fd_directions", "fd_step",
+                                           "fd_max_rel_diff", "forward_ms", "adjoint_ms",
+                                           "gradient_kernel_ms", "gradient_kernel_GBs")},
+        }
+        log(f"[bench] north star {north}")
 
     cpu = None
     if not args.no_cpu:
@@ -634,6 +719,8 @@ def main():
             "seconds_per_solve": elapsed / args.steps,
             "free_dof": free_dof, "iterations_per_solve": iters,
             "axhelm_ms": t_ax * 1e3, "gs_ms": t_gs * 1e3,
+            "vcycle_ms": (t_mgv * 1e3 if hsmg else None),
+            "kernels": kernels,
             "ms_per_iteration": elapsed / total_iters * 1e3,
             "copy_GBs": copy_gbs,
             "host_arrays": host_io,
@@ -641,10 +728,11 @@ def main():
         "roofline": {
             "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS if achieved else None, "traffic": traffic,
-            "kernel": ax_kernel,
-            "bytes_per_launch": ax_bytes,
+            "kernel": dom["kernel"], "ms_per_launch": dom["ms"],
+            "bytes_per_launch": dom["bytes_per_launch"],
         },
         "cpu_baseline": cpu,
+        "north_star": north,
     }
     print(json.dumps(line), flush=True)
 

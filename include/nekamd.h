@@ -331,7 +331,9 @@ int nkd_cggo_noalloc(int mesh, double *x, const double *f, const double *h1, con
 /* Mirror of Nek's CTIMER counters (hmholtz.f:112-113 naxhm/taxhm, :26-31
    nhmhz/thmhz, dssum.f:52-95 tdsum).  When enabled, solves record HIP events
    around the axhelm and gather-scatter launches of each iteration that does
-   work, on the stream those kernels run on.  name: "axhm", "dsum", "hmhz". */
+   work, on the stream those kernels run on.  name: "axhm", "dsum", "hmhz";
+   in the flexible CG with h1mg also "fdm" (the fine-level local FDM solves
+   of each V-cycle) and "mgv" (each V-cycle, on the fine level's stream). */
 int nkh_timers_enable(int on);
 int nkh_timers_reset(void);
 int nkh_timers_read(const char *name, double *seconds, int64_t *calls);

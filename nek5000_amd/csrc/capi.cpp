@@ -1111,6 +1111,12 @@ int nkh_timers_read(const char *name, double *seconds, int64_t *calls) {
   } else if (std::strcmp(name, "hmhz") == 0) {
     s = t.thmhz;
     c = t.nhmhz;
+  } else if (std::strcmp(name, "fdm") == 0) {
+    s = t.tfdm;
+    c = t.nfdm;
+  } else if (std::strcmp(name, "mgv") == 0) {
+    s = t.tmgv;
+    c = t.nmgv;
   } else {
     set_error("nkh_timers_read: unknown timer '%s'", name);
     return NKH_ERR_ARG;

@@ -147,6 +147,7 @@ int run_gmres(Mesh &m, double *res, Coef h1, Coef h2, const double *wt, const do
   const double *mgmask = mg.lev[mg.lmax].mask;
   const dim3 G(kGmBlocks), B(256);
   const double norm_fac = 1.0 / std::sqrt(m.vol);
+  const bool nullsp = mg_null_space_on(mg, h2);
   // hmholtz.f-style tolerance (gmres.f:352-356)
   double tolps = 0.0;
   NK_TRY(run_chktcg1(m, &tolps, res, h1, h2, mask, wt));
@@ -188,6 +189,7 @@ int run_gmres(Mesh &m, double *res, Coef h1, Coef h2, const double *wt, const do
       double *vj = V + (int64_t)(j - 1) * n, *zj = Z + (int64_t)(j - 1) * n;
       hipLaunchKernelGGL(k_gm_copy_mask, dim3(mg_grid(n)), B, 0, s, w, vj, mgmask, n);
       NK_TRY(mg_solve(m, zj, w, s, prm.param40, prm.precision));
+      if (nullsp) NK_TRY(mg_ortho(mg, zj, n, s));  // gmres.f:427
       NK_TRY(launch_axhelm(m, w, zj, h1, h2, s));
       NK_TRY(gs_op(m.gs, w, 0, s));
       // classical Gram-Schmidt, one pass (gmres.f:464-474)
@@ -242,8 +244,9 @@ int run_gmres(Mesh &m, double *res, Coef h1, Coef h2, const double *wt, const do
     hipLaunchKernelGGL(k_gm_xupd, dim3(mg_grid(n)), B, 0, s, x, Z, dc, j, n);
     NK_HIP(hipGetLastError());
   }
-  // res = x (copy, gmres.f:535)
+  // res = x (copy, gmres.f:535), orthogonalised against the null space (:534)
   NK_HIP(hipMemcpyAsync(res, x, sizeof(double) * n, hipMemcpyDeviceToDevice, s));
+  if (nullsp) NK_TRY(mg_ortho(mg, res, n, s));
   NK_HIP(hipStreamSynchronize(s));
   if (out) {
     out->niter = iter;

@@ -58,8 +58,16 @@ struct Mg {
   hipStream_t sc = nullptr;  // third: intermediate-level Schwarz beside the coarse solve
   hipEvent_t ev_fork = nullptr, ev_join = nullptr;
   hipEvent_t ev_lr[4] = {}, ev_le[4] = {};  // per level: r ready, e ready
+  // timing events (nkh_timers): fine local solves start/end, V-cycle start/end
+  hipEvent_t tev[4] = {};
   // coarse problem (navier8.f:83-233): unknowns = free vertex ids
   int ncrs = 0;
+  // no Dirichlet face: the constant is in the null space of the pressure
+  // operator (the reference's ifvcor); the coarse unknown ncrs-1 is held at
+  // zero and the coarse solution's mean removed (crs_xxt.c:883-955), and the
+  // Krylov solvers apply ortho (navier1.f:223-255)
+  bool null_space = false;
+  double *opart = nullptr;  // ortho partial sums
   int32_t *slot = nullptr;                   // per level-1 node: coarse unknown or -1
   int32_t *goff = nullptr, *gidx = nullptr;  // unknown -> level-1 nodes (CSR)
   double *cb = nullptr, *cx = nullptr;       // coarse rhs and solution
@@ -84,7 +92,13 @@ constexpr int kGmresM = 30;  // SIZE lgmres: Krylov dimension before a restart
 
 int mg_grid(int64_t n);
 // allocate and fill the single-precision smoother data of the fine level
-int mg_prepare_fp32(Mg &mg, hipStream_t s);
+int mg_prepare_fp32(Mg &mg, hipStream_t s, bool inner);
+// navier1.f:223-255 ORTHO: v <- v - sum(v)/n over the n local nodes (device)
+int mg_ortho(Mg &mg, double *v, int64_t n, hipStream_t s);
+// the constant null space is present for this solve (no Dirichlet face, h2 = 0)
+inline bool mg_null_space_on(const Mg &mg, Coef h2) {
+  return mg.null_space && h2.uniform() && h2.s == 0.0;
+}
 int mg_dalloc(double **p, int64_t n);
 
 // Schwarz solve on level l of r (masked): writes the level's e (not yet

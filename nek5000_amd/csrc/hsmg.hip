@@ -141,31 +141,52 @@ __global__ void __launch_bounds__(Fdm<NL>::kThreads)
   for (int64_t ep = blockIdx.x; ep < nel; ep += gridDim.x) {
     const int64_t x8 = ep % 8;
     const int64_t e = x8 * q8 + (x8 < r8 ? x8 : r8) + ep / 8;
-    __syncthreads();
     const RI *re = r + e * N3;
     const int32_t *ge = gin + e * 6 * N2;
     const R *Se = S + e * 3 * M2;
+    // every load of the element issued before the first wait
+    constexpr int NR = (N3 + T - 1) / T, NS = (3 * M2 + T - 1) / T, NG = (6 * N2 + T - 1) / T;
+    R vr[NR], vs[NS], vg[NG], vl;
+    int32_t gi[NG], gd[NG];
+    const int32_t *go = gout + e * 6 * N2;
 #pragma unroll
-    for (int it = 0; it < (3 * M2 + T - 1) / T; ++it) {
-      const int q = tid + it * T;
-      if (q < 3 * M2) sS[q] = Se[q];
+    for (int it = 0; it < NG; ++it) {
+      const int q = tid + it * T, qc = q < 6 * N2 ? q : 0;
+      gi[it] = q < 6 * N2 ? ge[qc] : -1;
+      gd[it] = q < 6 * N2 ? go[qc] : -1;
     }
-    if (tid < NLAM) sl[tid] = lam[e * NLAM + tid];
 #pragma unroll
-    for (int it = 0; it < (N3 + T - 1) / T; ++it) {
+    for (int it = 0; it < NR; ++it) {
+      const int q = tid + it * T;
+      vr[it] = R(re[q < N3 ? q : N3 - 1]);
+    }
+#pragma unroll
+    for (int it = 0; it < NS; ++it) {
+      const int q = tid + it * T;
+      vs[it] = Se[q < 3 * M2 ? q : 3 * M2 - 1];
+    }
+    vl = lam[e * NLAM + (tid < NLAM ? tid : NLAM - 1)];
+#pragma unroll
+    for (int it = 0; it < NG; ++it) vg[it] = R(r[gi[it] >= 0 ? gi[it] : 0]);
+    __syncthreads();  // the previous element's LDS reads are done
+#pragma unroll
+    for (int it = 0; it < NS; ++it) {
+      const int q = tid + it * T;
+      if (q < 3 * M2) sS[q] = vs[it];
+    }
+    if (tid < NLAM) sl[tid] = vl;
+#pragma unroll
+    for (int it = 0; it < NR; ++it) {
       const int q = tid + it * T;
       if (q < N3) {
         const int i = q % N, j = (q / N) % N, k = q / N2;
-        u[(i + 1) + P * (j + 1) + PL * (k + 1)] = R(re[q]);
+        u[(i + 1) + P * (j + 1) + PL * (k + 1)] = vr[it];
       }
     }
 #pragma unroll
-    for (int it = 0; it < (6 * N2 + T - 1) / T; ++it) {
+    for (int it = 0; it < NG; ++it) {
       const int q = tid + it * T;
-      if (q < 6 * N2) {
-        const int32_t sidx = ge[q];
-        u[gpos(q)] = (sidx >= 0) ? R(r[sidx]) : R(0);
-      }
+      if (q < 6 * N2) u[gpos(q)] = gi[it] >= 0 ? vg[it] : R(0);
     }
     for (int q = tid; q < M2 * NL; q += T) {  // ghost edges and corners
       const int i = q % NL, j = (q / NL) % NL, k = q / M2;
@@ -206,23 +227,293 @@ __global__ void __launch_bounds__(Fdm<NL>::kThreads)
         wi[q] = u[(i + 1) + P * (j + 1) + PL * (k + 1)];
       }
     }
-    const int32_t *go = gout + e * 6 * N2;
 #pragma unroll
-    for (int it = 0; it < (6 * N2 + T - 1) / T; ++it) {
+    for (int it = 0; it < NG; ++it) {
       const int q = tid + it * T;
-      if (q < 6 * N2) {
-        const int32_t d = go[q];
-        if (d >= 0) wgh[d] = u[gpos(q)];
-      }
+      if (gd[it] >= 0) wgh[gd[it]] = u[gpos(q)];
     }
   }
 }
 
-// Overlap sum (hsmg.f:475-481): e = interior of the local solution, plus on
-// the first interior layer the neighbour's solution at the ghost node that
-// coincides with it (delivered to this element's ghost slots by k_sch_fdm;
-// 0 where there is no neighbour), face by face in the order r-, r+, s-, s+,
-// t-, t+.
+// ---------------------------------------------------------------------------
+// FP64 local FDM solves (the drop-in's smoother): the same arithmetic as
+// k_sch_fdm in the same order, organised for the FP64 vector units of
+// CDNA4:
+//   * the element's three NL x NL eigenvector matrices are read from LDS into
+//     registers, 16 consecutive entries per register across the 16 lanes of
+//     each row, and fed to the FMAs by a DPP row broadcast
+//     (v_fmac_f64 ... row_newbcast:n): no LDS traffic for S inside the
+//     contractions;
+//   * every global load of an element (interior, ghost ids and the gathered
+//     neighbour layers, S, eigenvalues, the ghost destinations) is issued
+//     before the first one is waited for;
+//   * the t contraction, the diagonal and the transposed t contraction run
+//     in registers on each lane's t-pencils, without an LDS round trip;
+//   * 1/(lr + ls + lt) by the hardware reciprocal refined with two
+//     Newton-Raphson steps r <- r + r (1 - d r).
+// ---------------------------------------------------------------------------
+template <int L>
+__device__ __forceinline__ void fma_rowbcast(double &acc, double s, double x) {
+  asm("v_fmac_f64_dpp %0, %1, %2 row_newbcast:%3 row_mask:0xf bank_mask:0xf"
+      : "+v"(acc)
+      : "v"(s), "v"(x), "i"(L));
+}
+
+// y[q][i] = sum_m Sm(i,m) x[q][m] for the PPT pencils of a lane (TR: Sm(m,i)),
+// Sm entry f = row + NL * col held in register f / 16 of lane f % 16 of each
+// row; outputs two at a time (independent accumulator chains)
+template <int NL, int PPT, bool TR, int I, int M>
+__device__ __forceinline__ void fdm_fma2(double (&acc)[2][PPT], const double *sr,
+                                         const double (&x)[PPT][NL]) {
+  constexpr int f0 = TR ? M + NL * I : I + NL * M;
+#pragma unroll
+  for (int q = 0; q < PPT; ++q) fma_rowbcast<f0 % 16>(acc[0][q], sr[f0 / 16], x[q][M]);
+  if constexpr (I + 1 < NL) {
+    constexpr int f1 = TR ? M + NL * (I + 1) : (I + 1) + NL * M;
+#pragma unroll
+    for (int q = 0; q < PPT; ++q) fma_rowbcast<f1 % 16>(acc[1][q], sr[f1 / 16], x[q][M]);
+  }
+}
+template <int NL, int PPT, bool TR, int I, int... Ms>
+__device__ __forceinline__ void fdm_rows(double (&acc)[2][PPT], const double *sr,
+                                         const double (&x)[PPT][NL],
+                                         std::integer_sequence<int, Ms...>) {
+  (fdm_fma2<NL, PPT, TR, I, Ms>(acc, sr, x), ...);
+}
+template <int NL, int PPT, bool TR, int... Ps>
+__device__ __forceinline__ void fdm_matvec(double (&y)[PPT][NL], const double *sr,
+                                           const double (&x)[PPT][NL],
+                                           std::integer_sequence<int, Ps...>) {
+  auto pair = [&](auto p_tag) {
+    constexpr int I = 2 * decltype(p_tag)::value;
+    double acc[2][PPT];
+#pragma unroll
+    for (int q = 0; q < PPT; ++q) acc[0][q] = acc[1][q] = 0.0;
+    fdm_rows<NL, PPT, TR, I>(acc, sr, x, std::make_integer_sequence<int, NL>());
+#pragma unroll
+    for (int q = 0; q < PPT; ++q) {
+      y[q][I] = acc[0][q];
+      if constexpr (I + 1 < NL) y[q][I + 1] = acc[1][q];
+    }
+  };
+  (pair(std::integral_constant<int, Ps>()), ...);
+}
+
+template <int NL>
+__global__ void __launch_bounds__(Fdm<NL>::kThreads)
+    k_sch_fdm_d(double *__restrict__ wint, double *__restrict__ wgh,
+                const double *__restrict__ r, const int32_t *__restrict__ gin,
+                const int32_t *__restrict__ gout, const double *__restrict__ S,
+                const double *__restrict__ lam, int64_t nel) {
+  constexpr int N = NL - 2, N2 = N * N, N3 = N2 * N, M2 = NL * NL, NLAM = 3 * NL + 1;
+  constexpr int P = Fdm<NL>::P, PL = P * NL, T = Fdm<NL>::kThreads;
+  constexpr int PPT = Fdm<NL>::PPT, NA = Fdm<NL>::kActive;
+  constexpr int NSR = (M2 + 15) / 16;  // registers per matrix
+  constexpr int NR = (N3 + T - 1) / T, NS = (3 * M2 + T - 1) / T, NG = (6 * N2 + T - 1) / T;
+  __shared__ double u[PL * NL];
+  __shared__ double sS[3 * M2 + 16];
+  __shared__ double sl[NLAM];
+  const int tid = threadIdx.x;
+  auto gpos = [](int q) {
+    const int f = q / N2, a = q % N, b = (q / N) % N;
+    int i, j, k;
+    switch (f) {
+      case 0: i = 0; j = a + 1; k = b + 1; break;
+      case 1: i = NL - 1; j = a + 1; k = b + 1; break;
+      case 2: i = a + 1; j = 0; k = b + 1; break;
+      case 3: i = a + 1; j = NL - 1; k = b + 1; break;
+      case 4: i = a + 1; j = b + 1; k = 0; break;
+      default: i = a + 1; j = b + 1; k = NL - 1; break;
+    }
+    return i + P * j + PL * k;
+  };
+  // pencils of this thread: p = tid + q*NA (q < PPT), valid when p < M2;
+  // an invalid pencil reads pencil 0 and its results are dropped (all lanes
+  // take part in the broadcast FMAs)
+  int pa[PPT], pb[PPT];
+  bool pv[PPT];
+#pragma unroll
+  for (int q = 0; q < PPT; ++q) {
+    const int p = tid + q * NA;
+    pv[q] = (tid < NA) && (p < M2);
+    pa[q] = pv[q] ? p % NL : 0;
+    pb[q] = pv[q] ? p / NL : 0;
+  }
+  double sr[NSR];
+  auto load_s = [&](int d) {
+#pragma unroll
+    for (int v = 0; v < NSR; ++v) sr[v] = sS[d * M2 + 16 * v + (tid & 15)];
+    asm volatile("s_nop 1");
+  };
+  auto base_of = [&](auto dir_tag, int q) {
+    constexpr int D = decltype(dir_tag)::value;
+    return D == 0 ? P * pa[q] + PL * pb[q] : (D == 1 ? pa[q] + PL * pb[q] : pa[q] + P * pb[q]);
+  };
+  auto load_x = [&](auto dir_tag, double (&x)[PPT][NL]) {
+    constexpr int D = decltype(dir_tag)::value;
+    constexpr int st = D == 0 ? 1 : (D == 1 ? P : PL);
+#pragma unroll
+    for (int q = 0; q < PPT; ++q) {
+      const int b = base_of(dir_tag, q);
+#pragma unroll
+      for (int m = 0; m < NL; ++m) x[q][m] = u[b + st * m];
+    }
+  };
+  auto store_y = [&](auto dir_tag, const double (&y)[PPT][NL]) {
+    constexpr int D = decltype(dir_tag)::value;
+    constexpr int st = D == 0 ? 1 : (D == 1 ? P : PL);
+#pragma unroll
+    for (int q = 0; q < PPT; ++q) {
+      const int b = base_of(dir_tag, q);
+      if (pv[q]) {
+#pragma unroll
+        for (int m = 0; m < NL; ++m) u[b + st * m] = y[q][m];
+      }
+    }
+  };
+  using R_ = std::integral_constant<int, 0>;
+  using S_ = std::integral_constant<int, 1>;
+  using T_ = std::integral_constant<int, 2>;
+  using Pairs = std::make_integer_sequence<int, (NL + 1) / 2>;
+  // element order as in k_sch_fdm (face neighbours on one XCD together)
+  const int64_t q8 = nel / 8, r8 = nel % 8;
+  for (int64_t ep = blockIdx.x; ep < nel; ep += gridDim.x) {
+    const int64_t x8 = ep % 8;
+    const int64_t e = x8 * q8 + (x8 < r8 ? x8 : r8) + ep / 8;
+    const double *re = r + e * N3;
+    const int32_t *ge = gin + e * 6 * N2;
+    const int32_t *go = gout + e * 6 * N2;
+    const double *Se = S + e * 3 * M2;
+    double vr[NR], vs[NS], vg[NG], vl;
+    int32_t gi[NG], gd[NG];
+#pragma unroll
+    for (int it = 0; it < NG; ++it) {
+      const int q = tid + it * T, qc = q < 6 * N2 ? q : 6 * N2 - 1;
+      gi[it] = q < 6 * N2 ? ge[qc] : -1;
+      gd[it] = q < 6 * N2 ? go[qc] : -1;
+    }
+#pragma unroll
+    for (int it = 0; it < NR; ++it) {
+      const int q = tid + it * T;
+      vr[it] = re[q < N3 ? q : N3 - 1];
+    }
+#pragma unroll
+    for (int it = 0; it < NS; ++it) {
+      const int q = tid + it * T;
+      vs[it] = Se[q < 3 * M2 ? q : 3 * M2 - 1];
+    }
+    vl = lam[e * NLAM + (tid < NLAM ? tid : NLAM - 1)];
+#pragma unroll
+    for (int it = 0; it < NG; ++it) vg[it] = r[gi[it] >= 0 ? gi[it] : 0];
+    __syncthreads();  // the previous element's LDS reads are done
+#pragma unroll
+    for (int it = 0; it < NR; ++it) {
+      const int q = tid + it * T;
+      if (q < N3) {
+        const int i = q % N, j = (q / N) % N, k = q / N2;
+        u[(i + 1) + P * (j + 1) + PL * (k + 1)] = vr[it];
+      }
+    }
+#pragma unroll
+    for (int it = 0; it < NS; ++it) {
+      const int q = tid + it * T;
+      if (q < 3 * M2) sS[q] = vs[it];
+    }
+    if (tid < 16) sS[3 * M2 + tid] = 0.0;
+    if (tid < NLAM) sl[tid] = vl;
+#pragma unroll
+    for (int it = 0; it < NG; ++it) {
+      const int q = tid + it * T;
+      if (q < 6 * N2) u[gpos(q)] = gi[it] >= 0 ? vg[it] : 0.0;
+    }
+    for (int q = tid; q < M2 * NL; q += T) {  // ghost edges and corners
+      const int i = q % NL, j = (q / NL) % NL, k = q / M2;
+      const int g = (i == 0 || i == NL - 1) + (j == 0 || j == NL - 1) + (k == 0 || k == NL - 1);
+      if (g >= 2) u[i + P * j + PL * k] = 0.0;
+    }
+    __syncthreads();
+    double x[PPT][NL], y[PPT][NL];
+    load_s(0);
+    load_x(R_(), x);
+    fdm_matvec<NL, PPT, true>(y, sr, x, Pairs());
+    store_y(R_(), y);
+    __syncthreads();
+    load_s(1);
+    load_x(S_(), x);
+    fdm_matvec<NL, PPT, true>(y, sr, x, Pairs());
+    store_y(S_(), y);
+    __syncthreads();
+    load_s(2);
+    load_x(T_(), x);
+    fdm_matvec<NL, PPT, true>(y, sr, x, Pairs());
+    // the diagonal (hsmg.f:753-770: 0 where lr + ls + lt <= eps)
+#pragma unroll
+    for (int q = 0; q < PPT; ++q) {
+      const double lij = sl[pa[q]] + sl[NL + pb[q]], eps = sl[3 * NL];
+#pragma unroll
+      for (int k = 0; k < NL; ++k) {
+        const double d = lij + sl[2 * NL + k];
+        double rc = __builtin_amdgcn_rcp(d);
+        rc = fma(rc, fma(-d, rc, 1.0), rc);
+        rc = fma(rc, fma(-d, rc, 1.0), rc);
+        y[q][k] = (d > eps) ? y[q][k] * rc : 0.0;
+      }
+    }
+    asm volatile("s_nop 1");
+    fdm_matvec<NL, PPT, false>(x, sr, y, Pairs());
+    store_y(T_(), x);
+    __syncthreads();
+    load_s(1);
+    load_x(S_(), x);
+    fdm_matvec<NL, PPT, false>(y, sr, x, Pairs());
+    store_y(S_(), y);
+    __syncthreads();
+    load_s(0);
+    load_x(R_(), x);
+    fdm_matvec<NL, PPT, false>(y, sr, x, Pairs());
+    store_y(R_(), y);
+    __syncthreads();
+    double *wi = wint + e * N3;
+#pragma unroll
+    for (int it = 0; it < NR; ++it) {
+      const int q = tid + it * T;
+      if (q < N3) {
+        const int i = q % N, j = (q / N) % N, k = q / N2;
+        wi[q] = u[(i + 1) + P * (j + 1) + PL * (k + 1)];
+      }
+    }
+#pragma unroll
+    for (int it = 0; it < NG; ++it) {
+      const int q = tid + it * T;
+      if (gd[it] >= 0) wgh[gd[it]] = u[gpos(q)];
+    }
+  }
+}
+
+// Overlap sum (hsmg.f:475-481) at node (i,j,k) of element el: the interior
+// of the local solution, plus on the first interior layer the neighbour's
+// solution at the ghost node that coincides with it (delivered to this
+// element's ghost slots by k_sch_fdm; 0 where there is no neighbour), face by
+// face in the order r-, r+, s-, s+, t-, t+.  k_sch_out, the fused
+// prolongation (k_tnsr MODE 4) and the fused gather-scatter (k_gs_fin_ov)
+// all form it here, in this order, so their values are identical.
+template <int N, typename R>
+__device__ __forceinline__ R overlap_value(const R *__restrict__ wint,
+                                           const R *__restrict__ wgh, int64_t el, int i, int j,
+                                           int k) {
+  constexpr int N2 = N * N, N3 = N2 * N;
+  const R *g = wgh + el * 6 * N2;
+  R a = wint[el * N3 + i + N * (j + N * k)];
+  if (i == 1) a += g[0 * N2 + j + N * k];
+  if (i == N - 2) a += g[1 * N2 + j + N * k];
+  if (j == 1) a += g[2 * N2 + i + N * k];
+  if (j == N - 2) a += g[3 * N2 + i + N * k];
+  if (k == 1) a += g[4 * N2 + i + N * j];
+  if (k == N - 2) a += g[5 * N2 + i + N * j];
+  return a;
+}
+
+// Overlap sum of every node into e (the path without the fused kernels).
 // Blocks of 64 walk elements; the element size N is a compile-time constant
 // so the node coordinates cost no divisions, and each lane issues all its
 // loads of an element before using them.
@@ -232,28 +523,13 @@ __global__ void __launch_bounds__(64)
               int64_t nel) {
   constexpr int N2 = N * N, N3 = N2 * N, U = (N3 + 63) / 64;
   for (int64_t e = blockIdx.x; e < nel; e += gridDim.x) {
-    const R *g = wgh + e * 6 * N2;
-    const R *wi = wint + e * N3;
     EO *out = eo + e * N3;
-    R v[U];
-#pragma unroll
-    for (int u = 0; u < U; ++u) {
-      const int q = (int)threadIdx.x + 64 * u;
-      v[u] = q < N3 ? wi[q] : R(0);
-    }
 #pragma unroll
     for (int u = 0; u < U; ++u) {
       const int q = (int)threadIdx.x + 64 * u;
       if (q >= N3) continue;
       const int i = q % N, j = (q / N) % N, k = q / N2;
-      R a = v[u];
-      if (i == 1) a += g[0 * N2 + j + N * k];
-      if (i == N - 2) a += g[1 * N2 + j + N * k];
-      if (j == 1) a += g[2 * N2 + i + N * k];
-      if (j == N - 2) a += g[3 * N2 + i + N * k];
-      if (k == 1) a += g[4 * N2 + i + N * j];
-      if (k == N - 2) a += g[5 * N2 + i + N * j];
-      out[q] = EO(a);
+      out[q] = EO(overlap_value<N, R>(wint, wgh, e, i, j, k));
     }
   }
 }
@@ -286,10 +562,15 @@ int launch_sch_out(EO *e, const R *wint, const R *wgh, int nh, int64_t nel, hipS
 //           finer level): v = v * w + s.
 //   MODE 3 (prolongation into the fine level before gs_fin): v = s on
 //           element-boundary nodes, v = eb * w + s inside.
-template <int NV, int NU, int MODE, typename EB, typename UT, typename VT, typename WT>
+//   MODE 4: MODE 3 with eb formed from the local solutions (eb = wint, eg =
+//           wgh, of type EB) by the overlap sum, rounded to EO as a stored
+//           overlap sum would be.
+template <int NV, int NU, int MODE, typename EB, typename UT, typename VT, typename WT,
+          typename EO = EB>
 __global__ void k_tnsr(VT *__restrict__ v, const UT *__restrict__ u,
                        const WT *__restrict__ w, const double *__restrict__ A, int64_t nel,
-                       const EB *__restrict__ eb, const double *__restrict__ om) {
+                       const EB *__restrict__ eb, const double *__restrict__ om,
+                       const EB *__restrict__ eg) {
   constexpr int NV3 = NV * NV * NV, NU3 = NU * NU * NU;
   __shared__ double sA[NV * NU];
   __shared__ double su[NU3];
@@ -329,6 +610,12 @@ __global__ void k_tnsr(VT *__restrict__ v, const UT *__restrict__ u,
         const int i = t % NV, j = (t / NV) % NV, k = t / (NV * NV);
         const bool bnd = (i == 0 || i == NV - 1 || j == 0 || j == NV - 1 || k == 0 || k == NV - 1);
         v[o] = VT(bnd ? s : double(eb[o]) * double(w[o]) + s);
+      } else if constexpr (MODE == 4) {
+        const int i = t % NV, j = (t / NV) % NV, k = t / (NV * NV);
+        const bool bnd = (i == 0 || i == NV - 1 || j == 0 || j == NV - 1 || k == 0 || k == NV - 1);
+        v[o] = VT(bnd ? s
+                      : double(EO(overlap_value<NV, EB>(eb, eg, e, i, j, k))) * double(w[o]) +
+                            s);
       } else {
         v[o] = VT(om ? s * om[o] : s);
       }
@@ -359,6 +646,53 @@ __global__ void k_crs_scatter(double *__restrict__ e, const double *__restrict__
 }
 
 __global__ void k_zero_int(int *flag) { *flag = 0; }
+
+// the held coarse unknown's right-hand side: 0
+__global__ void k_crs_pin(double *b, int pin) { b[pin] = 0.0; }
+
+// x <- x - mean(x) over n entries, one block, fixed summation order
+__global__ void __launch_bounds__(1024) k_sub_mean_1b(double *__restrict__ x, int n) {
+  __shared__ double sw[16];
+  double a = 0.0;
+  for (int i = threadIdx.x; i < n; i += blockDim.x) a += x[i];
+  for (int o = 32; o > 0; o >>= 1) a += __shfl_down(a, o, 64);
+  if ((threadIdx.x & 63) == 0) sw[threadIdx.x >> 6] = a;
+  __syncthreads();
+  double s = 0.0;
+  for (int w = 0; w < (int)(blockDim.x >> 6); ++w) s += sw[w];
+  const double m = s / (double)n;
+  for (int i = threadIdx.x; i < n; i += blockDim.x) x[i] -= m;
+}
+
+// ORTHO (navier1.f:223-255): partial sums of v over a fixed grid, then every
+// block of the second kernel forms the same total and subtracts the mean
+constexpr int kOrthoBlocks = 1024;
+__global__ void __launch_bounds__(256) k_ortho_part(const double *__restrict__ v, int64_t n,
+                                                    double *__restrict__ part) {
+  __shared__ double sw[4];
+  double a = 0.0;
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x)
+    a += v[i];
+  for (int o = 32; o > 0; o >>= 1) a += __shfl_down(a, o, 64);
+  if ((threadIdx.x & 63) == 0) sw[threadIdx.x >> 6] = a;
+  __syncthreads();
+  if (threadIdx.x == 0) part[blockIdx.x] = (sw[0] + sw[1]) + (sw[2] + sw[3]);
+}
+__global__ void __launch_bounds__(256) k_ortho_sub(double *__restrict__ v, int64_t n,
+                                                   const double *__restrict__ part, int nb) {
+  __shared__ double tot;
+  if (threadIdx.x == 0) {
+    double s = 0.0;
+    for (int b = 0; b < nb; ++b) s += part[b];
+    tot = s / (double)n;
+  }
+  __syncthreads();
+  const double m = tot;
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x)
+    v[i] -= m;
+}
 
 // 1 where a != b (flag set, never cleared here)
 __global__ void k_neq(const double *a, const double *b, int64_t n, int *flag) {
@@ -537,8 +871,12 @@ template <int NL, typename R, typename RI>
 void launch_sch_fdm(R *wint, R *wgh, const R *S, const R *lam, const MgLevel &L, const RI *r,
                     int64_t nel, hipStream_t s) {
   const int grid = (int)std::min<int64_t>((nel + 7) / 8 * 8, 1 << 20);
-  hipLaunchKernelGGL((k_sch_fdm<NL, R, RI>), dim3(grid), dim3(Fdm<NL>::kThreads), 0, s, wint,
-                     wgh, r, L.gin, L.gout, S, lam, nel);
+  if constexpr (std::is_same<R, double>::value && std::is_same<RI, double>::value)
+    hipLaunchKernelGGL((k_sch_fdm_d<NL>), dim3(grid), dim3(Fdm<NL>::kThreads), 0, s, wint, wgh,
+                       r, L.gin, L.gout, S, lam, nel);
+  else
+    hipLaunchKernelGGL((k_sch_fdm<NL, R, RI>), dim3(grid), dim3(Fdm<NL>::kThreads), 0, s, wint,
+                       wgh, r, L.gin, L.gout, S, lam, nel);
 }
 
 }  // namespace
@@ -592,7 +930,68 @@ static int schwarz_local(Mg &mg, int l, const RI *r, EO *e, hipStream_t s) {
       return NKH_ERR_UNSUPPORTED;
   }
   NK_HIP(hipGetLastError());
-  NK_TRY((launch_sch_out<R, EO>(e, wint, wgh, L.nh, mg.nel, s)));
+  // e == nullptr: the overlap sum is left to the fused consumers
+  if (e) NK_TRY((launch_sch_out<R, EO>(e, wint, wgh, L.nh, mg.nel, s)));
+  NK_HIP(hipGetLastError());
+  return NKH_OK;
+}
+
+// The last step of the V-cycle at the fine level (gs_fin's dssum + weights +
+// coarse correction + dsavg, ic.f:1871-1897) with the Schwarz sum e of each
+// member formed from the local solutions by the overlap sum (rounded to EO,
+// the type a stored overlap sum would have) instead of read from memory.
+// Same group walk and summation order as k_gs_fin.
+template <int N, typename R, typename EO, typename Z, typename W>
+__global__ void __launch_bounds__(256)
+    k_gs_fin_ov(Z *__restrict__ z, const R *__restrict__ wint, const R *__restrict__ wgh,
+                const W *__restrict__ w, const int32_t *__restrict__ off,
+                const int32_t *__restrict__ idx, int64_t ngroups, int64_t chunk) {
+  constexpr int N2 = N * N, N3 = N2 * N;
+  const int nb = gridDim.x, b = blockIdx.x;
+  const int64_t lb = (int64_t)(b % 8) * (nb / 8) + b / 8;
+  const int64_t s0 = lb * chunk;
+  const int64_t s1 = (s0 + chunk < ngroups) ? s0 + chunk : ngroups;
+  auto ev = [&](int32_t l) {
+    const int64_t el = l / N3;
+    const int q = (int)(l - el * N3);
+    return EO(overlap_value<N, R>(wint, wgh, el, q % N, (q / N) % N, q / N2));
+  };
+  for (int64_t g = s0 + threadIdx.x; g < s1; g += blockDim.x) {
+    const int32_t a = off[g], c = off[g + 1];
+    EO Se = ev(idx[a]);
+    for (int32_t q = a + 1; q < c; ++q) Se += ev(idx[q]);
+    const double S = double(Se);
+    double acc = S * double(w[idx[a]]) + double(z[idx[a]]);
+    for (int32_t q = a + 1; q < c; ++q) {
+      const int32_t l = idx[q];
+      acc += S * double(w[l]) + double(z[l]);
+    }
+    acc = acc * (1.0 / (double)(c - a));
+    for (int32_t q = a; q < c; ++q) z[idx[q]] = Z(acc);
+  }
+}
+
+template <typename R, typename EO, typename Z, typename W>
+static int gs_fin_ov(const GsMap &gs, int nh, Z *z, const R *wint, const R *wgh, const W *w,
+                     hipStream_t s) {
+  if (gs.ngroups == 0) return NKH_OK;
+  int64_t nb = (gs.ngroups + 511) / 512;
+  nb = (nb + 7) / 8 * 8;
+  nb = nb < 64 ? 64 : (nb > kGsBlocks ? kGsBlocks : nb);
+  const int64_t chunk = (gs.ngroups + nb - 1) / nb;
+  switch (nh) {
+#define NK_G(N)                                                                               \
+  case N:                                                                                     \
+    hipLaunchKernelGGL((k_gs_fin_ov<N, R, EO, Z, W>), dim3((int)nb), dim3(256), 0, s, z, wint, \
+                       wgh, w, gs.off, gs.idx, gs.ngroups, chunk);                            \
+    break;
+    NK_G(3) NK_G(4) NK_G(5) NK_G(6) NK_G(7) NK_G(8) NK_G(9) NK_G(10) NK_G(11) NK_G(12)
+    NK_G(13) NK_G(14) NK_G(15) NK_G(16)
+#undef NK_G
+    default:
+      set_error("fused overlap gather-scatter of %d points not instantiated", nh);
+      return NKH_ERR_UNSUPPORTED;
+  }
   NK_HIP(hipGetLastError());
   return NKH_OK;
 }
@@ -612,16 +1011,16 @@ int mg_schwarz_local(Mg &mg, int l, const double *r, double *e, hipStream_t s) {
 // om (MODE 1 only, may be null): 0/1 mask applied to the output.  UT/VT:
 // storage of the input / output field (arithmetic in FP64).
 template <int MODE, typename EB = double, typename UT = double, typename VT = double,
-          typename WT = double>
+          typename WT = double, typename EO = EB>
 static int run_tnsr(VT *v, int nv, const UT *u, int nu, const WT *w, const double *A,
                     int64_t nel, hipStream_t s, const EB *eb = nullptr,
-                    const double *om = nullptr) {
+                    const double *om = nullptr, const EB *eg = nullptr) {
   const int grid = (int)std::min<int64_t>(nel, 1 << 20);
 #define NK_T(V, U)                                                                       \
   if (nv == V && nu == U) {                                                              \
-    hipLaunchKernelGGL((k_tnsr<V, U, MODE, EB, UT, VT, WT>), dim3(grid),                \
+    hipLaunchKernelGGL((k_tnsr<V, U, MODE, EB, UT, VT, WT, EO>), dim3(grid),            \
                        dim3(V * V * V > 256 ? 256 : 64),                                  \
-                       0, s, v, u, w, A, nel, eb, om);                                   \
+                       0, s, v, u, w, A, nel, eb, om, eg);                               \
     NK_HIP(hipGetLastError());                                                           \
     return NKH_OK;                                                                       \
   }
@@ -654,9 +1053,12 @@ static int coarse_solve(Mg &mg, double *e1, const double *r1, int crs_mode, hipS
     hipLaunchKernelGGL(k_crs_gather, dim3(mg_grid(n)), dim3(256), 0, s, mg.cb, r1, mg.goff,
                        mg.gidx, n);
     NK_HIP(hipGetLastError());
+    if (mg.null_space) hipLaunchKernelGGL(k_crs_pin, dim3(1), dim3(1), 0, s, mg.cb, n - 1);
     if (crs_mode == 1) NK_TRY(mg.crs.vcycle_only(mg.cx, mg.cb, s));
     else if (crs_mode == 2) NK_TRY(mg.crs.solve(mg.cx, mg.cb, s, kInnerCoarseRtol));
     else NK_TRY(mg.crs.solve(mg.cx, mg.cb, s));
+    if (mg.null_space)
+      hipLaunchKernelGGL(k_sub_mean_1b, dim3(1), dim3(1024), 0, s, mg.cx, n);
   }
   hipLaunchKernelGGL(k_crs_scatter, dim3(mg_grid(8 * mg.nel)), dim3(256), 0, s, e1, mg.cx,
                      mg.slot, 8 * mg.nel);
@@ -686,10 +1088,12 @@ static int mg_solve_t(Mesh &mesh, FT *z, const FT *r, hipStream_t s, int crs_mod
   // otherwise dssum now and the plain sequence after the join.
   MgLevel &T = mg.lev[Lm];
   const bool merged = mesh.gs.interior_zero;
+  const bool timed = timers().on;
+  if (timed) NK_HIP(hipEventRecord(mg.tev[2], s));
   NK_CHECK(merged || !f32io, NKH_ERR_UNSUPPORTED,
            "single-precision V-cycle needs global ids zero exactly at element interiors");
   const bool fp32 = (precision == 1 || f32io) && merged;
-  if (fp32) NK_TRY(mg_prepare_fp32(mg, s));
+  if (fp32) NK_TRY(mg_prepare_fp32(mg, s, f32io));
   // The streaming head of the coarse chain runs first, on s: the
   // restriction of the fine residual, its dssum and (when that level is 2)
   // the restriction to level 1.  Beside the fine local solves these
@@ -719,12 +1123,16 @@ static int mg_solve_t(Mesh &mesh, FT *z, const FT *r, hipStream_t s, int crs_mod
   }
   NK_HIP(hipEventRecord(mg.ev_fork, s));
   NK_HIP(hipStreamWaitEvent(b, mg.ev_fork, 0));
+  // merged: the overlap sum is formed inside the prolongation and the final
+  // gather-scatter (k_tnsr MODE 4, k_gs_fin_ov) from the local solutions
+  if (timed) NK_HIP(hipEventRecord(mg.tev[0], s));
   if (fp32) {
-    NK_TRY((schwarz_local<float, FT>(mg, Lm, r, T.e32, s)));
+    NK_TRY((schwarz_local<float, FT>(mg, Lm, r, (float *)nullptr, s)));
   } else if constexpr (!f32io) {
-    NK_TRY((schwarz_local<double, FT>(mg, Lm, r, merged ? T.e : z, s)));
+    NK_TRY((schwarz_local<double, FT>(mg, Lm, r, merged ? (double *)nullptr : z, s)));
     if (!merged) NK_TRY(gs_op(T.gs, z, 0, s));
   }
+  if (timed) NK_HIP(hipEventRecord(mg.tev[1], s));
   // coarser levels on b: restrict, dssum, mask; the Schwarz solve of each
   // intermediate level on c (it only reads the masked residual), beside the
   // restriction to level 1 and the coarse solve on b
@@ -765,22 +1173,33 @@ static int mg_solve_t(Mesh &mesh, FT *z, const FT *r, hipStream_t s, int crs_mod
   MgLevel &C = mg.lev[Lm - 1];
   if (fp32) {
     if constexpr (f32io) {
-      NK_TRY((run_tnsr<3, float, double, FT, float>(z, T.nh, C.e, C.nh, T.mswt32, C.J, nel, s,
-                                                    T.e32)));
-      NK_TRY(gs_fin(mesh.gs, z, T.e32, T.mswt32, s));
+      NK_TRY((run_tnsr<4, float, double, FT, float>(z, T.nh, C.e, C.nh, T.mswt32, C.J, nel, s,
+                                                    T.wint32, nullptr, T.wgh32)));
+      NK_TRY((gs_fin_ov<float, float>(mesh.gs, T.nh, z, T.wint32, T.wgh32, T.mswt32, s)));
     } else {
-      NK_TRY((run_tnsr<3, float, double, FT>(z, T.nh, C.e, C.nh, T.mswt, C.J, nel, s, T.e32)));
-      NK_TRY(gs_fin(mesh.gs, z, T.e32, T.mswt, s));
+      NK_TRY((run_tnsr<4, float, double, FT>(z, T.nh, C.e, C.nh, T.mswt, C.J, nel, s, T.wint32,
+                                             nullptr, T.wgh32)));
+      NK_TRY((gs_fin_ov<float, float>(mesh.gs, T.nh, z, T.wint32, T.wgh32, T.mswt, s)));
     }
   } else if constexpr (!f32io) {
     if (merged) {
-      NK_TRY(run_tnsr<3>(z, T.nh, C.e, C.nh, T.mswt, C.J, nel, s, T.e));
-      NK_TRY(gs_fin(mesh.gs, z, T.e, T.mswt, s));
+      NK_TRY(run_tnsr<4>(z, T.nh, C.e, C.nh, T.mswt, C.J, nel, s, T.wint, nullptr, T.wgh));
+      NK_TRY((gs_fin_ov<double, double>(mesh.gs, T.nh, z, T.wint, T.wgh, T.mswt, s)));
     } else {
       NK_TRY(run_tnsr<2>(z, T.nh, C.e, C.nh, T.mswt, C.J, nel, s));
       NK_TRY(gs_op(mesh.gs, z, 4, s));
     }
   }
+  if (timed) NK_HIP(hipEventRecord(mg.tev[3], s));
+  return NKH_OK;
+}
+
+int mg_ortho(Mg &mg, double *v, int64_t n, hipStream_t s) {
+  if (!mg.opart) NK_HIP(hipMalloc(&mg.opart, sizeof(double) * kOrthoBlocks));
+  hipLaunchKernelGGL(k_ortho_part, dim3(kOrthoBlocks), dim3(256), 0, s, v, n, mg.opart);
+  hipLaunchKernelGGL(k_ortho_sub, dim3(kOrthoBlocks), dim3(256), 0, s, v, n, mg.opart,
+                     kOrthoBlocks);
+  NK_HIP(hipGetLastError());
   return NKH_OK;
 }
 
@@ -989,12 +1408,12 @@ int run_flex_cg(Mesh &m, double *res, Coef h1, Coef h2, const double *wt, const 
   Mg &mg = *m.mg;
   hipStream_t s = cur_stream();
   const int64_t n = m.n;
-  if (!mg.fr) {
+  // (the mixed solver allocates fr..fw on its own, so each piece is checked)
+  if (!mg.fr)
     for (double **p : {&mg.fr, &mg.fp, &mg.fz, &mg.fw}) NK_TRY(mg_dalloc(p, n));
-    NK_TRY(mg_dalloc(&mg.fpart, 3 * kFcgBlocks));
-    NK_HIP(hipMalloc(&mg.fst, sizeof(FcgState)));
-    NK_HIP(hipMalloc(&mg.fflag, sizeof(int)));
-  }
+  if (!mg.fpart) NK_TRY(mg_dalloc(&mg.fpart, 3 * kFcgBlocks));
+  if (!mg.fst) NK_HIP(hipMalloc(&mg.fst, sizeof(FcgState)));
+  if (!mg.fflag) NK_HIP(hipMalloc(&mg.fflag, sizeof(int)));
   const double *mgmask = mg.lev[mg.lmax].mask;
   double *r = mg.fr, *p = mg.fp, *z = mg.fz, *w = mg.fw;
   double *pa = mg.fpart, *pb = mg.fpart + kFcgBlocks;
@@ -1076,6 +1495,14 @@ int run_flex_cg(Mesh &m, double *res, Coef h1, Coef h2, const double *wt, const 
       tm.naxhm += 1;
       tm.tdsum += c * 1e-3;
       tm.ndsum += 1;
+      if (m.mg) {
+        (void)hipEventElapsedTime(&a, m.mg->tev[0], m.mg->tev[1]);
+        (void)hipEventElapsedTime(&c, m.mg->tev[2], m.mg->tev[3]);
+        tm.tfdm += a * 1e-3;
+        tm.nfdm += 1;
+        tm.tmgv += c * 1e-3;
+        tm.nmgv += 1;
+      }
     }
     iter = hs.iter;
     if (rnorm_history) rnorm_history[k - 1] = hs.rnorm;
@@ -1085,6 +1512,8 @@ int run_flex_cg(Mesh &m, double *res, Coef h1, Coef h2, const double *wt, const 
     for (auto &e : ev)
       if (e) (void)hipEventDestroy(e);
   NK_TRY(rc);
+  // hmholtz.f:2272: orthogonalise the solution against the null space
+  if (mg_null_space_on(mg, h2)) NK_TRY(mg_ortho(mg, res, n, s));
   if (iter == 0) {
     NK_HIP(hipMemcpyAsync(&hs, mg.fst, sizeof(FcgState), hipMemcpyDeviceToHost, s));
     NK_HIP(hipStreamSynchronize(s));

@@ -474,13 +474,24 @@ int falloc(float **p, int64_t n) {
 // single-precision smoother data of level l: S, eigenvalues, local solution
 // and ghost slots
 static int prepare_level_fp32(Mg &mg, MgLevel &L, hipStream_t s) {
+  if (L.fs32) return NKH_OK;  // this level is prepared (all four arrays or none)
   const int nl = L.nh + 2;
   const int64_t ns = (int64_t)nl * nl * 3 * mg.nel, nlam = (int64_t)(3 * nl + 1) * mg.nel;
   const int64_t ngh = (int64_t)6 * L.nh * L.nh * mg.nel;
-  NK_TRY(falloc(&L.fs32, ns));
-  NK_TRY(falloc(&L.fl32, nlam));
-  NK_TRY(falloc(&L.wint32, L.n));
-  NK_TRY(falloc(&L.wgh32, ngh));
+  float *fs = nullptr, *fl = nullptr, *wi = nullptr, *wg = nullptr;
+  int rc = falloc(&fs, ns);
+  if (rc == NKH_OK) rc = falloc(&fl, nlam);
+  if (rc == NKH_OK) rc = falloc(&wi, L.n);
+  if (rc == NKH_OK) rc = falloc(&wg, ngh);
+  if (rc != NKH_OK) {
+    for (float *p : {fs, fl, wi, wg})
+      if (p) (void)hipFree(p);
+    return rc;
+  }
+  L.fs32 = fs;
+  L.fl32 = fl;
+  L.wint32 = wi;
+  L.wgh32 = wg;
   hipLaunchKernelGGL(k_to_f32, dim3(mg_grid(ns)), dim3(256), 0, s, L.fs32, L.fs, ns);
   hipLaunchKernelGGL(k_to_f32, dim3(mg_grid(nlam)), dim3(256), 0, s, L.fl32, L.fl, nlam);
   // ghost slots without a neighbour are never written: 0
@@ -489,16 +500,17 @@ static int prepare_level_fp32(Mg &mg, MgLevel &L, hipStream_t s) {
   return NKH_OK;
 }
 
-int mg_prepare_fp32(Mg &mg, hipStream_t s) {
+// Single-precision smoother data: the fine level's (precision 1 and 2) and,
+// for the mixed solver's inner V-cycles (inner), the intermediate levels'.
+// Each level is prepared once.
+int mg_prepare_fp32(Mg &mg, hipStream_t s, bool inner) {
   MgLevel &L = mg.lev[mg.lmax];
-  if (L.fs32) return NKH_OK;
-  // intermediate levels (the mixed solver's inner V-cycles smooth them in
-  // FP32 too)
-  for (int l = 2; l < mg.lmax; ++l) NK_TRY(prepare_level_fp32(mg, mg.lev[l], s));
+  if (inner)
+    for (int l = 2; l < mg.lmax; ++l) NK_TRY(prepare_level_fp32(mg, mg.lev[l], s));
   NK_TRY(prepare_level_fp32(mg, L, s));
-  NK_TRY(falloc(&L.e32, L.n));
-  NK_TRY(falloc(&L.mswt32, L.n));
-  NK_TRY(falloc(&L.rwt32, L.n));
+  if (L.mswt32 && L.rwt32) return NKH_OK;
+  if (!L.mswt32) NK_TRY(falloc(&L.mswt32, L.n));
+  if (!L.rwt32) NK_TRY(falloc(&L.rwt32, L.n));
   hipLaunchKernelGGL(k_to_f32, dim3(mg_grid(L.n)), dim3(256), 0, s, L.mswt32, L.mswt, L.n);
   hipLaunchKernelGGL(k_to_f32, dim3(mg_grid(L.n)), dim3(256), 0, s, L.rwt32, L.rwt, L.n);
   NK_HIP(hipGetLastError());
@@ -528,8 +540,11 @@ void mg_free(Mesh &mesh) {
                   (void *)mg->fflag, (void *)mg->gV, (void *)mg->gZ, (void *)mg->gx,
                   (void *)mg->gr, (void *)mg->gw, (void *)mg->gpart, (void *)mg->gsm,
                   (void *)mg->xr, (void *)mg->xp, (void *)mg->xz, (void *)mg->xw,
-                  (void *)mg->xx, (void *)mg->xwt, (void *)mg->xpart, mg->xst})
+                  (void *)mg->xx, (void *)mg->xwt, (void *)mg->xpart, mg->xst,
+                  (void *)mg->opart})
     if (p) (void)hipFree(p);
+  for (hipEvent_t e : mg->tev)
+    if (e) (void)hipEventDestroy(e);
   if (mg->ev_fork) (void)hipEventDestroy(mg->ev_fork);
   if (mg->ev_join) (void)hipEventDestroy(mg->ev_join);
   for (int l = 0; l < 4; ++l) {
@@ -569,9 +584,13 @@ int mg_setup(Mesh &mesh, const int *fbc_h, const int64_t *vertex, const double *
   }
   NK_HIP(hipEventCreateWithFlags(&mg.ev_fork, hipEventDisableTiming));
   NK_HIP(hipEventCreateWithFlags(&mg.ev_join, hipEventDisableTiming));
+  for (hipEvent_t &e : mg.tev) NK_HIP(hipEventCreate(&e));
   int nx[4] = {0, 0, 0, 0};
   mg_levels(m->np, &mg.lmax, nx);
   NK_TRY(upload(&mg.fbc, std::vector<int>(fbc_h, fbc_h + 6 * nel)));
+  mg.null_space = true;
+  for (int64_t q = 0; q < 6 * nel; ++q)
+    if (fbc_h[q] == 1) mg.null_space = false;
   for (int l = 1; l <= mg.lmax; ++l) {
     MgLevel &L = mg.lev[l];
     L.nx = nx[l];
@@ -836,8 +855,22 @@ int mg_setup(Mesh &mesh, const int *fbc_h, const int64_t *vertex, const double *
         }
     std::vector<int32_t> rp(ncrs + 1, 0), ci;
     std::vector<double> va;
+    // null space: the last unknown held at zero (its row and column dropped,
+    // unit diagonal), as the oracle's dense factor does
+    const int32_t pin = (mg.null_space && ncrs > 0) ? ncrs - 1 : -1;
     for (int i = 0; i < ncrs; ++i) {
       auto &rw = rows[(size_t)i];
+      if (pin >= 0) {
+        if (i == pin) {
+          rw.assign(1, std::make_pair(pin, 1.0));
+        } else {
+          rw.erase(std::remove_if(rw.begin(), rw.end(),
+                                  [pin](const std::pair<int32_t, double> &a) {
+                                    return a.first == pin;
+                                  }),
+                   rw.end());
+        }
+      }
       std::stable_sort(rw.begin(), rw.end(),
                        [](const std::pair<int32_t, double> &a,
                           const std::pair<int32_t, double> &b) { return a.first < b.first; });

@@ -4,7 +4,9 @@
 // gather-scatter and norms in FP64); each correction is an inner flexible CG
 // in single precision: its vectors, the operator (geometric factors stored
 // in FP32) and the fine level of the V-cycle in FP32, its inner products
-// accumulated in FP64, the coarse levels and the coarse solve in FP64.  The
+// accumulated in FP64; the local solves of the intermediate levels in FP32
+// between FP64 residuals and corrections; the coarse solve in FP64, to a
+// loose relative tolerance inside the inner V-cycles.  The
 // inner solve stops at kInnerRtol times the outer residual (or at half the
 // outer tolerance), so two or three outer steps reach 1e-8.
 #include <algorithm>
@@ -402,13 +404,11 @@ int run_flex_cg_mixed(Mesh &m, double *res, Coef h1, Coef h2, const double *wt,
            "ids zero at element interiors");
   {
     int differ = 0;
-    int *flag = nullptr;
-    NK_HIP(hipMalloc(&flag, sizeof(int)));
-    hipLaunchKernelGGL(k_mx_zero_flag, dim3(1), dim3(1), 0, s, flag);
-    hipLaunchKernelGGL(k_mx_neq, dim3(mg_grid(n)), dim3(256), 0, s, mask, mgmask, n, flag);
-    NK_HIP(hipMemcpyAsync(&differ, flag, sizeof(int), hipMemcpyDeviceToHost, s));
+    if (!mg.fflag) NK_HIP(hipMalloc(&mg.fflag, sizeof(int)));
+    hipLaunchKernelGGL(k_mx_zero_flag, dim3(1), dim3(1), 0, s, mg.fflag);
+    hipLaunchKernelGGL(k_mx_neq, dim3(mg_grid(n)), dim3(256), 0, s, mask, mgmask, n, mg.fflag);
+    NK_HIP(hipMemcpyAsync(&differ, mg.fflag, sizeof(int), hipMemcpyDeviceToHost, s));
     NK_HIP(hipStreamSynchronize(s));
-    (void)hipFree(flag);
     NK_CHECK(!differ, NKH_ERR_UNSUPPORTED,
              "mixed-precision solve: the pressure mask must be the multigrid's fine mask");
   }
@@ -485,6 +485,8 @@ int run_flex_cg_mixed(Mesh &m, double *res, Coef h1, Coef h2, const double *wt,
     rn = std::sqrt(t / m.vol);
   }
   NK_HIP(hipMemcpyAsync(res, x64, sizeof(double) * n, hipMemcpyDeviceToDevice, s));
+  // hmholtz.f:2272: orthogonalise the solution against the null space
+  if (mg_null_space_on(mg, h2)) NK_TRY(mg_ortho(mg, res, n, s));
   NK_HIP(hipStreamSynchronize(s));
   if (out) {
     out->niter = total;

@@ -132,6 +132,10 @@ struct Timers {
   bool on = false;
   double taxhm = 0, tdsum = 0, thmhz = 0;
   int64_t naxhm = 0, ndsum = 0, nhmhz = 0;
+  // flexible CG + h1mg: the fine-level local FDM solves and the whole
+  // V-cycle on the fine level's stream, per iteration
+  double tfdm = 0, tmgv = 0;
+  int64_t nfdm = 0, nmgv = 0;
 };
 Timers &timers();
 

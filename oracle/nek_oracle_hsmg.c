@@ -216,6 +216,8 @@ typedef struct {
   int ncrs;
   int64_t *crs_slot; /* per level-1 local node: dense index or -1 */
   double *Lc;        /* dense Cholesky factor ncrs x ncrs */
+  int null_space;    /* no Dirichlet face: the pressure's constant null space
+                        (the reference's ifvcor, navier8.f:201-205) */
   double *vmult;     /* fine multiplicity inverse */
   const nko_gs *gs_fine;
 } nko_mg;
@@ -693,6 +695,17 @@ static int coarse_setup(nko_mg *mg, const nko_field *fine, const int64_t *vertex
         if (si >= 0 && sj >= 0) A[si + (int64_t)ncrs * sj] += Ae[64 * e + i + 8 * j];
       }
   free(Ae);
+  /* with the null space (crs_xxt.c:883-900, :932-955): one unknown, here the
+     last, is held at zero -- its row and column dropped, unit diagonal -- and
+     the solution's mean is removed after the solve */
+  if (mg->null_space && ncrs > 0) {
+    int pn = ncrs - 1;
+    for (int i = 0; i < ncrs; i++) {
+      A[pn + (int64_t)ncrs * i] = 0.0;
+      A[i + (int64_t)ncrs * pn] = 0.0;
+    }
+    A[pn + (int64_t)ncrs * pn] = 1.0;
+  }
   /* Cholesky, lower, in place (column-major) */
   for (int j = 0; j < ncrs; j++) {
     double s = A[j + (int64_t)ncrs * j];
@@ -721,6 +734,7 @@ static void coarse_solve(const nko_mg *mg, double *x, const double *b) {
   double *v = calloc((size_t)nc + 1, sizeof(double));
   for (int64_t q = 0; q < 8 * (int64_t)mg->nel; q++)
     if (mg->crs_slot[q] >= 0) v[mg->crs_slot[q]] += b[q];
+  if (mg->null_space && nc > 0) v[nc - 1] = 0.0;
   const double *L = mg->Lc;
   for (int i = 0; i < nc; i++) {
     double s = v[i];
@@ -731,6 +745,12 @@ static void coarse_solve(const nko_mg *mg, double *x, const double *b) {
     double s = v[i];
     for (int k = i + 1; k < nc; k++) s -= L[k + (int64_t)nc * i] * v[k];
     v[i] = s / L[i + (int64_t)nc * i];
+  }
+  if (mg->null_space && nc > 0) { /* crs_xxt.c:950-955, one rank: plain mean */
+    double s = 0.0;
+    for (int i = 0; i < nc; i++) s += v[i];
+    s = s / nc;
+    for (int i = 0; i < nc; i++) v[i] -= s;
   }
   for (int64_t q = 0; q < 8 * (int64_t)mg->nel; q++)
     x[q] = mg->crs_slot[q] >= 0 ? v[mg->crs_slot[q]] : 0.0;
@@ -751,6 +771,9 @@ nko_mg *nko_mg_setup(const nko_field *fine, const int64_t *vertex, const int *fb
   mg->vmult = (double *)fine->mult;
   mg_levels(lx1, &mg->lmax, mg->nx);
   gen_fbc(mg, fbc);
+  mg->null_space = 1;
+  for (int64_t q = 0; q < 6 * nel; q++)
+    if (fbc[q] == 1) mg->null_space = 0;
   for (int l = 1; l <= mg->lmax; l++) {
     int n = mg->nx[l];
     mg->nh[l] = n + 1;
@@ -921,6 +944,22 @@ static void ax(const nko_field *f, double *w, const double *x, const double *h1,
   for (int64_t i = 0; i < n; i++) w[i] = w[i] * f->mask[i];
 }
 
+/* navier1.f:223-255 ORTHO (ifvcor): v <- v - sum(v)/n over the local nodes */
+static void ortho(double *v, int64_t n) {
+  double s = 0.0;
+  for (int64_t i = 0; i < n; i++) s += v[i];
+  const double rlam = s / (double)n;
+  for (int64_t i = 0; i < n; i++) v[i] -= rlam;
+}
+
+/* the pressure's null space is present: no Dirichlet face and h2 = 0 */
+static int null_space_on(const nko_mg *mg, const double *h2, int64_t n) {
+  if (!mg->null_space) return 0;
+  for (int64_t i = 0; i < n; i++)
+    if (h2[i] != 0.0) return 0;
+  return 1;
+}
+
 static double glsc3(const double *a, const double *b, const double *c, int64_t n) {
   double s = 0.0;
   for (int64_t i = 0; i < n; i++) s += a[i] * b[i] * c[i];
@@ -975,6 +1014,7 @@ int nko_flex_cg(const nko_field *f, nko_mg *mg, double *res, const double *h1,
     if (rnorm_hist) rnorm_hist[k - 1] = rnorm;
     if (rnorm < tolpss) break;
   }
+  if (null_space_on(mg, h2, n)) ortho(res, n); /* hmholtz.f:2272 */
   free(r);
   free(r1);
   free(p);
@@ -1006,6 +1046,7 @@ int nko_gmres(const nko_field *f, nko_mg *mg, double *res, const double *h1, con
   if (istep == 0) tolps = 1.e-4;
   double tolpss = tolps, div0 = 0.0, rnorm = 0.0;
   int iter = 0, iconv = 0, j = 0;
+  const int nullsp = null_space_on(mg, h2, n);
   while (!iconv) {
     if (iter == 0) {
       memcpy(r, res, sizeof(double) * n);
@@ -1028,6 +1069,7 @@ int nko_gmres(const nko_field *f, nko_mg *mg, double *res, const double *h1, con
       double *vj = v + n * (j - 1), *zj = z + n * (j - 1);
       memcpy(w, vj, sizeof(double) * n);
       nko_mg_solve(mg, zj, w);
+      if (nullsp) ortho(zj, n); /* gmres.f:427 */
       ax(f, w, zj, h1, h2);
       for (int i = 1; i <= j; i++) { /* classical Gram-Schmidt, one pass */
         double sum = 0.0;
@@ -1087,6 +1129,7 @@ int nko_gmres(const nko_field *f, nko_mg *mg, double *res, const double *h1, con
   }
 #undef HH
   memcpy(res, x, sizeof(double) * n);
+  if (nullsp) ortho(res, n); /* gmres.f:534 */
   if (div0_out) *div0_out = div0;
   free(x); free(r); free(w); free(v); free(z); free(H); free(gam); free(c); free(sn);
   return iter;
@@ -1108,3 +1151,4 @@ const double *nko_mg_jh(const nko_mg *mg, int l) { return mg->jh[l]; }
 int nko_mg_ncrs(const nko_mg *mg) { return mg->ncrs; }
 void nko_mg_coarse_solve(const nko_mg *mg, double *x, const double *b) { coarse_solve(mg, x, b); }
 void nko_mg_schwarz(nko_mg *mg, int l, double *e, double *r) { schwarz(mg, l, e, r, 1.0); }
+int nko_mg_null_space(const nko_mg *mg) { return mg->null_space; }

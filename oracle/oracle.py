@@ -105,6 +105,8 @@ def lib():
         L.nko_mg_lengths.restype = ctypes.POINTER(c_double)
         L.nko_mg_ncrs.argtypes = [vp]
         L.nko_mg_ncrs.restype = c_int
+        L.nko_mg_null_space.argtypes = [vp]
+        L.nko_mg_null_space.restype = c_int
         L.nko_mg_coarse_solve.argtypes = [vp, vp, vp]
         L.nko_mg_schwarz.argtypes = [vp, c_int, vp, vp]
         L.nko_mg_apply_fdm.argtypes = [vp, c_int, vp, vp]

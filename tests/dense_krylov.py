@@ -152,9 +152,18 @@ def cggo(S, f, tin, maxit, istep=1, param18=0.0, param22=1e-7, restol=1e-7):
     return x, it, np.array(hist), rbn0
 
 
-def flex_cg(S, M, res, maxit, param21, istep=1):
+def ortho(x, count):
+    """navier1.f:223-255 ORTHO on a continuous vector held globally: the
+    local-node mean, sum(count * x) / sum(count) with count the number of
+    local copies of each global node, is subtracted."""
+    return x - float(np.sum(count * x)) / float(np.sum(count))
+
+
+def flex_cg(S, M, res, maxit, param21, istep=1, null_count=None):
     """hmh_flex_cg with the preconditioner M (a function of a global
-    vector): (solution, iterations, rnorm history, div0)."""
+    vector): (solution, iterations, rnorm history, div0).  null_count: the
+    local-copy count of each global node when the constant null space is
+    present (then ortho at the end, hmholtz.f:2272)."""
     tolps = chktcg1(S, 0.0, res)
     if param21 > 0 and tolps > abs(param21):
         tolps = abs(param21)
@@ -190,10 +199,12 @@ def flex_cg(S, M, res, maxit, param21, istep=1):
         hist.append(rnorm)
         if rnorm < tolpss:
             break
+    if null_count is not None:
+        x = ortho(x, null_count)
     return x, it, np.array(hist), div0
 
 
-def gmres(S, M, res, maxit, param21, m=30, istep=1):
+def gmres(S, M, res, maxit, param21, m=30, istep=1, null_count=None):
     """hmh_gmres (ml = mu = 1, as uzawa_gmres_split sets them):
     (solution, iterations, rnorm history, div0)."""
     norm_fac = 1.0 / np.sqrt(S.vol)
@@ -230,6 +241,8 @@ def gmres(S, M, res, maxit, param21, m=30, istep=1):
         for j in range(m):
             it += 1
             z = M(V[j])
+            if null_count is not None:
+                z = ortho(z, null_count)  # gmres.f:427
             Z.append(z)
             w = S.Am(z)
             for i in range(j + 1):
@@ -267,4 +280,6 @@ def gmres(S, M, res, maxit, param21, m=30, istep=1):
             y[k] = t / H[k, k]
         for i in range(jj):
             x = x + y[i] * Z[i]
+    if null_count is not None:
+        x = ortho(x, null_count)  # gmres.f:534
     return x, it, np.array(hist), div0

@@ -362,6 +362,23 @@ class DenseMG:
             gi = L1.gid[e].reshape(-1)  # (k,j,i) in {0,1}: index c = i + 2j + 4k
             Gc[np.ix_(gi, gi)] += G
         free = L1.mask > 0
+        # no Dirichlet face: the constant is in the null space; one unknown is
+        # held at zero and the solution's mean removed after the solve
+        # (crs_xxt.c:883-955).  The unknown held is the vertex whose id comes
+        # last in order of first appearance over (element, corner) -- the
+        # numbering of the coarse unknowns.
+        self.null_space = not np.any(self.codes == 1)
+        if self.null_space:
+            vtx = np.asarray(self.mesh.vertex).reshape(nel, 8)
+            seen, last = set(), None
+            for e in range(nel):
+                for c in range(8):
+                    if vtx[e, c] not in seen:
+                        seen.add(vtx[e, c])
+                        last = (e, c)
+            self.pin = int(L1.gid[last[0]].reshape(-1)[last[1]])
+            free = free.copy()
+            free[self.pin] = False
         self.free = free
         self.Ac = Gc
         self.Acinv = np.linalg.inv(Gc[np.ix_(free, free)])
@@ -381,6 +398,8 @@ class DenseMG:
         r1 = (self.P[0].T @ r[1]) * self.lev[0].mask
         e1 = np.zeros(self.lev[0].N)
         e1[self.free] = self.Acinv @ r1[self.free]
+        if self.null_space:
+            e1 = e1 - np.mean(e1)
         e1 = e1 * self.lev[0].mask
         eloc[0] = self.lev[0].gather(e1)
         for lev in range(1, L):

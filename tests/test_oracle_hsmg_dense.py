@@ -24,7 +24,8 @@ from nek5000_amd.mesh import BoxMesh
 from oracle import oracle as O
 from problems import Problem
 
-SYM_CODES = {"dirichlet": (1, 1, 1, 1, 1, 1), "mixed": (1, 1, 2, 2, 1, 2)}
+SYM_CODES = {"dirichlet": (1, 1, 1, 1, 1, 1), "mixed": (1, 1, 2, 2, 1, 2),
+             "neumann": (2, 2, 2, 2, 2, 2)}
 
 
 def build(nel, lx1, codes="dirichlet", seed=0, rotate=True, **kw):
@@ -48,6 +49,9 @@ CASES = [
     dict(nel=(2, 3, 2), lx1=5, deform=0.1, seed=4, codes="mixed"),
     dict(nel=(2, 2, 3), lx1=4, jitter=0.1, seed=5),
     dict(nel=(3, 3, 3), lx1=6, deform=0.1, seed=6, rotate=False),
+    # no Dirichlet face: the pressure's constant null space (ifvcor)
+    dict(nel=(3, 2, 2), lx1=6, deform=0.1, seed=7, codes="neumann"),
+    dict(nel=(2, 3, 2), lx1=5, jitter=0.1, seed=8, codes="neumann"),
 ]
 
 

@@ -194,3 +194,36 @@ def test_planted_krylov_mistakes_fail():
         _, _, hg_o, _ = omg.gmres(rl.copy(), h1, h2, 200, -1e-10, m=30)
     k = min(len(hg_o), len(hg_d))
     assert len(hg_o) != len(hg_d) or np.max(np.abs(hg_o[:k] / hg_d[:k] - 1)) > 1e-3
+
+
+def test_pressure_null_space_equals_dense():
+    """No Dirichlet face and h2 = 0 (the reference's ifvcor): the coarse
+    problem holds one unknown at zero and removes the mean of its solution
+    (crs_xxt.c:883-955), and the flexible CG and GMRES end with ortho
+    (hmholtz.f:2272; gmres.f:427, 534).  Oracle and dense restatements agree
+    iterate by iterate on a consistent right-hand side; the solution's
+    local-node mean is zero."""
+    rb, P, dm, omg = build(nel=(3, 2, 2), lx1=6, deform=0.1, seed=13, codes="neumann")
+    lev = dm.lev[-1]
+    assert dm.null_space and O.lib().nko_mg_null_space(omg.h) == 1
+    h1, h2 = np.ones(P.shape), np.zeros(P.shape)
+    S = K.System(lev, P.field.axhelm, P.bm1, h1, h2, lev.mask)
+    rhs = lev.assemble(P.bm1 * lev.gather(np.random.default_rng(31).standard_normal(lev.N)))
+    rhs = rhs - np.mean(rhs)  # consistent: orthogonal to the constant
+    count = lev.assemble(np.ones(P.shape))
+    for solver in ("flex_cg", "gmres"):
+        if solver == "flex_cg":
+            x_o, n_o, h_o, d_o = omg.flex_cg(np.ascontiguousarray(lev.gather(rhs)), h1, h2,
+                                             200, -1e-10)
+            x_d, n_d, h_d, d_d = K.flex_cg(S, dm.solve, rhs, 200, -1e-10, null_count=count)
+        else:
+            x_o, n_o, h_o, d_o = omg.gmres(np.ascontiguousarray(lev.gather(rhs)), h1, h2, 200,
+                                           -1e-10, m=30)
+            x_d, n_d, h_d, d_d = K.gmres(S, dm.solve, rhs, 200, -1e-10, m=30,
+                                         null_count=count)
+        assert n_o == n_d, (solver, n_o, n_d)
+        np.testing.assert_allclose(h_o, h_d, rtol=1e-6, err_msg=solver)
+        assert np.linalg.norm(x_o - lev.gather(x_d)) <= 1e-9 * np.linalg.norm(x_d), solver
+        assert abs(np.sum(x_o)) <= 1e-12 * np.sum(np.abs(x_o)), solver
+        # the residual of the returned solution meets the tolerance
+        assert np.sqrt(np.sum((rhs - S.Am(x_d)) ** 2) / S.vol) <= 1e-10 * d_d * 1.01, solver
