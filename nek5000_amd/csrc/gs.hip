@@ -5,15 +5,21 @@
 // the same id; glo_num == 0 nodes are untouched (setvert3d gives element
 // interiors id 0, navier8.f:2337-2347).
 //
-// Layout: at setup the nonzero ids are sorted (id, local index) on the host;
-// groups of >= 2 members become segments (off/idx, int32, ordered by id so
-// that the segments of one face or edge are adjacent), unpartnered ids become
-// `single`.  One thread reduces one segment in index order, so results are
-// deterministic and independent of the launch shape; no atomics are needed
-// because every segment is owned by exactly one thread.
+// Layout: at setup the local nodes are grouped by nonzero id on the host
+// (off/idx, int32; members in increasing local index; groups in the order of
+// their first member, so that neighbouring groups lie in the same few
+// elements).  Runs of whole groups of at most 64 members form packets, one
+// wavefront each: lane l takes member l of the packet (one coalesced index
+// load, one gather), and a group's sum is formed across its lanes by
+// cross-lane reads in member order.  Every group is owned by one wavefront
+// and summed in member order, so results are deterministic, independent of
+// the launch shape and need no atomics.  Groups of more than 64 members
+// (none on conforming hexahedral meshes of valence <= 64) are kept after the
+// packets and reduced one thread per group.
 #include <algorithm>
 #include <climits>
 #include <numeric>
+#include <type_traits>
 #include <vector>
 
 #include "nekamd_internal.h"
@@ -86,6 +92,40 @@ int gs_build(GsMap &gs, const int64_t *glo, int64_t n, int np, int64_t nel) {
     }
   }
   off.push_back((int32_t)idx.size());
+  // groups of more than 64 members after all others; then the packets
+  int64_t nbig = 0;
+  for (size_t g = 0; g + 1 < off.size(); ++g) nbig += (off[g + 1] - off[g] > 64);
+  if (nbig > 0) {
+    std::vector<int32_t> off2, idx2;
+    off2.reserve(off.size());
+    idx2.reserve(idx.size());
+    for (int pass = 0; pass < 2; ++pass)
+      for (size_t g = 0; g + 1 < off.size(); ++g) {
+        const int32_t a = off[g], c = off[g + 1];
+        if ((c - a > 64) != (pass == 1)) continue;
+        off2.push_back((int32_t)idx2.size());
+        idx2.insert(idx2.end(), idx.begin() + a, idx.begin() + c);
+      }
+    off2.push_back((int32_t)idx2.size());
+    off.swap(off2);
+    idx.swap(idx2);
+  }
+  const int64_t gpk = (int64_t)off.size() - 1 - nbig;
+  std::vector<int32_t> pk_m, pk_g;
+  std::vector<uint64_t> pk_head;
+  for (int64_t g = 0; g < gpk;) {
+    const int32_t q0 = off[(size_t)g];
+    uint64_t hm = 0;
+    pk_m.push_back(q0);
+    pk_g.push_back((int32_t)g);
+    while (g < gpk && off[(size_t)g + 1] - q0 <= 64) {
+      hm |= 1ull << (off[(size_t)g] - q0);
+      ++g;
+    }
+    pk_head.push_back(hm);
+  }
+  pk_m.push_back(off[(size_t)gpk]);
+  pk_g.push_back((int32_t)gpk);
   // structural check used by the fused CG kernels: ids are zero exactly at
   // element-interior nodes (then those nodes can be finished inside axhelm)
   bool iz = (n == (int64_t)np * np * np * nel);
@@ -105,12 +145,20 @@ int gs_build(GsMap &gs, const int64_t *glo, int64_t n, int np, int64_t nel) {
   gs.nsingle = nsingle;
   gs.interior_zero = iz;
   gs.max_mult = maxm;
-  NK_HIP(hipMalloc(&gs.off, sizeof(int32_t) * off.size()));
-  NK_HIP(hipMemcpy(gs.off, off.data(), sizeof(int32_t) * off.size(), hipMemcpyHostToDevice));
-  if (!idx.empty()) {
-    NK_HIP(hipMalloc(&gs.idx, sizeof(int32_t) * idx.size()));
-    NK_HIP(hipMemcpy(gs.idx, idx.data(), sizeof(int32_t) * idx.size(), hipMemcpyHostToDevice));
-  }
+  gs.npk = (int64_t)pk_head.size();
+  gs.gpk = gpk;
+  auto up = [](auto **dst, const auto &v) -> int {
+    using T = typename std::decay_t<decltype(v)>::value_type;
+    NK_HIP(hipMalloc(dst, sizeof(T) * std::max<size_t>(v.size(), 1)));
+    if (!v.empty())
+      NK_HIP(hipMemcpy(*dst, v.data(), sizeof(T) * v.size(), hipMemcpyHostToDevice));
+    return NKH_OK;
+  };
+  NK_TRY(up(&gs.off, off));
+  if (!idx.empty()) NK_TRY(up(&gs.idx, idx));
+  NK_TRY(up(&gs.pk_m, pk_m));
+  NK_TRY(up(&gs.pk_g, pk_g));
+  NK_TRY(up(&gs.pk_head, pk_head));
   return NKH_OK;
 }
 
@@ -118,6 +166,9 @@ void gs_free(GsMap &gs) {
   if (gs.flag) (void)hipFree(gs.flag);
   if (gs.off) (void)hipFree(gs.off);
   if (gs.idx) (void)hipFree(gs.idx);
+  if (gs.pk_m) (void)hipFree(gs.pk_m);
+  if (gs.pk_g) (void)hipFree(gs.pk_g);
+  if (gs.pk_head) (void)hipFree(gs.pk_head);
   gs = GsMap();
 }
 
@@ -129,10 +180,88 @@ __device__ __forceinline__ double gs_combine(double a, double b) {
   else return b > a ? b : a;
 }
 
+// ---- packets: one wavefront per packet, lane l on member l --------------
+// Packet p of a launch of 4-wavefront blocks; the blocks resident on one XCD
+// (workgroups are dealt to the 8 XCDs round-robin) take neighbouring
+// packets and share their L2 lines.  Wavefront-uniform (held in a scalar
+// register, so the packet's descriptors are scalar loads).
+__device__ __forceinline__ int pk_index() {
+  const int nb = gridDim.x, b = blockIdx.x;
+  const int lb = (b % 8) * (nb / 8) + b / 8;
+  return __builtin_amdgcn_readfirstlane(lb * 4 + (int)(threadIdx.x >> 6));
+}
+
+// This lane's member of packet p: node index, first lane of its group,
+// group length (0 past the packet's members) and group rank in the packet.
+struct PkLane {
+  int node, head, len, local;
+  bool act;
+};
+__device__ __forceinline__ PkLane pk_lane(const int32_t *__restrict__ pk_m,
+                                          const uint64_t *__restrict__ pk_head,
+                                          const int32_t *__restrict__ idx, int p) {
+  const int lane = threadIdx.x & 63;
+  const int q0 = pk_m[p], cnt = pk_m[p + 1] - q0;
+  const uint64_t hm = pk_head[p];  // bit 0 is always set
+  PkLane L;
+  L.act = lane < cnt;
+  L.node = L.act ? idx[q0 + lane] : 0;
+  const uint64_t upto = lane == 63 ? ~0ull : ((2ull << lane) - 1);
+  L.head = 63 - __clzll((long long)(hm & upto));
+  const uint64_t after = L.head == 63 ? 0ull : (hm & ~((2ull << L.head) - 1));
+  const int end = after ? __ffsll((long long)after) - 1 : cnt;
+  L.len = L.act ? end - L.head : 0;
+  L.local = __popcll(hm & ((1ull << L.head) - 1));
+  return L;
+}
+
+// The group's combination of v in member order (first member, then each
+// next one), on every lane of the group.  The loop runs to the longest
+// group of the wavefront.
+template <typename T, typename F>
+__device__ __forceinline__ T pk_reduce(T v, const PkLane &L, F comb) {
+  T s = __shfl(v, L.head, 64);
+  for (int k = 1; __ballot(k < L.len) != 0; ++k) {
+    const T t = __shfl(v, L.head + k, 64);
+    if (k < L.len) s = comb(s, t);
+  }
+  return s;
+}
+
+template <int OP>
+struct GsComb {
+  __device__ double operator()(double a, double b) const {
+    if constexpr (OP == 0 || OP == 4) return a + b;
+    else if constexpr (OP == 1) return a * b;
+    else if constexpr (OP == 2) return b < a ? b : a;
+    else return b > a ? b : a;
+  }
+};
+struct AddComb {
+  template <typename T>
+  __device__ T operator()(T a, T b) const { return a + b; }
+};
+
+template <int OP>
+__global__ void __launch_bounds__(256)
+    k_gs_op_pk(double *__restrict__ u, const int32_t *__restrict__ pk_m,
+               const uint64_t *__restrict__ pk_head, const int32_t *__restrict__ idx,
+               int64_t npk) {
+  const int p = pk_index();
+  if (p >= npk) return;
+  const PkLane L = pk_lane(pk_m, pk_head, idx, p);
+  const double v = L.act ? u[L.node] : 0.0;
+  double acc = pk_reduce(v, L, GsComb<OP>());
+  // op 4 (dsavg, ic.f:1871-1897): the sum times vmult = 1/multiplicity
+  if constexpr (OP == 4) acc = acc * (1.0 / (double)L.len);
+  if (L.len >= 2) u[L.node] = acc;
+}
+
+// one thread per group, groups [g0, ng) (those of more than 64 members)
 template <int OP>
 __global__ void k_gs_op(double *__restrict__ u, const int32_t *__restrict__ off,
-                        const int32_t *__restrict__ idx, int64_t nseg) {
-  for (int64_t s = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; s < nseg;
+                        const int32_t *__restrict__ idx, int64_t g0, int64_t nseg) {
+  for (int64_t s = g0 + blockIdx.x * (int64_t)blockDim.x + threadIdx.x; s < nseg;
        s += (int64_t)gridDim.x * blockDim.x) {
     const int32_t a = off[s], b = off[s + 1];
     if (b - a < 2) continue;
@@ -144,17 +273,32 @@ __global__ void k_gs_op(double *__restrict__ u, const int32_t *__restrict__ off,
   }
 }
 
+// blocks of 4 packets, a multiple of 8 (the XCD mapping of pk_index)
+static int pk_blocks(int64_t npk) {
+  const int64_t b = (npk + 3) / 4;
+  return (int)((b + 7) / 8 * 8);
+}
+static_assert(sizeof(int) == 4, "packet indices are int32");
+
+template <int OP>
+static void gs_op_launch(const GsMap &gs, double *u, hipStream_t s) {
+  if (gs.npk > 0)
+    hipLaunchKernelGGL(k_gs_op_pk<OP>, dim3(pk_blocks(gs.npk)), dim3(256), 0, s, u, gs.pk_m,
+                       gs.pk_head, gs.idx, gs.npk);
+  const int64_t nbig = gs.ngroups - gs.gpk;
+  if (nbig > 0)
+    hipLaunchKernelGGL(k_gs_op<OP>, dim3((int)std::min<int64_t>((nbig + 255) / 256, 65536)),
+                       dim3(256), 0, s, u, gs.off, gs.idx, gs.gpk, gs.ngroups);
+}
+
 int gs_op(const GsMap &gs, double *u, int op, hipStream_t s) {
   if (gs.nseg == 0) return NKH_OK;
-  const int64_t ng = gs.ngroups;
-  int grid = (int)((ng + 255) / 256);
-  if (grid > 65536) grid = 65536;
   switch (op) {
-    case 0: hipLaunchKernelGGL(k_gs_op<0>, dim3(grid), dim3(256), 0, s, u, gs.off, gs.idx, ng); break;
-    case 1: hipLaunchKernelGGL(k_gs_op<1>, dim3(grid), dim3(256), 0, s, u, gs.off, gs.idx, ng); break;
-    case 2: hipLaunchKernelGGL(k_gs_op<2>, dim3(grid), dim3(256), 0, s, u, gs.off, gs.idx, ng); break;
-    case 3: hipLaunchKernelGGL(k_gs_op<3>, dim3(grid), dim3(256), 0, s, u, gs.off, gs.idx, ng); break;
-    case 4: hipLaunchKernelGGL(k_gs_op<4>, dim3(grid), dim3(256), 0, s, u, gs.off, gs.idx, ng); break;
+    case 0: gs_op_launch<0>(gs, u, s); break;
+    case 1: gs_op_launch<1>(gs, u, s); break;
+    case 2: gs_op_launch<2>(gs, u, s); break;
+    case 3: gs_op_launch<3>(gs, u, s); break;
+    case 4: gs_op_launch<4>(gs, u, s); break;
     default: set_error("gs_op: unknown op %d", op); return NKH_ERR_ARG;
   }
   NK_HIP(hipGetLastError());
@@ -169,13 +313,31 @@ int gs_op(const GsMap &gs, double *u, int op, hipStream_t s) {
 // and share their L2 lines.
 template <typename T>
 __global__ void __launch_bounds__(256)
+    k_gs_assemble_pk(T *__restrict__ w, const int32_t *__restrict__ pk_m,
+                     const int32_t *__restrict__ pk_g, const uint64_t *__restrict__ pk_head,
+                     const int32_t *__restrict__ idx, const uint8_t *__restrict__ flag,
+                     int64_t npk, const CgState *__restrict__ st) {
+  if (st && st->done) return;
+  const int p = pk_index();
+  if (p >= npk) return;
+  const PkLane L = pk_lane(pk_m, pk_head, idx, p);
+  const T v = L.act ? w[L.node] : T(0);
+  const uint8_t f = L.act ? flag[pk_g[p] + L.local] : uint8_t(2);
+  const T acc = pk_reduce(v, L, AddComb());
+  if (f == 1) w[L.node] = T(0);
+  else if (f == 0) w[L.node] = acc;
+}
+
+// one thread per group, groups [g0, ngroups) in chunks
+template <typename T>
+__global__ void __launch_bounds__(256)
     k_gs_assemble(T *__restrict__ w, const int32_t *__restrict__ off,
                   const int32_t *__restrict__ idx, const uint8_t *__restrict__ flag,
-                  int64_t ngroups, int64_t chunk, const CgState *__restrict__ st) {
+                  int64_t g0, int64_t ngroups, int64_t chunk, const CgState *__restrict__ st) {
   if (st && st->done) return;
   const int nb = gridDim.x, b = blockIdx.x;
   const int64_t lb = (int64_t)(b % 8) * (nb / 8) + b / 8;
-  const int64_t s0 = lb * chunk;
+  const int64_t s0 = g0 + lb * chunk;
   const int64_t s1 = (s0 + chunk < ngroups) ? s0 + chunk : ngroups;
   for (int64_t s = s0 + threadIdx.x; s < s1; s += blockDim.x) {
     const uint8_t f = flag[s];
@@ -198,12 +360,29 @@ __global__ void __launch_bounds__(256)
 // 1/multiplicity).  Every id group, single members included.
 template <typename E, typename Z, typename W>
 __global__ void __launch_bounds__(256)
+    k_gs_fin_pk(Z *__restrict__ z, const E *__restrict__ e, const W *__restrict__ w,
+                const int32_t *__restrict__ pk_m, const uint64_t *__restrict__ pk_head,
+                const int32_t *__restrict__ idx, int64_t npk) {
+  const int p = pk_index();
+  if (p >= npk) return;
+  const PkLane L = pk_lane(pk_m, pk_head, idx, p);
+  const E ev = L.act ? e[L.node] : E(0);
+  const double wv = L.act ? double(w[L.node]) : 0.0;
+  const double zv = L.act ? double(z[L.node]) : 0.0;
+  const double S = double(pk_reduce(ev, L, AddComb()));
+  double acc = pk_reduce(S * wv + zv, L, AddComb());
+  acc = acc * (1.0 / (double)L.len);
+  if (L.act) z[L.node] = Z(acc);
+}
+
+template <typename E, typename Z, typename W>
+__global__ void __launch_bounds__(256)
     k_gs_fin(Z *__restrict__ z, const E *__restrict__ e, const W *__restrict__ w,
-             const int32_t *__restrict__ off, const int32_t *__restrict__ idx, int64_t ngroups,
-             int64_t chunk) {
+             const int32_t *__restrict__ off, const int32_t *__restrict__ idx, int64_t g0,
+             int64_t ngroups, int64_t chunk) {
   const int nb = gridDim.x, b = blockIdx.x;
   const int64_t lb = (int64_t)(b % 8) * (nb / 8) + b / 8;
-  const int64_t s0 = lb * chunk;
+  const int64_t s0 = g0 + lb * chunk;
   const int64_t s1 = (s0 + chunk < ngroups) ? s0 + chunk : ngroups;
   for (int64_t g = s0 + threadIdx.x; g < s1; g += blockDim.x) {
     const int32_t a = off[g], c = off[g + 1];
@@ -277,14 +456,22 @@ static int gs_blocks(int64_t ngroups) {
   return (int)(b < 64 ? 64 : (b > kGsBlocks ? kGsBlocks : b));
 }
 
+// nblocks (> 0): grid of the per-group kernel of the large groups
 template <typename T>
 static int gs_assemble_t(const GsMap &gs, T *w, const CgState *st, hipStream_t s,
                          const uint8_t *flag, int nblocks = 0) {
   if (gs.ngroups == 0) return NKH_OK;
-  const int nb = nblocks > 0 ? nblocks : gs_blocks(gs.ngroups);
-  const int64_t chunk = (gs.ngroups + nb - 1) / nb;
-  hipLaunchKernelGGL(k_gs_assemble<T>, dim3(nb), dim3(256), 0, s, w, gs.off, gs.idx,
-                     flag ? flag : gs.flag, gs.ngroups, chunk, st);
+  const uint8_t *fl = flag ? flag : gs.flag;
+  if (gs.npk > 0)
+    hipLaunchKernelGGL(k_gs_assemble_pk<T>, dim3(pk_blocks(gs.npk)), dim3(256), 0, s, w,
+                       gs.pk_m, gs.pk_g, gs.pk_head, gs.idx, fl, gs.npk, st);
+  const int64_t nbig = gs.ngroups - gs.gpk;
+  if (nbig > 0) {
+    const int nb = nblocks > 0 ? nblocks : gs_blocks(nbig);
+    const int64_t chunk = (nbig + nb - 1) / nb;
+    hipLaunchKernelGGL(k_gs_assemble<T>, dim3(nb), dim3(256), 0, s, w, gs.off, gs.idx, fl,
+                       gs.gpk, gs.ngroups, chunk, st);
+  }
   NK_HIP(hipGetLastError());
   return NKH_OK;
 }
@@ -302,10 +489,16 @@ int gs_assemble(const GsMap &gs, float *w, const CgState *st, hipStream_t s,
 template <typename E, typename Z, typename W>
 static int gs_fin_t(const GsMap &gs, Z *z, const E *e, const W *w, hipStream_t s) {
   if (gs.ngroups == 0) return NKH_OK;
-  const int nb = gs_blocks(gs.ngroups);
-  const int64_t chunk = (gs.ngroups + nb - 1) / nb;
-  hipLaunchKernelGGL((k_gs_fin<E, Z, W>), dim3(nb), dim3(256), 0, s, z, e, w, gs.off, gs.idx,
-                     gs.ngroups, chunk);
+  if (gs.npk > 0)
+    hipLaunchKernelGGL((k_gs_fin_pk<E, Z, W>), dim3(pk_blocks(gs.npk)), dim3(256), 0, s, z, e,
+                       w, gs.pk_m, gs.pk_head, gs.idx, gs.npk);
+  const int64_t nbig = gs.ngroups - gs.gpk;
+  if (nbig > 0) {
+    const int nb = gs_blocks(nbig);
+    const int64_t chunk = (nbig + nb - 1) / nb;
+    hipLaunchKernelGGL((k_gs_fin<E, Z, W>), dim3(nb), dim3(256), 0, s, z, e, w, gs.off,
+                       gs.idx, gs.gpk, gs.ngroups, chunk);
+  }
   NK_HIP(hipGetLastError());
   return NKH_OK;
 }

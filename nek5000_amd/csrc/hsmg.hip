@@ -21,6 +21,7 @@
 // beside it, and join before the last prolongation.
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <type_traits>
 #include <vector>
@@ -867,16 +868,29 @@ void launch_sch_fdm_mx(float *wint, float *wgh, const float *S, const float *lam
                      L.gin, L.gout, S, lam, nel);
 }
 
+// NEKAMD_OVERLAP_FUSED=1 (read once) forms the fine level's overlap sum
+// inside the prolongation and the final gather-scatter (k_tnsr MODE 4,
+// k_gs_fin_ov) instead of in its own pass before the join.  Measured at
+// E = 64^3, N = 7: 235.7 ms per FP64 solve fused, 232.0 separate (mixed
+// 156.3 / 150.8): the fused kernels move work behind the coarse chain.
+static bool overlap_fused() {
+  static const bool v = [] {
+    const char *e = std::getenv("NEKAMD_OVERLAP_FUSED");
+    return e && std::atoi(e) != 0;
+  }();
+  return v;
+}
+
 template <int NL, typename R, typename RI>
 void launch_sch_fdm(R *wint, R *wgh, const R *S, const R *lam, const MgLevel &L, const RI *r,
                     int64_t nel, hipStream_t s) {
   const int grid = (int)std::min<int64_t>((nel + 7) / 8 * 8, 1 << 20);
   if constexpr (std::is_same<R, double>::value && std::is_same<RI, double>::value)
-    hipLaunchKernelGGL((k_sch_fdm_d<NL>), dim3(grid), dim3(Fdm<NL>::kThreads), 0, s, wint, wgh,
-                       r, L.gin, L.gout, S, lam, nel);
-  else
-    hipLaunchKernelGGL((k_sch_fdm<NL, R, RI>), dim3(grid), dim3(Fdm<NL>::kThreads), 0, s, wint,
+    hipLaunchKernelGGL((k_sch_fdm_d<NL>), dim3(grid), dim3(Fdm<NL>::kThreads), 0, s, wint,
                        wgh, r, L.gin, L.gout, S, lam, nel);
+  else
+    hipLaunchKernelGGL((k_sch_fdm<NL, R, RI>), dim3(grid), dim3(Fdm<NL>::kThreads), 0, s,
+                       wint, wgh, r, L.gin, L.gout, S, lam, nel);
 }
 
 }  // namespace
@@ -1088,12 +1102,15 @@ static int mg_solve_t(Mesh &mesh, FT *z, const FT *r, hipStream_t s, int crs_mod
   // otherwise dssum now and the plain sequence after the join.
   MgLevel &T = mg.lev[Lm];
   const bool merged = mesh.gs.interior_zero;
+  const bool fuse_ov = merged && overlap_fused();
   const bool timed = timers().on;
   if (timed) NK_HIP(hipEventRecord(mg.tev[2], s));
   NK_CHECK(merged || !f32io, NKH_ERR_UNSUPPORTED,
            "single-precision V-cycle needs global ids zero exactly at element interiors");
   const bool fp32 = (precision == 1 || f32io) && merged;
   if (fp32) NK_TRY(mg_prepare_fp32(mg, s, f32io));
+  if (fp32 && !fuse_ov && !T.e32)  // single-precision overlap sum, formed on its own
+    NK_HIP(hipMalloc(&T.e32, sizeof(float) * std::max<int64_t>(T.n, 1)));
   // The streaming head of the coarse chain runs first, on s: the
   // restriction of the fine residual, its dssum and (when that level is 2)
   // the restriction to level 1.  Beside the fine local solves these
@@ -1123,16 +1140,21 @@ static int mg_solve_t(Mesh &mesh, FT *z, const FT *r, hipStream_t s, int crs_mod
   }
   NK_HIP(hipEventRecord(mg.ev_fork, s));
   NK_HIP(hipStreamWaitEvent(b, mg.ev_fork, 0));
-  // merged: the overlap sum is formed inside the prolongation and the final
-  // gather-scatter (k_tnsr MODE 4, k_gs_fin_ov) from the local solutions
-  if (timed) NK_HIP(hipEventRecord(mg.tev[0], s));
+  // the local solves and their overlap sum (into T.e / T.e32; with
+  // fuse_ov the sum is formed after the join from the local solutions)
+  // (on sf when the compute units are split, joined back before the end)
+  hipStream_t f = mg.sf ? mg.sf : s;
+  if (mg.sf) NK_HIP(hipStreamWaitEvent(f, mg.ev_fork, 0));
+  if (timed) NK_HIP(hipEventRecord(mg.tev[0], f));
   if (fp32) {
-    NK_TRY((schwarz_local<float, FT>(mg, Lm, r, (float *)nullptr, s)));
+    NK_TRY((schwarz_local<float, FT>(mg, Lm, r, fuse_ov ? (float *)nullptr : T.e32, f)));
   } else if constexpr (!f32io) {
-    NK_TRY((schwarz_local<double, FT>(mg, Lm, r, merged ? (double *)nullptr : z, s)));
-    if (!merged) NK_TRY(gs_op(T.gs, z, 0, s));
+    NK_TRY((schwarz_local<double, FT>(mg, Lm, r,
+                                      fuse_ov ? (double *)nullptr : (merged ? T.e : z), f)));
+    if (!merged) NK_TRY(gs_op(T.gs, z, 0, f));
   }
-  if (timed) NK_HIP(hipEventRecord(mg.tev[1], s));
+  if (timed) NK_HIP(hipEventRecord(mg.tev[1], f));
+  if (mg.sf) NK_HIP(hipEventRecord(mg.ev_fj, f));
   // coarser levels on b: restrict, dssum, mask; the Schwarz solve of each
   // intermediate level on c (it only reads the masked residual), beside the
   // restriction to level 1 and the coarse solve on b
@@ -1169,9 +1191,10 @@ static int mg_solve_t(Mesh &mesh, FT *z, const FT *r, hipStream_t s, int crs_mod
   }
   NK_HIP(hipEventRecord(mg.ev_join, b));
   NK_HIP(hipStreamWaitEvent(s, mg.ev_join, 0));
+  if (mg.sf) NK_HIP(hipStreamWaitEvent(s, mg.ev_fj, 0));
   // z = dssum(e)*(mask*wt) + J e_{lmax-1}, then dsavg (ic.f:1871-1897)
   MgLevel &C = mg.lev[Lm - 1];
-  if (fp32) {
+  if (fp32 && fuse_ov) {
     if constexpr (f32io) {
       NK_TRY((run_tnsr<4, float, double, FT, float>(z, T.nh, C.e, C.nh, T.mswt32, C.J, nel, s,
                                                     T.wint32, nullptr, T.wgh32)));
@@ -1181,10 +1204,22 @@ static int mg_solve_t(Mesh &mesh, FT *z, const FT *r, hipStream_t s, int crs_mod
                                              nullptr, T.wgh32)));
       NK_TRY((gs_fin_ov<float, float>(mesh.gs, T.nh, z, T.wint32, T.wgh32, T.mswt, s)));
     }
+  } else if (fp32) {
+    if constexpr (f32io) {
+      NK_TRY((run_tnsr<3, float, double, FT, float>(z, T.nh, C.e, C.nh, T.mswt32, C.J, nel, s,
+                                                    T.e32)));
+      NK_TRY(gs_fin(mesh.gs, z, T.e32, T.mswt32, s));
+    } else {
+      NK_TRY((run_tnsr<3, float, double, FT>(z, T.nh, C.e, C.nh, T.mswt, C.J, nel, s, T.e32)));
+      NK_TRY(gs_fin(mesh.gs, z, T.e32, T.mswt, s));
+    }
   } else if constexpr (!f32io) {
-    if (merged) {
+    if (fuse_ov) {
       NK_TRY(run_tnsr<4>(z, T.nh, C.e, C.nh, T.mswt, C.J, nel, s, T.wint, nullptr, T.wgh));
       NK_TRY((gs_fin_ov<double, double>(mesh.gs, T.nh, z, T.wint, T.wgh, T.mswt, s)));
+    } else if (merged) {
+      NK_TRY(run_tnsr<3>(z, T.nh, C.e, C.nh, T.mswt, C.J, nel, s, T.e));
+      NK_TRY(gs_fin(mesh.gs, z, T.e, T.mswt, s));
     } else {
       NK_TRY(run_tnsr<2>(z, T.nh, C.e, C.nh, T.mswt, C.J, nel, s));
       NK_TRY(gs_op(mesh.gs, z, 4, s));

@@ -75,6 +75,12 @@ struct GsMap {
   uint8_t *flag = nullptr;    // device, ngroups (gs_prepare)
   bool interior_zero = false; // glo_num == 0 exactly at element-interior nodes
   int max_mult = 0;
+  // packets: runs of whole groups of at most 64 members, one wavefront each
+  // (groups [0, gpk)); groups [gpk, ngroups) have more than 64 members
+  int64_t npk = 0, gpk = 0;
+  int32_t *pk_m = nullptr;    // device, npk+1: first member of each packet
+  int32_t *pk_g = nullptr;    // device, npk+1: first group of each packet
+  uint64_t *pk_head = nullptr;  // device, npk: bit l set where a group starts at lane l
 };
 
 // State of one PCG solve, resident on the device.  The scalar kernels update
