@@ -57,10 +57,6 @@ struct Mg {
   hipStream_t sb = nullptr;  // second stream: coarse levels beside the fine one
   hipStream_t sc = nullptr;  // third: intermediate-level Schwarz beside the coarse solve
   hipEvent_t ev_fork = nullptr, ev_join = nullptr;
-  // with a compute-unit split: the fine-level local solves on sf, on the
-  // complement of the coarse stream's compute units
-  hipStream_t sf = nullptr;
-  hipEvent_t ev_fj = nullptr;
   hipEvent_t ev_lr[4] = {}, ev_le[4] = {};  // per level: r ready, e ready
   // timing events (nkh_timers): fine local solves start/end, V-cycle start/end
   hipEvent_t tev[4] = {};
@@ -99,7 +95,6 @@ int mg_grid(int64_t n);
 int mg_prepare_fp32(Mg &mg, hipStream_t s, bool inner);
 // navier1.f:223-255 ORTHO: v <- v - sum(v)/n over the n local nodes (device)
 int mg_ortho(Mg &mg, double *v, int64_t n, hipStream_t s);
-int mg_create_streams(Mg &mg);
 // the constant null space is present for this solve (no Dirichlet face, h2 = 0)
 inline bool mg_null_space_on(const Mg &mg, Coef h2) {
   return mg.null_space && h2.uniform() && h2.s == 0.0;

@@ -514,32 +514,58 @@ __device__ __forceinline__ R overlap_value(const R *__restrict__ wint,
   return a;
 }
 
-// Overlap sum of every node into e (the path without the fused kernels).
-// Blocks of 64 walk elements; the element size N is a compile-time constant
-// so the node coordinates cost no divisions, and each lane issues all its
-// loads of an element before using them.
+// Overlap sum of every node into e (the path without the fused kernels),
+// the sum of overlap_value in the same order.  One wavefront per element:
+// the interior values and the six ghost faces are loaded coalesced and
+// unconditionally (one round trip), the faces staged in LDS, where the
+// first-interior-layer nodes pick up their neighbour values.
 template <typename R, int N, typename EO = R>
 __global__ void __launch_bounds__(64)
     k_sch_out(EO *__restrict__ eo, const R *__restrict__ wint, const R *__restrict__ wgh,
               int64_t nel) {
-  constexpr int N2 = N * N, N3 = N2 * N, U = (N3 + 63) / 64;
+  constexpr int N2 = N * N, N3 = N2 * N, U = (N3 + 63) / 64, G = 6 * N2, UG = (G + 63) / 64;
+  __shared__ R sg[G];
+  const int l = (int)threadIdx.x;
   for (int64_t e = blockIdx.x; e < nel; e += gridDim.x) {
+    R w[U], gv[UG];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int q = l + 64 * u;
+      w[u] = q < N3 ? wint[e * N3 + q] : R(0);
+    }
+#pragma unroll
+    for (int u = 0; u < UG; ++u) {
+      const int q = l + 64 * u;
+      gv[u] = q < G ? wgh[e * G + q] : R(0);
+    }
+    __syncthreads();  // the previous element's reads of sg are done
+#pragma unroll
+    for (int u = 0; u < UG; ++u) {
+      const int q = l + 64 * u;
+      if (q < G) sg[q] = gv[u];
+    }
+    __syncthreads();
     EO *out = eo + e * N3;
 #pragma unroll
     for (int u = 0; u < U; ++u) {
-      const int q = (int)threadIdx.x + 64 * u;
+      const int q = l + 64 * u;
       if (q >= N3) continue;
       const int i = q % N, j = (q / N) % N, k = q / N2;
-      out[q] = EO(overlap_value<N, R>(wint, wgh, e, i, j, k));
+      R a = w[u];
+      if (i == 1) a += sg[0 * N2 + j + N * k];
+      if (i == N - 2) a += sg[1 * N2 + j + N * k];
+      if (j == 1) a += sg[2 * N2 + i + N * k];
+      if (j == N - 2) a += sg[3 * N2 + i + N * k];
+      if (k == 1) a += sg[4 * N2 + i + N * j];
+      if (k == N - 2) a += sg[5 * N2 + i + N * j];
+      out[q] = EO(a);
     }
   }
 }
 
 template <typename R, typename EO = R>
 int launch_sch_out(EO *e, const R *wint, const R *wgh, int nh, int64_t nel, hipStream_t s) {
-  // a few elements per block: one small element per block is dominated by
-  // block launch
-  const int grid = (int)std::min<int64_t>(nel, 16384);
+  const int grid = (int)std::min<int64_t>(nel, 1 << 20);
   switch (nh) {
 #define NK_O(N)                                                                             \
   case N:                                                                                   \
@@ -1142,9 +1168,7 @@ static int mg_solve_t(Mesh &mesh, FT *z, const FT *r, hipStream_t s, int crs_mod
   NK_HIP(hipStreamWaitEvent(b, mg.ev_fork, 0));
   // the local solves and their overlap sum (into T.e / T.e32; with
   // fuse_ov the sum is formed after the join from the local solutions)
-  // (on sf when the compute units are split, joined back before the end)
-  hipStream_t f = mg.sf ? mg.sf : s;
-  if (mg.sf) NK_HIP(hipStreamWaitEvent(f, mg.ev_fork, 0));
+  hipStream_t f = s;
   if (timed) NK_HIP(hipEventRecord(mg.tev[0], f));
   if (fp32) {
     NK_TRY((schwarz_local<float, FT>(mg, Lm, r, fuse_ov ? (float *)nullptr : T.e32, f)));
@@ -1154,7 +1178,6 @@ static int mg_solve_t(Mesh &mesh, FT *z, const FT *r, hipStream_t s, int crs_mod
     if (!merged) NK_TRY(gs_op(T.gs, z, 0, f));
   }
   if (timed) NK_HIP(hipEventRecord(mg.tev[1], f));
-  if (mg.sf) NK_HIP(hipEventRecord(mg.ev_fj, f));
   // coarser levels on b: restrict, dssum, mask; the Schwarz solve of each
   // intermediate level on c (it only reads the masked residual), beside the
   // restriction to level 1 and the coarse solve on b
@@ -1191,7 +1214,6 @@ static int mg_solve_t(Mesh &mesh, FT *z, const FT *r, hipStream_t s, int crs_mod
   }
   NK_HIP(hipEventRecord(mg.ev_join, b));
   NK_HIP(hipStreamWaitEvent(s, mg.ev_join, 0));
-  if (mg.sf) NK_HIP(hipStreamWaitEvent(s, mg.ev_fj, 0));
   // z = dssum(e)*(mask*wt) + J e_{lmax-1}, then dsavg (ic.f:1871-1897)
   MgLevel &C = mg.lev[Lm - 1];
   if (fp32 && fuse_ov) {

@@ -3,7 +3,6 @@
 // set_up_h1_crs), host side with device kernels for the per-node weights and
 // the Galerkin coarse matrices.
 #include <algorithm>
-#include <cstdlib>
 #include <cmath>
 #include <cstring>
 #include <thread>
@@ -528,49 +527,11 @@ static void free_level(MgLevel &L) {
   if (L.own_gs) gs_free(L.gs);
 }
 
-// Streams of the V-cycle.  NEKAMD_CU_SPLIT=k (k = 4, 8, 12 or 16; read
-// once) gives the coarse stream k compute units of each XCD and the fine and
-// intermediate levels the others (a device of 256 compute units in 8 XCDs).
-// Mask bit i is the coarse stream's when (i mod 8 - i div 32) mod 8 < k/4:
-// k per XCD whether the driver deals bit i to XCD i mod 8, to XCD i div 32,
-// or applies the first 32 bits to every XCD.
-static int cu_split() {
-  static const int v = [] {
-    const char *e = std::getenv("NEKAMD_CU_SPLIT");
-    const int k = e ? std::atoi(e) : 0;
-    return (k == 4 || k == 8 || k == 12 || k == 16) ? k : 0;
-  }();
-  return v;
-}
-
-int mg_create_streams(Mg &mg) {
-  int dev = 0, ncu = 0;
-  NK_HIP(hipGetDevice(&dev));
-  NK_HIP(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
-  const int k = cu_split();
-  if (k > 0 && ncu == 256) {
-    uint32_t small[8] = {}, big[8] = {};
-    for (int i = 0; i < 256; ++i) {
-      const bool mine = ((i % 8) - (i / 32) + 8) % 8 < k / 4;
-      (mine ? small : big)[i / 32] |= 1u << (i % 32);
-    }
-    NK_HIP(hipExtStreamCreateWithCUMask(&mg.sb, 8, small));
-    NK_HIP(hipExtStreamCreateWithCUMask(&mg.sc, 8, big));
-    NK_HIP(hipExtStreamCreateWithCUMask(&mg.sf, 8, big));
-    NK_HIP(hipEventCreateWithFlags(&mg.ev_fj, hipEventDisableTiming));
-    return NKH_OK;
-  }
-  NK_HIP(hipStreamCreateWithFlags(&mg.sb, hipStreamNonBlocking));
-  NK_HIP(hipStreamCreateWithFlags(&mg.sc, hipStreamNonBlocking));
-  return NKH_OK;
-}
-
 void mg_free(Mesh &mesh) {
   Mg *mg = mesh.mg;
   if (!mg) return;
   if (mg->sb) (void)hipStreamSynchronize(mg->sb);
   if (mg->sc) (void)hipStreamSynchronize(mg->sc);
-  if (mg->sf) (void)hipStreamSynchronize(mg->sf);
   for (int l = 1; l <= mg->lmax; ++l) free_level(mg->lev[l]);
   mg->crs.release();
   for (void *p : {(void *)mg->fbc, (void *)mg->slot, (void *)mg->goff, (void *)mg->gidx,
@@ -590,10 +551,8 @@ void mg_free(Mesh &mesh) {
     if (mg->ev_lr[l]) (void)hipEventDestroy(mg->ev_lr[l]);
     if (mg->ev_le[l]) (void)hipEventDestroy(mg->ev_le[l]);
   }
-  if (mg->ev_fj) (void)hipEventDestroy(mg->ev_fj);
   if (mg->sb) (void)hipStreamDestroy(mg->sb);
   if (mg->sc) (void)hipStreamDestroy(mg->sc);
-  if (mg->sf) (void)hipStreamDestroy(mg->sf);
   delete mg;
   mesh.mg = nullptr;
 }
@@ -617,7 +576,8 @@ int mg_setup(Mesh &mesh, const int *fbc_h, const int64_t *vertex, const double *
   Mg &mg = *pmg;
   const int64_t nel = m->nel;
   mg.nel = nel;
-  NK_TRY(mg_create_streams(mg));
+  NK_HIP(hipStreamCreateWithFlags(&mg.sb, hipStreamNonBlocking));
+  NK_HIP(hipStreamCreateWithFlags(&mg.sc, hipStreamNonBlocking));
   for (int l = 0; l < 4; ++l) {
     NK_HIP(hipEventCreateWithFlags(&mg.ev_lr[l], hipEventDisableTiming));
     NK_HIP(hipEventCreateWithFlags(&mg.ev_le[l], hipEventDisableTiming));
