@@ -108,6 +108,15 @@ int nkh_re2_header(const char *path, int *nelgt, int *ndim, int *nelgv, int *nfi
    for the first nfields boundary sections.  Any output may be NULL. */
 int nkh_re2_read(const char *path, int nelgt, int ndim, int nfields, int *group, double *xc,
                  double *yc, double *zc, double *curve, char *ccurve, char *cbc, double *bc);
+/* map2.f:712-942 READ_MAP, header: element count and active-element count
+   of the case's .map (text) or .ma2 (binary, either byte order) file, the
+   format chosen by the path's suffix. */
+int nkh_map_header(const char *path, int *nel, int *nactive);
+/* map2.f:712-942 READ_MAP: per element its partition (0-based, as genmap
+   writes it) and its 2^ndim vertex ids, vertex(2^ndim, nel) in the
+   hypercube corner order nkh_setvert3d reads (Nek's vertex array).  nel must
+   equal the file's count.  Either output may be NULL. */
+int nkh_map_read(const char *path, int nel, int ndim, int *part, int64_t *vertex);
 /* Element vertex ids from corner coincidence (what Nek takes from the .ma2
    map, map2.f:712-942, or its parallel connectivity step): corners closer than
    tol_rel times the shortest element edge share an id, periodic faces
@@ -117,10 +126,13 @@ int nkh_re2_read(const char *path, int nelgt, int ndim, int nfields, int *group,
 int nkh_mesh_vertex_ids(int nel, int ndim, const double *xc, const double *yc, const double *zc,
                         const char *cbc, const double *bc, double tol_rel, int64_t *vertex,
                         int64_t *nvert);
-/* genxyz.f:1235-1400 LINQUAD (xyzlin, and xyzquad with gh_face_extend for
-   elements with 'm' midside edges), 3-D: GLL coordinates xm1, ym1, zm1
-   (lx1^3 per element) from the corners and curve data of nkh_re2_read.
-   Other curve types return NKH_ERR_UNSUPPORTED. */
+/* genxyz.f:562-605 GENXYZ, 3-D: GLL coordinates xm1, ym1, zm1 (lx1^3 per
+   element) from the corners and curve data of nkh_re2_read: LINQUAD
+   (:1235-1400, xyzlin, and xyzquad with gh_face_extend for elements with
+   'm' midside edges), then per element the 's' faces (SPHSRF :677-845;
+   curve(1..4) = centre and radius) and the 'C' edges 1..8 (ARCSRF :2-103;
+   curve(1) = signed radius).  'e' faces and other curve types return
+   NKH_ERR_UNSUPPORTED. */
 int nkh_mesh_xyz_from_corners(int lx1, int nel, const double *xc, const double *yc,
                               const double *zc, const char *ccurve, const double *curve,
                               double *xm1, double *ym1, double *zm1);

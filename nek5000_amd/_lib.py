@@ -67,6 +67,8 @@ SIGNATURES = {
     "nkh_dgll": (c_int, [c_void_p, c_void_p, c_void_p, c_int]),
     "nkh_setvert3d": (c_int, [c_void_p, P(c_int64), c_int, c_int, c_void_p]),
     "nkh_re2_header": (c_int, [c_char_p, P(c_int), P(c_int), P(c_int), P(c_int)]),
+    "nkh_map_header": (c_int, [c_char_p, P(c_int), P(c_int)]),
+    "nkh_map_read": (c_int, [c_char_p, c_int, c_int, c_void_p, c_void_p]),
     "nkh_re2_read": (c_int, [c_char_p, c_int, c_int, c_int] + [c_void_p] * 8),
     "nkh_mesh_vertex_ids": (c_int, [c_int, c_int, c_void_p, c_void_p, c_void_p, c_void_p,
                                     c_void_p, c_double, c_void_p, P(c_int64)]),

@@ -1000,6 +1000,12 @@ int nkh_re2_read(const char *path, int nelgt, int ndim, int nfields, int *group,
                  double *yc, double *zc, double *curve, char *ccurve, char *cbc, double *bc) {
   return re2_read(path, nelgt, ndim, nfields, group, xc, yc, zc, curve, ccurve, cbc, bc);
 }
+int nkh_map_header(const char *path, int *nel, int *nactive) {
+  return map_header(path, nel, nactive);
+}
+int nkh_map_read(const char *path, int nel, int ndim, int *part, int64_t *vertex) {
+  return map_read(path, nel, ndim, part, vertex);
+}
 int nkh_mesh_vertex_ids(int nel, int ndim, const double *xc, const double *yc, const double *zc,
                         const char *cbc, const double *bc, double tol_rel, int64_t *vertex,
                         int64_t *nvert) {

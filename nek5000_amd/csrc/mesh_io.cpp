@@ -1,9 +1,11 @@
 // Mesh ingestion for real Nek cases (host code): the .re2 binary mesh reader
-// (reader_re2.f), element connectivity by corner coincidence (what the .ma2
-// map or the parallel connectivity step supplies to Nek as `vertex`), and the
-// GLL coordinates of straight-sided and midside-curved ('m') elements
-// (genxyz.f xyzlin / xyzquad with navier5.f gh_face_extend).
+// (reader_re2.f), the .map/.ma2 partition and vertex file (map2.f read_map),
+// element connectivity by corner coincidence (what the map or the parallel
+// connectivity step supplies to Nek as `vertex`, for cases without a map),
+// and the GLL coordinates of straight-sided, midside-curved ('m'),
+// circular-arc ('C') and spherical-face ('s') elements (genxyz.f).
 #include <algorithm>
+#include <cctype>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -223,6 +225,138 @@ int re2_read(const char *path, int nelgt, int ndim, int nfields, int *group, dou
   return NKH_OK;
 }
 
+// ---- map files -----------------------------------------------------------
+// map2.f:712-942 read_map: the case's .map (text) or .ma2 (binary) file
+// from genmap: header (version, element count, active count), then per
+// element its partition (0-based) and its 2^ndim vertex ids in the
+// hypercube (lexicographic) corner order (genmap.f:1641-1712 dmp_mapfile,
+// genmap.f:362 h2s).  .ma2: a 132-byte text header read as (a5, 2i12), the
+// real*4 6.54321 that gives the byte order, then (1 + nv) int32 per element.
+namespace {
+
+bool has_suffix(const char *path, const char *suf) {
+  const size_t n = std::strlen(path), m = std::strlen(suf);
+  return n >= m && std::strcmp(path + n - m, suf) == 0;
+}
+
+struct MapFile {
+  int nel = 0, nactive = 0;
+  bool binary = false, swap = false;
+  std::vector<unsigned char> data;  // .ma2 bytes
+  std::vector<long long> words;     // .map integers after the header line
+};
+
+int map_load(const char *path, MapFile &m) {
+  NK_CHECK(path, NKH_ERR_ARG, "map: null path");
+  m.binary = has_suffix(path, ".ma2");
+  FILE *fp = std::fopen(path, "rb");
+  NK_CHECK(fp, NKH_ERR_ARG, "map: cannot open %s", path);
+  if (m.binary) {
+    std::fseek(fp, 0, SEEK_END);
+    const long sz = std::ftell(fp);
+    std::fseek(fp, 0, SEEK_SET);
+    m.data.resize(sz > 0 ? (size_t)sz : 0);
+    const size_t got = m.data.empty() ? 0 : std::fread(m.data.data(), 1, m.data.size(), fp);
+    std::fclose(fp);
+    NK_CHECK(got == m.data.size() && m.data.size() >= 136, NKH_ERR_ARG,
+             "ma2: %s is shorter than its header", path);
+    char ver[6] = {0}, a[13] = {0}, b[13] = {0};
+    std::memcpy(ver, m.data.data(), 5);
+    std::memcpy(a, m.data.data() + 5, 12);
+    std::memcpy(b, m.data.data() + 17, 12);
+    char *end = nullptr;
+    m.nel = (int)std::strtol(a, &end, 10);
+    NK_CHECK(end != a && m.nel > 0, NKH_ERR_ARG, "ma2: bad header '%.29s'",
+             (const char *)m.data.data());
+    m.nactive = (int)std::strtol(b, nullptr, 10);
+    float t;
+    std::memcpy(&t, m.data.data() + 132, 4);
+    if (std::fabs(t - 6.54321f) > 0.0002f) {
+      unsigned char *q = m.data.data() + 132;
+      std::reverse(q, q + 4);
+      std::memcpy(&t, q, 4);
+      std::reverse(q, q + 4);
+      NK_CHECK(std::fabs(t - 6.54321f) <= 0.0002f, NKH_ERR_ARG,
+               "ma2: byte-order word is neither order of 6.54321");
+      m.swap = true;
+    }
+    return NKH_OK;
+  }
+  std::string text;
+  char buf[65536];
+  size_t k;
+  while ((k = std::fread(buf, 1, sizeof buf, fp)) > 0) text.append(buf, k);
+  std::fclose(fp);
+  const size_t eol = text.find('\n');
+  NK_CHECK(eol != std::string::npos, NKH_ERR_ARG, "map: %s has no header line", path);
+  long long h0 = 0, h1 = 0;
+  NK_CHECK(std::sscanf(text.c_str(), "%lld %lld", &h0, &h1) == 2 && h0 > 0, NKH_ERR_ARG,
+           "map: bad header line in %s", path);
+  m.nel = (int)h0;
+  m.nactive = (int)h1;
+  const char *s = text.c_str() + eol + 1;
+  while (*s) {
+    char *end = nullptr;
+    const long long v = std::strtoll(s, &end, 10);
+    if (end == s) {
+      NK_CHECK(std::isspace((unsigned char)*s), NKH_ERR_ARG, "map: stray '%c' in %s", *s, path);
+      ++s;
+      continue;
+    }
+    m.words.push_back(v);
+    s = end;
+  }
+  return NKH_OK;
+}
+
+}  // namespace
+
+int map_header(const char *path, int *nel, int *nactive) {
+  MapFile m;
+  NK_TRY(map_load(path, m));
+  if (nel) *nel = m.nel;
+  if (nactive) *nactive = m.nactive;
+  return NKH_OK;
+}
+
+int map_read(const char *path, int nel, int ndim, int *part, int64_t *vertex) {
+  NK_CHECK(ndim == 2 || ndim == 3, NKH_ERR_ARG, "map: ndim %d", ndim);
+  MapFile m;
+  NK_TRY(map_load(path, m));
+  NK_CHECK(nel == m.nel, NKH_ERR_ARG,
+           "map: %d elements requested, the file has %d (the .map and .re2 must agree)", nel,
+           m.nel);
+  const int nv = 1 << ndim, rec = 1 + nv;
+  if (m.binary) {
+    const size_t need = 136 + (size_t)nel * rec * 4;
+    NK_CHECK(m.data.size() == need, NKH_ERR_ARG, "ma2: %zu bytes, %zu expected for %d elements",
+             m.data.size(), need, nel);
+    for (int e = 0; e < nel; ++e)
+      for (int c = 0; c < rec; ++c) {
+        unsigned char w[4];
+        std::memcpy(w, m.data.data() + 136 + ((size_t)e * rec + c) * 4, 4);
+        if (m.swap) std::reverse(w, w + 4);
+        int32_t v;
+        std::memcpy(&v, w, 4);
+        if (c == 0) {
+          if (part) part[e] = v;
+        } else if (vertex) {
+          vertex[(size_t)nv * e + (c - 1)] = v;
+        }
+      }
+  } else {
+    NK_CHECK(m.words.size() == (size_t)nel * rec, NKH_ERR_ARG,
+             "map: %zu numbers after the header, %zu expected for %d elements", m.words.size(),
+             (size_t)nel * rec, nel);
+    for (int e = 0; e < nel; ++e) {
+      if (part) part[e] = (int)m.words[(size_t)e * rec];
+      for (int c = 0; c < nv && vertex; ++c)
+        vertex[(size_t)nv * e + c] = m.words[(size_t)e * rec + 1 + c];
+    }
+  }
+  return NKH_OK;
+}
+
 namespace {
 
 // preprocessor corner c (0-based) of lexicographic corner ix: the
@@ -399,10 +533,206 @@ void gh_extend_edges(double *x) {
 // (genxyz.f:1349-1353 eindx), 0-based
 constexpr int kEdgeNode[12] = {1, 5, 7, 3, 19, 23, 25, 21, 9, 11, 17, 15};
 
+// preprocessor face -> symmetric face (connect1.f:200-205 eface1), 1-based
+constexpr int kEface1[6] = {3, 2, 4, 1, 5, 6};
+
+double dot3(const double *a, const double *b) {
+  double s = 0.0;
+  for (int i = 0; i < 3; ++i) s += a[i] * b[i];
+  return s;
+}
+void cross3(double *v1, const double *v2, const double *v3) {
+  v1[0] = v2[1] * v3[2] - v2[2] * v3[1];
+  v1[1] = v2[2] * v3[0] - v2[0] * v3[2];
+  v1[2] = v2[0] * v3[1] - v2[1] * v3[0];
+}
+void norm3(double *v) {
+  const double l = std::sqrt(dot3(v, v));
+  if (l > 0) {
+    v[0] /= l;
+    v[1] /= l;
+    v[2] /= l;
+  }
+}
+
+// genxyz.f:846-882 edg3d: n points on the great circle from x1 towards x2
+// (unit sphere), at the GLL angles; p(q) = point q, stride st in pts.
+void edg3d(double *pts, int st, int n, const double *x1, const double *x2, const double *z) {
+  double u1[3] = {x1[0], x1[1], x1[2]}, u2[3] = {x2[0], x2[1], x2[2]}, vn[3], b[3];
+  norm3(u1);
+  norm3(u2);
+  cross3(vn, x1, x2);
+  cross3(b, vn, x1);
+  norm3(vn);
+  norm3(b);
+  const double theta = std::acos(dot3(u1, u2));
+  for (int q = 0; q < n; ++q) {
+    const double tp = 0.5 * theta * (z[q] + 1.0);
+    const double c = std::cos(tp), s = std::sin(tp);
+    for (int d = 0; d < 3; ++d) pts[(size_t)st * q + d] = c * u1[d] + s * b[d];
+  }
+}
+
+// genxyz.f:677-845 sphsrf with crn3d (:934-978): face ifce (preprocessor,
+// 0-based) of element e moved onto the sphere curve[0..2] (centre),
+// curve[3] (radius); the deflection of each face point is blended linearly
+// to zero at the opposite face.  X[d] = the element's coordinates (n^3,
+// i fastest); xc/yc/zc its preprocessor corners.
+int sphsrf(double *X[3], int n, const double *z, const double *xc, const double *yc,
+           const double *zc, const double *curve, int ifce, int e) {
+  static const int indvtx[6][4] = {{1, 5, 3, 7}, {2, 4, 6, 8}, {1, 2, 5, 6},
+                                   {3, 7, 4, 8}, {1, 3, 2, 4}, {5, 6, 7, 8}};
+  static const int indx[8] = {1, 2, 4, 3, 5, 6, 8, 7};
+  static const int cface[6][2] = {{1, 4}, {2, 1}, {3, 2}, {4, 3}, {1, 5}, {5, 1}};
+  const double ctr[3] = {curve[0], curve[1], curve[2]}, radius = curve[3];
+  const int iface = kEface1[ifce];
+  NK_CHECK(radius > 0.0, NKH_ERR_ARG, "sphere of radius %g on element %d face %d", radius,
+           e + 1, ifce + 1);
+  double xcv[4][3];
+  for (int i = 0; i < 4; ++i) {
+    const int k = indx[indvtx[iface - 1][i] - 1] - 1;
+    xcv[i][0] = xc[k] - ctr[0];
+    xcv[i][1] = yc[k] - ctr[1];
+    xcv[i][2] = zc[k] - ctr[2];
+    const double radt = std::sqrt(dot3(xcv[i], xcv[i]));
+    NK_CHECK(std::fabs(radt - radius) / radius <= 1.0e-4, NKH_ERR_ARG,
+             "element %d face %d: vertex not on the requested sphere", e + 1, ifce + 1);
+  }
+  // points (a, b) of the face on the unit sphere, a along corner 1 -> 2,
+  // b along corner 1 -> 3
+  std::vector<double> p((size_t)3 * n * n);
+  auto P = [&](int a, int b) { return &p[(size_t)3 * (a + (size_t)n * b)]; };
+  edg3d(P(0, 0), 3 * n, n, xcv[0], xcv[2], z);
+  edg3d(P(n - 1, 0), 3 * n, n, xcv[1], xcv[3], z);
+  edg3d(P(0, 0), 3, n, xcv[0], xcv[1], z);
+  edg3d(P(0, n - 1), 3, n, xcv[2], xcv[3], z);
+  const int iv = cface[ifce][0] - 1, io = cface[ifce][1] - 1;
+  const double vout[3] = {xc[iv] - xc[io], yc[iv] - yc[io], zc[iv] - zc[io]};
+  const double vsph[3] = {xc[iv] - ctr[0], yc[iv] - ctr[1], zc[iv] - ctr[2]};
+  const bool concave = !(dot3(vsph, vout) > 0);
+  for (int b = 1; b < n - 1; ++b) {
+    double vn1[3], vn2[3];
+    cross3(vn1, P(0, b), P(n - 1, b));
+    for (int a = 1; a < n - 1; ++a) {
+      cross3(vn2, P(a, 0), P(a, n - 1));
+      if (concave) cross3(P(a, b), vn2, vn1);
+      else cross3(P(a, b), vn1, vn2);
+    }
+  }
+  for (int q = 0; q < n * n; ++q) {
+    double *v = &p[(size_t)3 * q];
+    norm3(v);
+    for (int d = 0; d < 3; ++d) v[d] = v[d] * radius;
+    for (int d = 0; d < 3; ++d) v[d] = v[d] + ctr[d];
+  }
+  if (iface == 1 || iface == 4 || iface == 5)
+    for (int b = 0; b < n; ++b)
+      for (int a = b + 1; a < n; ++a)
+        for (int d = 0; d < 3; ++d) std::swap(P(a, b)[d], P(b, a)[d]);
+  // connect1.f:586-629 skpdat of the symmetric face: the face nodes as
+  // (j1, j2) of the array viewed as (n*n, n), 1-based
+  const int nx = n, ny = n, nz = n;
+  int sk[6];
+  switch (iface) {
+    case 1: sk[0] = 1; sk[1] = nx * (ny - 1) + 1; sk[2] = nx; sk[3] = 1;
+            sk[4] = ny * (nz - 1) + 1; sk[5] = ny; break;
+    case 2: sk[0] = nx; sk[1] = nx * ny; sk[2] = nx; sk[3] = 1;
+            sk[4] = ny * (nz - 1) + 1; sk[5] = ny; break;
+    case 3: sk[0] = 1; sk[1] = nx; sk[2] = 1; sk[3] = 1; sk[4] = ny * (nz - 1) + 1;
+            sk[5] = ny; break;
+    case 4: sk[0] = 1 + nx * (ny - 1); sk[1] = nx * ny; sk[2] = 1; sk[3] = 1;
+            sk[4] = ny * (nz - 1) + 1; sk[5] = ny; break;
+    case 5: sk[0] = 1; sk[1] = nx; sk[2] = 1; sk[3] = 1; sk[4] = ny; sk[5] = 1; break;
+    default: sk[0] = 1 + nx * ny * (nz - 1); sk[1] = nx + nx * ny * (nz - 1); sk[2] = 1;
+             sk[3] = 1; sk[4] = ny; sk[5] = 1; break;
+  }
+  const int iopp[3] = {nx - 1, nx * (ny - 1), nx * ny * (nz - 1)};
+  const int idir = 2 * (iface % 2) - 1, ifc2 = (iface + 1) / 2;
+  int q = 0;
+  for (int j2 = sk[3]; j2 <= sk[4]; j2 += sk[5])
+    for (int j1 = sk[0]; j1 <= sk[1]; j1 += sk[2]) {
+      const double *xs = &p[(size_t)3 * q++];
+      const int jopp = j1 + iopp[ifc2 - 1] * idir;
+      const size_t node = (size_t)(j1 - 1) + (size_t)nx * (j2 - 1);
+      const double dx[3] = {xs[0] - X[0][node], xs[1] - X[1][node], xs[2] - X[2][node]};
+      const int joff = (j1 - jopp) / (n - 1);
+      for (int ix = 2; ix <= n; ++ix) {
+        const int j = jopp + joff * (ix - 1);
+        const double zeta = 0.5 * (z[ix - 1] + 1.0);
+        const size_t o = (size_t)(j - 1) + (size_t)nx * (j2 - 1);
+        for (int d = 0; d < 3; ++d) X[d][o] = X[d][o] + dx[d] * zeta;
+      }
+    }
+  return NKH_OK;
+}
+
+// genxyz.f:2-103 arcsrf in 3-D: edge isid (1..8, preprocessor; the edges of
+// the faces t = -1, +1) bent into the circular arc of radius curve[0] in
+// its x-y plane (centre on the side given by the sign of the radius); the
+// arc's deviation from the chord is blended into the element with the
+// linear weights of the two faces the edge bounds.
+int arcsrf(double *X[3], int n, const double *z, const double *xc, const double *yc,
+           double radius, int isid, int e) {
+  const int i1 = isid - 1;
+  const int i2 = isid == 4 ? 0 : isid == 8 ? 4 : isid;
+  const double p1x = xc[i1], p1y = yc[i1], p2x = xc[i2], p2y = yc[i2];
+  const double gap = std::sqrt((p1x - p2x) * (p1x - p2x) + (p1y - p2y) * (p1y - p2y));
+  NK_CHECK(!(std::fabs(2.0 * radius) <= gap * 1.00001), NKH_ERR_ARG,
+           "too small a radius (%g) on side %d of element %d: chord %g", radius, isid, e + 1,
+           gap);
+  const double xs = p2y - p1y, ys = p1x - p2x;
+  const double xys = std::sqrt(xs * xs + ys * ys);
+  const double dtheta = std::fabs(std::asin(0.5 * gap / radius));
+  const double p12x = (p1x + p2x) / 2.0, p12y = (p1y + p2y) / 2.0;
+  const double xcen = p12x - xs / xys * radius * std::cos(dtheta);
+  const double ycen = p12y - ys / xys * radius * std::cos(dtheta);
+  const double theta0 = std::atan2(p12y - ycen, p12x - xcen);
+  const int s1 = (isid - 1) % 4 + 1;
+  std::vector<double> xcrv((size_t)n), ycrv((size_t)n), h1((size_t)n), h2((size_t)n);
+  for (int ix = 0; ix < n; ++ix) {
+    h1[(size_t)ix] = (1.0 - z[ix]) * 0.5;
+    h2[(size_t)ix] = (1.0 + z[ix]) * 0.5;
+  }
+  for (int ix = 0; ix < n; ++ix) {
+    const int ixt = s1 > 2 ? n - 1 - ix : ix;
+    double r = z[ix];
+    if (radius < 0.0) r = -r;
+    xcrv[(size_t)ixt] = xcen + std::fabs(radius) * std::cos(theta0 + r * dtheta) -
+                        (h1[(size_t)ix] * p1x + h2[(size_t)ix] * p2x);
+    ycrv[(size_t)ixt] = ycen + std::fabs(radius) * std::sin(theta0 + r * dtheta) -
+                        (h1[(size_t)ix] * p1y + h2[(size_t)ix] * p2y);
+  }
+  // math.f:476-492 addtnsr with the blending weights of the symmetric face
+  // the edge lies on and of the t face
+  const int sf = kEface1[s1 - 1];
+  const double *ht = (isid - 1) / 4 == 0 ? h1.data() : h2.data();
+  for (int d = 0; d < 2; ++d) {
+    const double *crv = d == 0 ? xcrv.data() : ycrv.data();
+    for (int k = 0; k < n; ++k)
+      for (int j = 0; j < n; ++j) {
+        for (int i = 0; i < n; ++i) {
+          double f1, f2;
+          if (sf <= 2) {  // varies along s, weight in r
+            f1 = (sf == 1 ? h1 : h2)[(size_t)i];
+            f2 = crv[j];
+          } else {  // varies along r, weight in s
+            f1 = crv[i];
+            f2 = (sf == 3 ? h1 : h2)[(size_t)j];
+          }
+          const double hh = f2 * ht[k];
+          X[d][(size_t)i + (size_t)n * (j + (size_t)n * k)] += hh * f1;
+        }
+      }
+  }
+  return NKH_OK;
+}
+
 }  // namespace
 
-// genxyz.f:1269-1400 xyzlin / xyzquad in 3-D: GLL coordinates from the
-// corners, with 'm' (midside node) edges; other curve types are not built.
+// genxyz.f:562-605 genxyz in 3-D: GLL coordinates from the corners by
+// linquad (:1235-1400 xyzlin / xyzquad, 'm' midside-node edges), then per
+// element the 's' (sphere) faces in face order and the 'C' (circular arc)
+// edges 1..8 in edge order.  'e' and other curve types are not built.
 int mesh_xyz_from_corners(int lx1, int nel, const double *xc, const double *yc,
                           const double *zc, const char *ccurve, const double *curve,
                           double *xm1, double *ym1, double *zm1) {
@@ -423,8 +753,10 @@ int mesh_xyz_from_corners(int lx1, int nel, const double *xc, const double *yc,
       const char c = ccurve[(size_t)12 * e + k];
       if (c == 'm') quad = true;
       else
-        NK_CHECK(c == ' ' || c == '\0', NKH_ERR_UNSUPPORTED,
-                 "xyz from corners: curve type '%c' on element %d edge %d", c, e + 1, k + 1);
+        NK_CHECK(c == ' ' || c == '\0' || (c == 'C' && k < 8) || (c == 's' && k < 6) ||
+                     (c == 'm'),
+                 NKH_ERR_UNSUPPORTED, "xyz from corners: curve type '%c' on element %d edge %d",
+                 c, e + 1, k + 1);
     }
     if (lx1 == 2) quad = false;  // genxyz.f:1258
     const double *src[3] = {xc + (size_t)8 * e, yc + (size_t)8 * e, zc + (size_t)8 * e};
@@ -467,6 +799,14 @@ int mesh_xyz_from_corners(int lx1, int nel, const double *xc, const double *yc,
             dst[d][i + lx1 * (j + lx1 * k)] = s;
           }
     }
+    if (!ccurve || lx1 < 3) continue;
+    const double *xe = xc + (size_t)8 * e, *ye = yc + (size_t)8 * e, *ze = zc + (size_t)8 * e;
+    for (int f = 0; f < 6; ++f)
+      if (ccurve[(size_t)12 * e + f] == 's')
+        NK_TRY(sphsrf(dst, lx1, z.data(), xe, ye, ze, &curve[((size_t)12 * e + f) * 5], f, e));
+    for (int k = 0; k < 8; ++k)
+      if (ccurve[(size_t)12 * e + k] == 'C')
+        NK_TRY(arcsrf(dst, lx1, z.data(), xe, ye, curve[((size_t)12 * e + k) * 5], k + 1, e));
   }
   return NKH_OK;
 }

@@ -318,6 +318,8 @@ int re2_read(const char *path, int nelgt, int ndim, int nfields, int *group, dou
 int mesh_vertex_ids(int nel, int ndim, const double *xc, const double *yc, const double *zc,
                     const char *cbc, const double *bc, double tol_rel, int64_t *vertex,
                     int64_t *nvert);
+int map_header(const char *path, int *nel, int *nactive);
+int map_read(const char *path, int nel, int ndim, int *part, int64_t *vertex);
 int mesh_xyz_from_corners(int lx1, int nel, const double *xc, const double *yc,
                           const double *zc, const char *ccurve, const double *curve,
                           double *xm1, double *ym1, double *zm1);

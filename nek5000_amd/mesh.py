@@ -14,6 +14,7 @@ import numpy as np
 
 from ._lib import load, check, c_int64
 import ctypes
+import os
 
 # ---------------------------------------------------------------------------
 # counter-based RNG (splitmix64 finaliser) keyed by (seed, id)
@@ -248,8 +249,9 @@ class BoxMesh(_HexMesh):
 class Re2Mesh(_HexMesh):
     """A Nek case mesh from its .re2 file (reader_re2.f) through the library:
     corners, curve data and boundary conditions as Nek's /INPUT/ holds them,
-    vertex ids by corner coincidence, GLL coordinates by genxyz's linquad
-    ('m' midside edges supported).
+    vertex ids from the case's .map/.ma2 (map2.f read_map, map_path) or by
+    corner coincidence, GLL coordinates by genxyz ('m' midside edges, 'C'
+    arcs, 's' sphere faces).
 
     field: which boundary-condition section drives dirichlet_faces (0 = the
             first, Nek's fluid field when the case has one).
@@ -257,7 +259,7 @@ class Re2Mesh(_HexMesh):
             every face with a boundary condition other than 'E  ' and 'P  '.
     """
 
-    def __init__(self, path, lx1, field=0, dirichlet=None, tol_rel=1e-2):
+    def __init__(self, path, lx1, field=0, dirichlet=None, tol_rel=1e-2, map_path=None):
         lib = load()
         self.path = path
         self.lx1 = lx1
@@ -286,6 +288,15 @@ class Re2Mesh(_HexMesh):
         self._vertex = None
         self._xyz = None
         self.tol_rel = tol_rel
+        # map_path: vertex ids (and partitions) from the case's .map/.ma2
+        # (map2.f read_map; "auto": <case>.map, else <case>.ma2, as Nek looks
+        # for them) instead of corner coincidence
+        if map_path == "auto":
+            base = path[:-4] if path.endswith(".re2") else path
+            map_path = next((p for p in (base + ".map", base + ".ma2") if os.path.exists(p)),
+                            None)
+        self.map_path = map_path
+        self.part = None
 
     @property
     def n(self):
@@ -293,7 +304,19 @@ class Re2Mesh(_HexMesh):
 
     @property
     def vertex(self):
-        """Global vertex ids (1-based), shape (nel,2,2,2) indexed [e,k,j,i]."""
+        """Global vertex ids, shape (nel,2,2,2) indexed [e,k,j,i]: the map
+        file's when there is one, else 1.. by corner coincidence."""
+        if self._vertex is None and self.map_path is not None:
+            lib = load()
+            p = self.map_path.encode()
+            ne, na = ctypes.c_int(), ctypes.c_int()
+            check(lib.nkh_map_header(p, ctypes.byref(ne), ctypes.byref(na)), "nkh_map_header")
+            v = np.zeros(self.nel * 2 ** self.ndim, dtype=np.int64)
+            self.part = np.zeros(self.nel, dtype=np.int32)
+            check(lib.nkh_map_read(p, self.nel, self.ndim, self.part.ctypes.data, v.ctypes.data),
+                  "nkh_map_read")
+            self.nvert = int(len(np.unique(v)))
+            self._vertex = v.reshape(self.nel, 2, 2, 2)
         if self._vertex is None:
             lib = load()
             v = np.zeros(self.nel * 8, dtype=np.int64)

@@ -6,11 +6,19 @@ package.
 
   * read_re2: reader_re2.f:2-620 (read_re2_hdr, readp_re2_mesh/curve/bc,
     buf_to_xyz/curve/bc) for #v001..#v004 files in either byte order;
+  * read_map: map2.f:712-942 read_map (.map text, .ma2 binary in either
+    byte order) on one rank;
   * vertex_ids: corner coincidence by brute force over all corner pairs (an
     independent algorithm from the library's sorted sweep), periodic faces
     joined after the translation between them;
   * xyz_from_corners: genxyz.f:1235-1400 linquad (xyzlin, xyzquad) with
-    navier5.f:2422-2560 gh_face_extend_3d, written as dense tensor products.
+    navier5.f:2422-2560 gh_face_extend_3d, written as dense tensor products;
+    then genxyz.f:562-605's 's' faces (sphsrf :677-845) and 'C' edges
+    (arcsrf :2-103), restated geometrically: face points where the great
+    circles through opposite edge points meet (the intersection on the
+    corners' side of the sphere), edge arcs by rotation about the centre
+    from the first corner, deflections blended linearly to the opposite face
+    (sphere) or with the two faces' linear weights (arc).
 """
 import numpy as np
 
@@ -83,6 +91,27 @@ def read_re2(path):
     assert off == len(data)
     out.update(curve=curve, ccurve=ccurve, cbc=cbcs, bc=bcs, ncurve=ncur)
     return out
+
+
+def read_map(path, ndim=3):
+    """map2.f:712-942 read_map for one rank: (nel, nactive, part (nel,),
+    vertex (nel, 2^ndim)) from a .map (text) or .ma2 (binary) file."""
+    nv = 2 ** ndim
+    if path.endswith(".ma2"):
+        data = open(path, "rb").read()
+        hdr = data[:132].decode("ascii")
+        nel, nact = int(hdr[5:17]), int(hdr[17:29])
+        little = abs(np.frombuffer(data[132:136], "<f4")[0] - 6.54321) <= 2e-4
+        w = np.frombuffer(data[136:], ("<" if little else ">") + "i4")
+        assert w.size == nel * (1 + nv)
+        w = w.reshape(nel, 1 + nv).astype(np.int64)
+    else:
+        lines = open(path).read().split("\n")
+        nel, nact = (int(t) for t in lines[0].split()[:2])
+        w = np.array([int(t) for ln in lines[1:] for t in ln.split()], np.int64)
+        assert w.size == nel * (1 + nv)
+        w = w.reshape(nel, 1 + nv)
+    return nel, nact, w[:, 0].astype(np.int32), w[:, 1:]
 
 
 def lex_corners(xc, yc, zc):
@@ -179,6 +208,108 @@ def _gh_edges(q):
     return v + e
 
 
+# preprocessor face -> symmetric face (connect1.f:200-205), 1-based, and
+# symmetric face -> (axis r/s/t = 0/1/2, side -1/+1 = 0/1)
+EFACE1 = [3, 2, 4, 1, 5, 6]
+SYM_FACE = {1: (0, 0), 2: (0, 1), 3: (1, 0), 4: (1, 1), 5: (2, 0), 6: (2, 1)}
+
+
+def _unit(v):
+    return v / np.linalg.norm(v, axis=-1, keepdims=True)
+
+
+def _slerp(u1, u2, t):
+    th = np.arccos(np.clip(np.dot(u1, u2), -1.0, 1.0))
+    return (np.sin((1 - t)[:, None] * th) * u1 + np.sin(t[:, None] * th) * u2) / np.sin(th)
+
+
+def _node(ax, m, ta, p, q):
+    idx = [0, 0, 0]
+    idx[ax], idx[ta[0]], idx[ta[1]] = m, p, q
+    return idx[2], idx[1], idx[0]  # [k, j, i]
+
+
+def sphere_face(X, z, lexc, crv, ifce):
+    """Face ifce (preprocessor, 0-based) of one element onto the sphere
+    crv = (xc, yc, zc, R); X = [x, y, z] arrays [k, j, i], changed in place;
+    lexc (8, 3) the corners in lexicographic order."""
+    n = len(z)
+    ax, side = SYM_FACE[EFACE1[ifce]]
+    ta = [a for a in range(3) if a != ax]
+    c, R = np.asarray(crv[:3], float), float(crv[3])
+    assert R > 0
+
+    def corner(p, q):
+        idx = [0, 0, 0]
+        idx[ax], idx[ta[0]], idx[ta[1]] = side, p, q
+        return lexc[idx[0] + 2 * idx[1] + 4 * idx[2]] - c
+
+    u = {(p, q): _unit(corner(p, q)) for p in (0, 1) for q in (0, 1)}
+    for p in (0, 1):
+        for q in (0, 1):
+            assert abs(np.linalg.norm(corner(p, q)) - R) <= 1e-4 * R
+    t = (z + 1) / 2
+    P = np.zeros((n, n, 3))  # [q, p]
+    P[0, :] = _slerp(u[0, 0], u[1, 0], t)
+    P[n - 1, :] = _slerp(u[0, 1], u[1, 1], t)
+    P[:, 0] = _slerp(u[0, 0], u[0, 1], t)
+    P[:, n - 1] = _slerp(u[1, 0], u[1, 1], t)
+    mean = sum(u.values())
+    for q in range(1, n - 1):
+        n1 = np.cross(P[q, 0], P[q, n - 1])
+        for p in range(1, n - 1):
+            d = np.cross(n1, np.cross(P[0, p], P[n - 1, p]))
+            P[q, p] = d if np.dot(d, mean) > 0 else -d
+    P = c + R * _unit(P)
+    w = (1 + z) / 2 if side == 1 else (1 - z) / 2
+    f = side * (n - 1)
+    for q in range(n):
+        for p in range(n):
+            fk = _node(ax, f, ta, p, q)
+            dx = [P[q, p, d] - X[d][fk] for d in range(3)]
+            for m in range(n):
+                nk = _node(ax, m, ta, p, q)
+                for d in range(3):
+                    X[d][nk] += dx[d] * w[m]
+
+
+def arc_edge(X, z, xp, yp, radius, isid):
+    """Edge isid (1..8, preprocessor) of one element bent into the circle
+    of radius |radius| through its corners (centre to the left of corner
+    isid -> next for radius > 0); xp, yp the preprocessor corners."""
+    n = len(z)
+    i1 = isid - 1
+    i2 = 0 if isid == 4 else 4 if isid == 8 else isid
+    p1, p2 = np.array([xp[i1], yp[i1]]), np.array([xp[i2], yp[i2]])
+    g = np.linalg.norm(p2 - p1)
+    assert 2 * abs(radius) > g * 1.00001
+    tang = (p2 - p1) / g
+    left = np.array([-tang[1], tang[0]])
+    h = np.sqrt(radius ** 2 - g ** 2 / 4)
+    ctr = (p1 + p2) / 2 + np.sign(radius) * h * left
+    sweep = 2 * np.arcsin(g / (2 * abs(radius))) * np.sign(radius)
+    t = (z + 1) / 2
+    a = (p1 - ctr)
+    ang = sweep * t
+    arc = ctr + np.stack([np.cos(ang) * a[0] - np.sin(ang) * a[1],
+                          np.sin(ang) * a[0] + np.cos(ang) * a[1]], axis=1)
+    pert = arc - ((1 - t)[:, None] * p1 + t[:, None] * p2)  # along corner isid -> next
+    s1 = (isid - 1) % 4  # 0: r up at s=-1, 1: s up at r=+1, 2: r down at s=+1, 3: s down at r=-1
+    if s1 >= 2:
+        pert = pert[::-1]
+    wt = (1 - z) / 2 if isid <= 4 else (1 + z) / 2
+    wlo, whi = (1 - z) / 2, (1 + z) / 2
+    for d in range(2):
+        if s1 == 0:
+            X[d] += np.einsum("k,j,i->kji", wt, wlo, pert[:, d])
+        elif s1 == 2:
+            X[d] += np.einsum("k,j,i->kji", wt, whi, pert[:, d])
+        elif s1 == 1:
+            X[d] += np.einsum("k,j,i->kji", wt, pert[:, d], whi)
+        else:
+            X[d] += np.einsum("k,j,i->kji", wt, pert[:, d], wlo)
+
+
 def xyz_from_corners(lx1, xc, yc, zc, ccurve=None, curve=None, z=None):
     """GLL coordinates (nel, lx1, lx1, lx1) [k, j, i] for x, y, z."""
     from oracle import oracle as O
@@ -207,4 +338,13 @@ def xyz_from_corners(lx1, xc, yc, zc, ccurve=None, curve=None, z=None):
                 q = cb
                 J = Jl
             out[d][e] = np.einsum("ia,jb,kc,cba->kji", J, J, J, q)
+        if ccurve is None or lx1 < 3:
+            continue
+        X = [out[d][e] for d in range(3)]
+        for f in range(6):
+            if ccurve[e, f] == b"s":
+                sphere_face(X, z, L[e], curve[e, f, :4], f)
+        for k in range(8):
+            if ccurve[e, k] == b"C":
+                arc_edge(X, z, xc[e], yc[e], curve[e, k, 0], k + 1)
     return out

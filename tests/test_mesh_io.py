@@ -193,7 +193,11 @@ def test_reader_errors(tmp_path):
     assert lib.nkh_re2_header(str(trunc).encode(), n, n, n, n) == 1
     assert b"truncated" in lib.nkh_last_error() or b"trailing" in lib.nkh_last_error()
     m = Re2Mesh(path("ethier.re2"), 4)
-    m.ccurve[0, 0] = b"C"
+    m.ccurve[0, 0] = b"e"  # genxyz.f gensrf: not built
+    with pytest.raises(_lib.NekAmdError, match="curve type"):
+        m.xyz()
+    m = Re2Mesh(path("ethier.re2"), 4)
+    m.ccurve[0, 9] = b"C"  # arcs only on edges 1..8
     with pytest.raises(_lib.NekAmdError, match="curve type"):
         m.xyz()
 
