@@ -557,7 +557,7 @@ def main():
         lib.nkh_timers_read(b"mgv", ctypes.byref(sec), ctypes.byref(calls))
         t_mgv = sec.value / max(calls.value, 1)
         nh = args.lx1
-        kernels.append(dict(kernel=f"k_sch_fdm<{nh + 2},double> (fine-level local FDM solves)",
+        kernels.append(dict(kernel=f"k_sch_fdm_d<{nh + 2}> (fine-level local FDM solves)",
                             ms=t_fdm * 1e3,
                             bytes_per_launch=fdm_bytes_per_element(nh) * box.nel))
     for k in kernels:

@@ -1168,20 +1168,39 @@ static int mg_solve_t(Mesh &mesh, FT *z, const FT *r, hipStream_t s, int crs_mod
   NK_HIP(hipStreamWaitEvent(b, mg.ev_fork, 0));
   // the local solves and their overlap sum (into T.e / T.e32; with
   // fuse_ov the sum is formed after the join from the local solutions)
-  hipStream_t f = s;
-  if (timed) NK_HIP(hipEventRecord(mg.tev[0], f));
-  if (fp32) {
-    NK_TRY((schwarz_local<float, FT>(mg, Lm, r, fuse_ov ? (float *)nullptr : T.e32, f)));
-  } else if constexpr (!f32io) {
-    NK_TRY((schwarz_local<double, FT>(mg, Lm, r,
-                                      fuse_ov ? (double *)nullptr : (merged ? T.e : z), f)));
-    if (!merged) NK_TRY(gs_op(T.gs, z, 0, f));
-  }
-  if (timed) NK_HIP(hipEventRecord(mg.tev[1], f));
-  // coarser levels on b: restrict, dssum, mask; the Schwarz solve of each
-  // intermediate level on c (it only reads the masked residual), beside the
-  // restriction to level 1 and the coarse solve on b
+  // (the timer events bracket the local solves alone)
+  auto fine = [&]() -> int {
+    if (timed) NK_HIP(hipEventRecord(mg.tev[0], s));
+    if (fp32) {
+      NK_TRY((schwarz_local<float, FT>(mg, Lm, r, (float *)nullptr, s)));
+      if (timed) NK_HIP(hipEventRecord(mg.tev[1], s));
+      if (!fuse_ov)
+        NK_TRY((launch_sch_out<float>(T.e32, T.wint32, T.wgh32, T.nh, nel, s)));
+    } else if constexpr (!f32io) {
+      NK_TRY((schwarz_local<double, FT>(mg, Lm, r, (double *)nullptr, s)));
+      if (timed) NK_HIP(hipEventRecord(mg.tev[1], s));
+      if (!fuse_ov)
+        NK_TRY((launch_sch_out<double>(merged ? T.e : z, T.wint, T.wgh, T.nh, nel, s)));
+      if (!merged) NK_TRY(gs_op(T.gs, z, 0, s));
+    }
+    return NKH_OK;
+  };
+  // the Schwarz solve of intermediate level l on c (it only reads the
+  // masked residual of its level)
   hipStream_t c = mg.sc;
+  auto level = [&](int l, hipEvent_t after) -> int {
+    MgLevel &C = mg.lev[l];
+    NK_HIP(hipStreamWaitEvent(c, after, 0));
+    if constexpr (f32io)  // inner solve of the mixed solver: FP32 local solves
+      NK_TRY((schwarz_local<float, double, double>(mg, l, C.r, C.e, c)));
+    else
+      NK_TRY(mg_schwarz_local(mg, l, C.r, C.e, c));
+    NK_TRY(gs_op(C.gs, C.e, 0, c));
+    NK_HIP(hipEventRecord(mg.ev_le[l], c));
+    return NKH_OK;
+  };
+  NK_TRY(fine());
+  // coarser levels on b: restrict, dssum, mask; then the coarse solve
   for (int l = Lm - 1; l >= 2; --l) {
     MgLevel &C = mg.lev[l], &F = mg.lev[l + 1];
     if (l < Lm - 1) {  // (the level below the fine one was done on s above)
@@ -1190,13 +1209,7 @@ static int mg_solve_t(Mesh &mesh, FT *z, const FT *r, hipStream_t s, int crs_mod
       NK_TRY(gs_op(C.gs, C.r, 0, b));
       NK_HIP(hipEventRecord(mg.ev_lr[l], b));
     }
-    NK_HIP(hipStreamWaitEvent(c, mg.ev_lr[l], 0));
-    if constexpr (f32io)  // inner solve of the mixed solver: FP32 local solves
-      NK_TRY((schwarz_local<float, double, double>(mg, l, C.r, C.e, c)));
-    else
-      NK_TRY(mg_schwarz_local(mg, l, C.r, C.e, c));
-    NK_TRY(gs_op(C.gs, C.e, 0, c));
-    NK_HIP(hipEventRecord(mg.ev_le[l], c));
+    NK_TRY(level(l, mg.ev_lr[l]));
     rsrc = C.r;
   }
   // the level-1 mask needs no pass of its own: the coarse gather reads only
