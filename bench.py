@@ -142,7 +142,8 @@ def order_sweep():
     """Polynomial-order sweep (jittered boxes, E^(1/3) ~ 368/N so every order
     carries about 5e7 unique DOF; h1 = h2 = 1; f ~ U(-1,1)): per order the
     autotuner's choice (made on the first apply), the CG-fused axhelm
-    kernel's time per kernel variant (HIP events of the library's timers
+    kernel's time per kernel variant (0/1 element/plane staged, 2/3 the
+    same with the wide slab packing; HIP events of the library's timers
     over 20 Jacobi-PCG iterations each), and 100 fixed Jacobi-PCG iterations
     of hmholtz with the FP64 operator (its axhelm kernel time and HBM
     fraction at 12 words per node) and with the FP32 one (mixed precision).
@@ -191,7 +192,7 @@ def order_sweep():
 
         u = torch.zeros_like(rhs)
         per_variant = {}
-        for v in (0, 1):
+        for v in (0, 1, 2, 3):
             solver.set_axhelm_tuning(N + 1, v, 0)
             timed_pcg(F, u, rhs, one, 5)
             per_variant[f"variant{v}_cg_ms"] = timed_pcg(F, u, rhs, one, 20)[1]

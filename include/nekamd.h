@@ -178,10 +178,14 @@ int nkd_axhelm(int mesh, double *au, const double *u, const double *h1, const do
 /* Kernel choice for the AXHELM applies of order lx1 (no counterpart in the
    reference, whose axhelm has one form): variant 0 stages the element in LDS,
    1 stages one t-plane at a time (3 lx1^2 instead of 3 lx1^3 doubles of LDS);
-   both give identical results.  grid = blocks of the plain apply (0: one per
-   element group).  Unless set, the first nkd_axhelm or solve of an order on
-   a mesh of >= 2^20 nodes times both variants at two grids and keeps the
-   fastest (*tuned = 1 afterwards). */
+   2 (plane) and 3 (element) pack more lx1 x lx1 thread slabs per block where
+   one leaves lanes of its last wavefront idle (lx1 = 5-7, 9, 10, 12-14;
+   elsewhere they run 1 and 0).  Every variant gives the same plain apply
+   bit for bit; the fused CG applies of 2 and 3 group their partial sums
+   differently (iterates equal to rounding).  grid = blocks of the plain
+   apply (0: one per element group).  Unless set, the first nkd_axhelm or
+   solve of an order on a mesh of >= 2^20 nodes times the order's variants
+   at two grids and keeps the fastest (*tuned = 1 afterwards). */
 int nkh_axhelm_tuning(int lx1, int *variant, int *grid, int *tuned);
 /* sets the plain, CG-fused and flexible-CG-fused applies to variant */
 int nkh_axhelm_set_tuning(int lx1, int variant, int grid);

@@ -11,12 +11,21 @@ static int axhelm_apply(const Mesh &m, double *au, const double *u, Coef h1, Coe
 #undef CALL
 }
 
+static int wide_mask(int np) {
+#define CALL(NP) axhelm_wide_mask<NP>()
+  NK_NP_SWITCH(np, CALL)
+#undef CALL
+}
+
 // Autotuner: on the first apply of an order on a mesh of at least 2^20
-// local nodes (and not inside a graph capture), time both kernels with two
-// grid sizes on the call's own data -- the plain apply is element-local, so
-// every candidate writes the same au -- and keep the fastest for that order;
-// then time both kernels of the CG-fused and flexible-CG-fused applies
-// (their grid is fixed) on scratch vectors and keep the faster of each.
+// local nodes (and not inside a graph capture), time every kernel variant
+// the order has (element- or plane-staged slabs; the orders whose NP x NP
+// slabs leave lanes idle also pack epb_wide slabs per block) with two grid
+// sizes on the call's own data -- the plain apply is element-local, so
+// every candidate writes the same au -- and keep the fastest for that
+// order; then time the variants of the CG-fused and flexible-CG-fused
+// applies (their grid is fixed) on scratch vectors and keep the fastest of
+// each.
 static int tune_axhelm(const Mesh &m, double *au, const double *u, Coef h1, Coef h2,
                        hipStream_t s) {
   AxhelmChoice &ch = g_axhelm_choice[m.np];
@@ -32,12 +41,15 @@ static int tune_axhelm(const Mesh &m, double *au, const double *u, Coef h1, Coef
     NK_HIP(hipEventElapsedTime(ms, e0, e1));
     return NKH_OK;
   };
-  const int cand_var[4] = {0, 0, 1, 1}, cand_grid[4] = {16384, 0, 16384, 0};
+  const int wm = wide_mask(m.np);
+  int vars[4] = {0, 1, 0, 0}, nv = 2;
+  if (wm & 4) vars[nv++] = 2;
+  if (wm & 8) vars[nv++] = 3;
   float best = 1e30f;
   AxhelmChoice pick = ch;
-  for (int c = 0; c < 4; ++c) {
-    ch.variant = cand_var[c];
-    ch.grid = cand_grid[c];
+  for (int c = 0; c < 2 * nv; ++c) {
+    ch.variant = vars[c / 2];
+    ch.grid = (c % 2 == 0) ? 16384 : 0;
     float ms = 0.f;
     NK_TRY(timed([&] { return axhelm_apply(m, au, u, h1, h2, s); }, &ms));
     if (ms < best) {
@@ -56,19 +68,25 @@ static int tune_axhelm(const Mesh &m, double *au, const double *u, Coef h1, Coef
     double *p = buf, *r = buf + n, *d = buf + 2 * n, *w = buf + 3 * n, *rho = buf + 4 * n;
     CgState *st = reinterpret_cast<CgState *>(buf + 4 * n + part);
     NK_HIP(hipMemsetAsync(buf, 0, bytes, s));
-    float ms[2] = {0.f, 0.f};
-    for (int v = 0; v < 2; ++v) {
-      ch.variant_cg = v;
+    float ms[4] = {0.f, 0.f, 0.f, 0.f};
+    for (int c = 0; c < nv; ++c) {
+      ch.variant_cg = vars[c];
       NK_TRY(timed([&] { return launch_axhelm_cg(m, w, p, r, d, h1, h2, r, st, rho, s); },
-                   &ms[v]));
+                   &ms[c]));
     }
-    pick.variant_cg = ms[1] < ms[0] ? 1 : 0;
-    for (int v = 0; v < 2; ++v) {
-      ch.variant_flex = v;
+    int bc = 0;
+    for (int c = 1; c < nv; ++c)
+      if (ms[c] < ms[bc]) bc = c;
+    pick.variant_cg = vars[bc];
+    for (int c = 0; c < nv; ++c) {
+      ch.variant_flex = vars[c];
       NK_TRY(timed([&] { return launch_axhelm_flex(m, w, p, r, h1, h2, &st->beta, rho, s); },
-                   &ms[v]));
+                   &ms[c]));
     }
-    pick.variant_flex = ms[1] < ms[0] ? 1 : 0;
+    bc = 0;
+    for (int c = 1; c < nv; ++c)
+      if (ms[c] < ms[bc]) bc = c;
+    pick.variant_flex = vars[bc];
     NK_HIP(hipStreamSynchronize(s));
     (void)hipFree(buf);
   } else {
@@ -109,7 +127,7 @@ int axhelm_tuning(int np, int *variant, int *grid, int *tuned) {
 }
 
 int axhelm_set_tuning(int np, int variant, int grid) {
-  NK_CHECK(np >= 2 && np <= kMaxNP && (variant == 0 || variant == 1) && grid >= 0,
+  NK_CHECK(np >= 2 && np <= kMaxNP && variant >= 0 && variant <= 3 && grid >= 0,
            NKH_ERR_ARG, "axhelm tuning: lx1=%d variant=%d grid=%d", np, variant, grid);
   g_axhelm_choice[np] = AxhelmChoice{variant, grid, variant, variant, true};
   return NKH_OK;

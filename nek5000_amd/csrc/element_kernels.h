@@ -24,6 +24,36 @@ constexpr int epb() {
   return (NP * NP >= 64) ? 1 : (64 / (NP * NP));
 }
 
+// The wide packing (the autotuner's variants 2 and 3): the number of
+// NP x NP slabs per block, at most 512 threads, that leaves the fewest idle
+// lanes in the block's last wavefront (the smallest such count on ties).
+// At NP = 5 two slabs fill 50 of 64 lanes, five fill 125 of 128; at NP = 12
+// one fills 144 of 192, three fill 432 of 448.
+template <int NP>
+constexpr int epb_wide() {
+  int best = epb<NP>();
+  double waste = 1.0;
+  for (int e = 1; e * NP * NP <= 512; ++e) {
+    const int t = e * NP * NP, w = (t + 63) / 64 * 64;
+    const double f = double(w - t) / double(w);
+    if (f < waste - 1e-12) {
+      waste = f;
+      best = e;
+    }
+  }
+  return best;
+}
+template <int NP>
+constexpr bool has_wide() {
+  return epb_wide<NP>() != epb<NP>();
+}
+// the element-staged kernel with the wide packing only where its LDS
+// (3 NP^3 doubles per slab) stays within 96 KB, so two blocks share a CU
+template <int NP>
+constexpr bool has_wide_elem() {
+  return has_wide<NP>() && 3 * NP * NP * NP * 8 * epb_wide<NP>() <= 96 * 1024;
+}
+
 // Sum over the block's threads through LDS (blockDim need not fill whole
 // wavefronts), in a fixed tree order -> deterministic.  sbuf holds blockDim
 // doubles.  Result valid on thread 0.
@@ -59,8 +89,8 @@ __device__ __forceinline__ void stnt(double v, double *p) { __builtin_nontempora
 // from *betap.  MODE 3: MODE 0 gated by a state (returns while st->done),
 // for applies that a captured iteration runs only on some passes.
 // ---------------------------------------------------------------------------
-template <int NP, int MODE, bool H1F, int H2M>
-__global__ void __launch_bounds__(NP * NP * epb<NP>())
+template <int NP, int MODE, bool H1F, int H2M, int EPB_ = epb<NP>()>
+__global__ void __launch_bounds__(NP * NP * EPB_)
     k_axhelm(DMat<NP> Dm, const double *__restrict__ Dg, double *__restrict__ au,
              const double *__restrict__ uin, const double *__restrict__ g, int64_t gstride,
              const double *__restrict__ h1f, double h1s, const double *__restrict__ h2f,
@@ -69,7 +99,7 @@ __global__ void __launch_bounds__(NP * NP * epb<NP>())
              const double *__restrict__ dinv, const double *__restrict__ mult,
              const CgState *__restrict__ st, double *__restrict__ rho_part,
              const double *__restrict__ betap) {
-  constexpr int NP2 = NP * NP, NP3 = NP2 * NP, EPB = epb<NP>();
+  constexpr int NP2 = NP * NP, NP3 = NP2 * NP, EPB = EPB_;
   __shared__ double s_D[NP][NP + 1];
   __shared__ double s_u[EPB][NP3];
   __shared__ double s_wr[EPB][NP3];
@@ -186,8 +216,8 @@ __global__ void __launch_bounds__(NP * NP * epb<NP>())
 // element-staged kernel holds 96 KB and one block per CU).  ur, us, wr and
 // ws of plane k need only plane k; the t-direction terms stay in registers.
 // Arithmetic and its order are those of k_axhelm, so results are identical.
-template <int NP, int MODE, bool H1F, int H2M>
-__global__ void __launch_bounds__(NP * NP * epb<NP>())
+template <int NP, int MODE, bool H1F, int H2M, int EPB_ = epb<NP>()>
+__global__ void __launch_bounds__(NP * NP * EPB_)
     k_axhelm_pl(DMat<NP> Dm, const double *__restrict__ Dg, double *__restrict__ au,
                 const double *__restrict__ uin, const double *__restrict__ g, int64_t gstride,
                 const double *__restrict__ h1f, double h1s, const double *__restrict__ h2f,
@@ -196,7 +226,7 @@ __global__ void __launch_bounds__(NP * NP * epb<NP>())
                 const double *__restrict__ dinv, const double *__restrict__ mult,
                 const CgState *__restrict__ st, double *__restrict__ rho_part,
                 const double *__restrict__ betap) {
-  constexpr int NP2 = NP * NP, EPB = epb<NP>();
+  constexpr int NP2 = NP * NP, EPB = EPB_;
   __shared__ double s_D[NP][NP + 1];
   __shared__ double s_u[EPB][NP2];
   __shared__ double s_wr[EPB][NP2];
@@ -327,16 +357,14 @@ struct AxhelmChoice {
 };
 inline AxhelmChoice g_axhelm_choice[kMaxNP + 1];
 
-template <int NP, int MODE>
-static int axhelm_dispatch(const Mesh &m, double *au, const double *u, Coef h1, Coef h2,
-                           double *pout, const double *r, const double *dinv,
-                           const double *mult, const CgState *st, double *rho_part,
-                           hipStream_t s, const double *betap = nullptr) {
-  DMat<NP> Dm;
-  for (int t = 0; t < NP * NP; ++t) Dm.d[t] = m.D[t];
-  constexpr int EPB = epb<NP>();
+// One launch of the axhelm kernel (element- or plane-staged, EPB slabs per
+// block) for the apply MODE.
+template <int NP, int MODE, bool PLANE, int EPB>
+static int axhelm_go(const Mesh &m, const DMat<NP> &Dm, double *au, const double *u, Coef h1,
+                     Coef h2, double *pout, const double *r, const double *dinv,
+                     const double *mult, const CgState *st, double *rho_part, hipStream_t s,
+                     const double *betap, int gridcap) {
   const int64_t ngroups = (m.nel + EPB - 1) / EPB;
-  const AxhelmChoice ch = g_axhelm_choice[NP];
   int grid;
   if (MODE == 3) {
     // gated: most launches find the gate closed, and an empty launch costs
@@ -345,9 +373,10 @@ static int axhelm_dispatch(const Mesh &m, double *au, const double *u, Coef h1, 
     g = g < 64 ? 64 : (g > 4096 ? 4096 : g);
     grid = (int)(ngroups < g ? ngroups : g);
   } else if (MODE >= 1) {
+    // one partial sum per block: the fixed count the CG's scalar kernels read
     grid = axhelm_cg_grid(m);
   } else {
-    const int64_t cap = ch.grid > 0 ? ch.grid : ((int64_t)1 << 20);
+    const int64_t cap = gridcap > 0 ? gridcap : ((int64_t)1 << 20);
     grid = (int)(ngroups < cap ? ngroups : cap);
   }
   if (grid < 1) grid = 1;
@@ -355,16 +384,15 @@ static int axhelm_dispatch(const Mesh &m, double *au, const double *u, Coef h1, 
   // h2 mode: 0 none (reference ifh2 false), 1 uniform, 2 field
   const int h2m = h2.uniform() ? (h2.s != 0.0 ? 1 : 0) : 2;
   const bool h1f = !h1.uniform();
-  const int var = MODE == 1 ? ch.variant_cg : (MODE == 2 ? ch.variant_flex : ch.variant);
-#define NK_LAUNCH(H1F_, H2M_)                                                              \
-  if (var == 1)                                                                             \
-    hipLaunchKernelGGL((k_axhelm_pl<NP, MODE, H1F_, H2M_>), dim3(grid), block, 0, s, Dm,    \
-                       m.Dd, au, u, m.g, m.n, h1.f, h1.s, h2.f, h2.s, m.bm1, m.nel, pout, r, \
-                       dinv, mult, st, rho_part, betap);                                    \
-  else                                                                                      \
-    hipLaunchKernelGGL((k_axhelm<NP, MODE, H1F_, H2M_>), dim3(grid), block, 0, s, Dm, m.Dd, \
-                       au, u, m.g, m.n, h1.f, h1.s, h2.f, h2.s, m.bm1, m.nel, pout, r, dinv, \
-                       mult, st, rho_part, betap)
+#define NK_LAUNCH(H1F_, H2M_)                                                                \
+  if constexpr (PLANE)                                                                        \
+    hipLaunchKernelGGL((k_axhelm_pl<NP, MODE, H1F_, H2M_, EPB>), dim3(grid), block, 0, s, Dm, \
+                       m.Dd, au, u, m.g, m.n, h1.f, h1.s, h2.f, h2.s, m.bm1, m.nel, pout, r,   \
+                       dinv, mult, st, rho_part, betap);                                      \
+  else                                                                                        \
+    hipLaunchKernelGGL((k_axhelm<NP, MODE, H1F_, H2M_, EPB>), dim3(grid), block, 0, s, Dm,    \
+                       m.Dd, au, u, m.g, m.n, h1.f, h1.s, h2.f, h2.s, m.bm1, m.nel, pout, r,   \
+                       dinv, mult, st, rho_part, betap)
   if (h1f) {
     if (h2m == 0) NK_LAUNCH(true, 0);
     else if (h2m == 1) NK_LAUNCH(true, 1);
@@ -377,6 +405,45 @@ static int axhelm_dispatch(const Mesh &m, double *au, const double *u, Coef h1, 
 #undef NK_LAUNCH
   NK_HIP(hipGetLastError());
   return NKH_OK;
+}
+
+// Variants: 0 element-staged, 1 plane-staged (epb slabs per block), 2
+// plane-staged and 3 element-staged with the wide packing (epb_wide); an
+// order without a distinct wide packing runs 1 for 2 and 0 for 3, as does
+// 3 where the wide element-staged kernel would not fit two blocks per CU.
+// The plain apply is element-local, so every variant gives the same au bit
+// for bit; the fused applies' partial sums of p.w group the elements by
+// block, so the wide packings round those sums differently.
+template <int NP, int MODE>
+static int axhelm_dispatch(const Mesh &m, double *au, const double *u, Coef h1, Coef h2,
+                           double *pout, const double *r, const double *dinv,
+                           const double *mult, const CgState *st, double *rho_part,
+                           hipStream_t s, const double *betap = nullptr) {
+  DMat<NP> Dm;
+  for (int t = 0; t < NP * NP; ++t) Dm.d[t] = m.D[t];
+  const AxhelmChoice ch = g_axhelm_choice[NP];
+  const int var = MODE == 1 ? ch.variant_cg : (MODE == 2 ? ch.variant_flex : ch.variant);
+  const int gc = ch.grid;
+#define NK_GO(PLANE_, EPB_)                                                                  \
+  return axhelm_go<NP, MODE, PLANE_, EPB_>(m, Dm, au, u, h1, h2, pout, r, dinv, mult, st,   \
+                                          rho_part, s, betap, gc)
+  if (var == 2) {
+    if constexpr (has_wide<NP>()) NK_GO(true, epb_wide<NP>());
+    else NK_GO(true, epb<NP>());
+  }
+  if (var == 3) {
+    if constexpr (has_wide_elem<NP>()) NK_GO(false, epb_wide<NP>());
+    else NK_GO(false, epb<NP>());
+  }
+  if (var == 1) NK_GO(true, epb<NP>());
+  NK_GO(false, epb<NP>());
+#undef NK_GO
+}
+
+// which variants an order has beyond 0 and 1 (bit 2: variant 2, bit 3: 3)
+template <int NP>
+constexpr int axhelm_wide_mask() {
+  return (has_wide<NP>() ? 4 : 0) | (has_wide_elem<NP>() ? 8 : 0);
 }
 
 #define NK_NP_SWITCH(np, CALL)                                      \
