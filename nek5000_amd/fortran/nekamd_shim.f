@@ -12,6 +12,13 @@ c
 c     nekamd_setup uploads an element set once, on its first solve: Nek's
 c     geometric factors (after setdef/sfastax, with IFDFRM), mass matrix,
 c     vertex ids and, for the PnPn pressure, the multigrid face codes.
+c
+c     Cases the library does not build -- 2-D meshes, axisymmetric ones
+c     (ifaxis, hmholtz.f:128-168) and the pressure with param(42) = 1 --
+c     go to the reference's own routines, linked under the names
+c     nek_hmholtz, nek_hsolve, nek_axhelm and nek_dssum (the reference's
+c     objects with their symbols renamed; the link recipe is in the
+c     integration notes).
 
 c-----------------------------------------------------------------------
       subroutine nekamd_params(prm)
@@ -121,6 +128,11 @@ c-----------------------------------------------------------------------
       type(nkh_result) res
       integer ierr
       common /iterhm/ niterhm
+      if (ldim.ne.3 .or. ifaxis .or.
+     $    (name.eq.'PRES' .and. int(param(42)).eq.1)) then
+         call nek_hmholtz(name,u,rhs,h1,h2,mask,mult,imsh,tli,maxit,isd)
+         return
+      endif
       if (nkh_mesh(imsh).lt.0) call nekamd_setup(imsh)
       call nekamd_params(prm)
       ierr = nkh_hmholtz(nkh_mesh(imsh), name//c_null_char, u, rhs,
@@ -147,6 +159,12 @@ c     caller (approx and napprox themselves are not read or written)
       integer ierr, k, p
       integer(c_intptr_t) key
       common /iterhm/ niterhm
+      if (ldim.ne.3 .or. ifaxis .or.
+     $    (name.eq.'PRES' .and. int(param(42)).eq.1)) then
+         call nek_hsolve(name,u,r,h1,h2,vmk,vml,imsh,tol,maxit,isd,
+     $                   approx,napprox,bi)
+         return
+      endif
       if (nkh_mesh(imsh).lt.0) call nekamd_setup(imsh)
       key = loc(approx)
       p = -1
@@ -177,8 +195,14 @@ c     caller (approx and napprox themselves are not read or written)
 c-----------------------------------------------------------------------
       subroutine axhelm(au,u,helm1,helm2,imesh,isd)
       use nekamd
+      include 'SIZE'
+      include 'TOTAL'
       real au(1),u(1),helm1(1),helm2(1)
       integer imesh, isd, ierr
+      if (ldim.ne.3 .or. ifaxis) then
+         call nek_axhelm(au,u,helm1,helm2,imesh,isd)
+         return
+      endif
       if (nkh_mesh(imesh).lt.0) call nekamd_setup(imesh)
       ierr = nkh_axhelm(nkh_mesh(imesh), au, u, helm1, helm2, isd)
       call nekamd_check(ierr, 'axhelm')
@@ -195,6 +219,10 @@ c     and for other orders, Nek's own gather-scatter handle does the sum
       include 'TOTAL'
       real u(1)
       integer nx, ny, nz, ifldt, imsh, ierr
+      if (ldim.ne.3) then
+         call nek_dssum(u,nx,ny,nz)
+         return
+      endif
       ifldt = ifield
       if (ifldt.eq.ifldmhd) ifldt = 1
       imsh = 2

@@ -7,6 +7,8 @@ c       dssum0           dssum before any set-up: Nek's own gs handle
 c       nogpu            nekamd_setup without a GPU: the EXIT line
 c       solve IN OUT     a case through the shims (hmholtz, axhelm and
 c                        dssum after set-up)
+c       route            axisymmetric and param(42) = 1 cases go to the
+c                        reference's routines (no library call)
       program drive
       use nekamd
       include 'SIZE'
@@ -73,6 +75,22 @@ c        passive scalar that shares the velocity's handle
             ifield = k
             call dssum(u, lx1, ly1, lz1)
          enddo
+
+      elseif (mode.eq.'route') then
+         allocate(u(8), rhs(8), h1(8), h2(8))
+         allocate(vmask(8), vmult(8), au(8))
+         param = 0.0
+         ifaxis = .true.
+         call hmholtz('VELX', u, rhs, h1, h2, vmask, vmult, 1, tli,
+     $                10, 1)
+         call axhelm(au, u, h1, h2, 1, 1)
+         call hsolve('TEMP', u, rhs, h1, h2, vmask, vmult, 2, tli,
+     $               10, 1, au, np, au)
+         ifaxis = .false.
+         param(42) = 1.0
+         call hmholtz('PRES', u, rhs, h1, h2, vmask, vmult, 1, tli,
+     $                10, 1)
+         write(6,'(a)') 'routed'
 
       elseif (mode.eq.'nogpu') then
          nelv = 1

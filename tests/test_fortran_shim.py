@@ -201,3 +201,16 @@ def test_undeformed_elements_through_shims(gpu, tmp_path):
     assert l2rel(u, u_o.reshape(-1)) < 1e-10
     au_o = P.field.axhelm(u.reshape(P.shape), h1, h2).reshape(-1)
     assert np.max(np.abs(au - au_o)) <= 1e-13 * np.max(np.abs(au_o))
+
+
+def test_unbuilt_cases_route_to_reference():
+    """Axisymmetric meshes (ifaxis) and the pressure with param(42) = 1 go to
+    the reference's own routines under their renamed symbols, before any
+    library call (this runs without a GPU: a set-up would end in EXIT)."""
+    r = drive("route")
+    lines = [ln.split() for ln in r.stdout.splitlines()]
+    assert ["nek_hmholtz", "VELX", "1", "1"] in lines
+    assert ["nek_axhelm", "1", "1"] in lines
+    assert ["nek_hsolve", "TEMP", "2", "1"] in lines
+    assert ["nek_hmholtz", "PRES", "1", "1"] in lines
+    assert lines[-1] == ["routed"] and "EXIT" not in r.stdout
