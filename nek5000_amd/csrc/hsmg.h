@@ -77,6 +77,8 @@ struct Mg {
   double *fpart = nullptr;
   FcgState *fst = nullptr;
   int *fflag = nullptr;
+  // the multiplicity weights as counts (wt = 1/count, count <= 255), plus a flag
+  uint8_t *wt8 = nullptr;
   // GMRES work: V (kGmresM+1 fields), Z (kGmresM fields), x, r, w, partials
   double *gV = nullptr, *gZ = nullptr, *gx = nullptr, *gr = nullptr, *gw = nullptr;
   double *gpart = nullptr, *gsm = nullptr;

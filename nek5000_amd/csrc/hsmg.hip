@@ -1369,18 +1369,43 @@ __global__ void __launch_bounds__(256)
   }
 }
 
+// The multiplicity weights wt (Nek's vmult = 1/dssum(1)) read from a byte
+// per node: code c holds wt = 1/c exactly (1/c in FP64 is the value Nek
+// computes), 1/8 of the weight array's traffic in the two streaming kernels
+// of every iteration.  k_wt_code builds the codes and flags any weight that
+// is not 1/c for a count c in 1..255 (then the FP64 weights are read).
+__global__ void k_wt_code(const double *__restrict__ wt, uint8_t *__restrict__ code, int64_t n,
+                          int *bad) {
+  int b = 0;
+  NK_GRID_LOOP(i, n) {
+    const double w = wt[i];
+    const double c = w > 0.0 ? rint(1.0 / w) : 0.0;
+    const bool ok = c >= 1.0 && c <= 255.0 && 1.0 / c == w;
+    code[i] = ok ? (uint8_t)c : (uint8_t)0;
+    b |= ok ? 0 : 1;
+  }
+  if (b) atomicOr(bad, 1);
+}
+__device__ __forceinline__ double wt_at(const double *__restrict__ w, int64_t i) {
+  return __builtin_nontemporal_load(w + i);
+}
+__device__ __forceinline__ double wt_at(const uint8_t *__restrict__ c, int64_t i) {
+  return 1.0 / (double)c[i];
+}
+
 // partials of glsc3(z,wt,r) and glsc3(z,wt,w).  The reference keeps the
 // previous residual r1 and forms rho2 = -glsc3(z,wt,r1-r) (hmholtz.f:2236-
 // 2241); r1 - r is the last update alpha*w exactly in exact arithmetic (w
 // still holds that iteration's A p), so rho2 = -alpha*glsc3(z,wt,w) without
 // the extra vector.
+template <typename WT>
 __global__ void __launch_bounds__(256)
     k_fcg_dots(const double *__restrict__ z, const double *__restrict__ r,
-               const double *__restrict__ w, const double *__restrict__ wt, int64_t n,
+               const double *__restrict__ w, const WT *__restrict__ wt, int64_t n,
                double *pa, double *pb) {
   double a = 0.0, b = 0.0;
   NK_GRID_LOOP(i, n) {
-    const double zw = z[i] * wt[i];
+    const double zw = z[i] * wt_at(wt, i);
     a += zw * r[i];
     b += zw * w[i];
   }
@@ -1439,9 +1464,10 @@ __global__ void __launch_bounds__(256) k_fcg_alpha(FcgState *st, const double *p
 }
 
 // res += alpha p; r -= alpha w; partial sum r*r*wt
+template <typename WT>
 __global__ void __launch_bounds__(256)
     k_fcg_update(double *__restrict__ res, double *__restrict__ r, const double *__restrict__ p,
-                 const double *__restrict__ w, const double *__restrict__ wt, const FcgState *st,
+                 const double *__restrict__ w, const WT *__restrict__ wt, const FcgState *st,
                  int64_t n, double *part) {
   const double alpha = st->alpha;
   double a = 0.0;
@@ -1453,7 +1479,7 @@ __global__ void __launch_bounds__(256)
     const double ri =
         __builtin_nontemporal_load(r + i) - alpha * __builtin_nontemporal_load(w + i);
     __builtin_nontemporal_store(ri, r + i);
-    a = a + ri * ri * __builtin_nontemporal_load(wt + i);
+    a = a + ri * ri * wt_at(wt, i);
   }
   block_part(a, part);
 }
@@ -1503,7 +1529,17 @@ int run_flex_cg(Mesh &m, double *res, Coef h1, Coef h2, const double *wt, const 
   hipLaunchKernelGGL(k_zero_int, dim3(1), dim3(1), 0, s, mg.fflag);
   hipLaunchKernelGGL(k_neq, dim3(mg_grid(n)), b, 0, s, mask, mgmask, n, mg.fflag);
   NK_HIP(hipMemcpyAsync(&differ, mg.fflag, sizeof(int), hipMemcpyDeviceToHost, s));
+  // the weights as counts, when every one is 1/count (the flag after the
+  // n code bytes)
+  const int64_t wt8_flag = (n + 7) / 8 * 8;
+  if (!mg.wt8) NK_HIP(hipMalloc(&mg.wt8, (size_t)wt8_flag + sizeof(int)));
+  int *wt8_bad = reinterpret_cast<int *>(mg.wt8 + wt8_flag);
+  hipLaunchKernelGGL(k_zero_int, dim3(1), dim3(1), 0, s, wt8_bad);
+  hipLaunchKernelGGL(k_wt_code, dim3(mg_grid(n)), b, 0, s, wt, mg.wt8, n, wt8_bad);
+  int wbad = 1;
+  NK_HIP(hipMemcpyAsync(&wbad, wt8_bad, sizeof(int), hipMemcpyDeviceToHost, s));
   NK_HIP(hipStreamSynchronize(s));
+  const uint8_t *wtc = wbad ? nullptr : mg.wt8;
   // den inside the fused axhelm (p . A_L p) and the mask inside the
   // assembly, when the conditions for that identity hold (p = beta p + z
   // is continuous and zero where pmask is when pmask is the fine mask)
@@ -1524,7 +1560,8 @@ int run_flex_cg(Mesh &m, double *res, Coef h1, Coef h2, const double *wt, const 
   for (int k = 1; k <= maxit; ++k) {
     if (differ && (rc = mul_inplace(r, mgmask, n, s)) != NKH_OK) break;
     if ((rc = mg_solve(m, z, r, s, prm.param40, prm.precision)) != NKH_OK) break;
-    hipLaunchKernelGGL(k_fcg_dots, g, b, 0, s, z, r, w, wt, n, pa, pb);
+    if (wtc) hipLaunchKernelGGL(k_fcg_dots<uint8_t>, g, b, 0, s, z, r, w, wtc, n, pa, pb);
+    else hipLaunchKernelGGL(k_fcg_dots<double>, g, b, 0, s, z, r, w, wt, n, pa, pb);
     hipLaunchKernelGGL(k_fcg_beta, dim3(1), b, 0, s, mg.fst, pa, pb);
     if (fused) {
       if (tm.on) (void)hipEventRecord(ev[0], s);
@@ -1544,7 +1581,10 @@ int run_flex_cg(Mesh &m, double *res, Coef h1, Coef h2, const double *wt, const 
       hipLaunchKernelGGL(k_fcg_den, g, b, 0, s, w, mask, wt, p, n, pa);
       hipLaunchKernelGGL(k_fcg_alpha, dim3(1), b, 0, s, mg.fst, pa);
     }
-    hipLaunchKernelGGL(k_fcg_update, g, b, 0, s, res, r, p, w, wt, mg.fst, n, pa);
+    if (wtc)
+      hipLaunchKernelGGL(k_fcg_update<uint8_t>, g, b, 0, s, res, r, p, w, wtc, mg.fst, n, pa);
+    else
+      hipLaunchKernelGGL(k_fcg_update<double>, g, b, 0, s, res, r, p, w, wt, mg.fst, n, pa);
     hipLaunchKernelGGL(k_fcg_rnorm, dim3(1), b, 0, s, mg.fst, pa, m.vol);
     if (hipGetLastError() != hipSuccess) {
       set_error("hmh_flex_cg: kernel launch failed");

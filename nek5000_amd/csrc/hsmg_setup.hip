@@ -541,7 +541,7 @@ void mg_free(Mesh &mesh) {
                   (void *)mg->gr, (void *)mg->gw, (void *)mg->gpart, (void *)mg->gsm,
                   (void *)mg->xr, (void *)mg->xp, (void *)mg->xz, (void *)mg->xw,
                   (void *)mg->xx, (void *)mg->xwt, (void *)mg->xpart, mg->xst,
-                  (void *)mg->opart})
+                  (void *)mg->opart, (void *)mg->wt8})
     if (p) (void)hipFree(p);
   for (hipEvent_t e : mg->tev)
     if (e) (void)hipEventDestroy(e);
