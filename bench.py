@@ -420,6 +420,10 @@ def main():
                          "exact XXT), 1 one AMG V-cycle (its param(40)=1)")
     ap.add_argument("--precision", type=int, default=0, choices=(0, 1, 2),
                     help="main line's nkh_params.precision (profiling the mixed solvers)")
+    ap.add_argument("--fdm-variant", type=int, default=0, choices=(0, 1, 2, 3),
+                    help="kernel of the local FDM solves for the main line (nkh_fdm_set_variant:"
+                         " 0 vector, 1/2 FP32/bf16 matrix cores for the single-precision "
+                         "smoother, 3 FP64 matrix cores)")
     ap.add_argument("--no-variants", action="store_true",
                     help="skip the extra short run of the other coarse solver")
     ap.add_argument("--batch", action="store_true",
@@ -486,6 +490,7 @@ def main():
         return solver.default_params(precision=precision)
 
     prm = params_for(args.crs, args.precision)
+    solver.set_fdm_variant(args.fdm_variant)
 
     def solve(r, p=None):
         u = torch.zeros_like(rhs0)
@@ -610,6 +615,10 @@ def main():
             solver.set_fdm_variant(1)
             measure("mixed FP32 inner solve, FDM contractions in FP32 on the matrix cores",
                     params_for(args.crs, 2))
+            # the FP64 smoother's contractions on the FP64 matrix cores
+            solver.set_fdm_variant(3)
+            measure("FP64 solve, FDM contractions in FP64 on the matrix cores",
+                    params_for(args.crs, 0))
             solver.set_fdm_variant(0)
         else:
             measure("mixed: FP32 Jacobi-PCG iteration (operator, vectors), FP64 dots and "
@@ -712,6 +721,7 @@ def main():
                             f"Jacobi-PCG {prec_name} to {args.tol:g} relative")),
             "nel": box.nel, "lx1": args.lx1,
             "preconditioner": "h1mg" if hsmg else "jacobi", "precision": prec_name,
+            "fdm_variant": args.fdm_variant,
             "multigrid": mg_info,
             "coarse_solver": (None if not hsmg else
                               ("one AMG V-cycle (param40=1)" if args.crs == 1 else

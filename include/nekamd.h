@@ -192,11 +192,12 @@ int nkh_axhelm_set_tuning(int lx1, int variant, int grid);
 /* the kernels of the CG-fused (cggo) and flexible-CG-fused applies, which
    the autotuner times separately on scratch vectors */
 int nkh_axhelm_tuning_fused(int lx1, int *variant_cg, int *variant_flex);
-/* Kernel of the single-precision local FDM solves (precision 1 and 2; no
-   counterpart in the reference, whose hsmg_do_fast has one form), for Schwarz
-   blocks of up to 16 points: 0 vector FMA (default), 1 the six contractions
-   as FP32 matrix-core products (v_mfma_f32_16x16x4_f32), 2 as bf16
-   matrix-core products with FP32 accumulation.  Process-wide. */
+/* Kernel of the local FDM solves (no counterpart in the reference, whose
+   hsmg_do_fast has one form), for Schwarz blocks of up to 16 points: 0 vector
+   FMA (default); for the single-precision smoother (precision 1 and 2) 1 the
+   six contractions as FP32 matrix-core products (v_mfma_f32_16x16x4_f32), 2
+   as bf16 matrix-core products with FP32 accumulation; for the FP64 smoother
+   3 as FP64 matrix-core products (v_mfma_f64_16x16x4_f64).  Process-wide. */
 int nkh_fdm_set_variant(int variant);
 int nkh_fdm_variant(int *variant);
 /* dssum.f:33-98 DSSUM: u_i <- sum over nodes sharing glo_num(i). */

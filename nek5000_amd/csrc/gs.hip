@@ -362,12 +362,12 @@ template <typename E, typename Z, typename W>
 __global__ void __launch_bounds__(256)
     k_gs_fin_pk(Z *__restrict__ z, const E *__restrict__ e, const W *__restrict__ w,
                 const int32_t *__restrict__ pk_m, const uint64_t *__restrict__ pk_head,
-                const int32_t *__restrict__ idx, int64_t npk) {
+                const int32_t *__restrict__ idx, int64_t npk, const double *__restrict__ lut) {
   const int p = pk_index();
   if (p >= npk) return;
   const PkLane L = pk_lane(pk_m, pk_head, idx, p);
   const E ev = L.act ? e[L.node] : E(0);
-  const double wv = L.act ? double(w[L.node]) : 0.0;
+  const double wv = L.act ? wdec(w, L.node, lut) : 0.0;
   const double zv = L.act ? double(z[L.node]) : 0.0;
   const double S = double(pk_reduce(ev, L, AddComb()));
   double acc = pk_reduce(S * wv + zv, L, AddComb());
@@ -379,7 +379,7 @@ template <typename E, typename Z, typename W>
 __global__ void __launch_bounds__(256)
     k_gs_fin(Z *__restrict__ z, const E *__restrict__ e, const W *__restrict__ w,
              const int32_t *__restrict__ off, const int32_t *__restrict__ idx, int64_t g0,
-             int64_t ngroups, int64_t chunk) {
+             int64_t ngroups, int64_t chunk, const double *__restrict__ lut) {
   const int nb = gridDim.x, b = blockIdx.x;
   const int64_t lb = (int64_t)(b % 8) * (nb / 8) + b / 8;
   const int64_t s0 = g0 + lb * chunk;
@@ -389,10 +389,10 @@ __global__ void __launch_bounds__(256)
     E Se = e[idx[a]];
     for (int32_t q = a + 1; q < c; ++q) Se += e[idx[q]];
     const double S = double(Se);
-    double acc = S * double(w[idx[a]]) + double(z[idx[a]]);
+    double acc = S * wdec(w, idx[a], lut) + double(z[idx[a]]);
     for (int32_t q = a + 1; q < c; ++q) {
       const int32_t l = idx[q];
-      acc += S * double(w[l]) + double(z[l]);
+      acc += S * wdec(w, l, lut) + double(z[l]);
     }
     acc = acc * (1.0 / (double)(c - a));
     for (int32_t q = a; q < c; ++q) z[idx[q]] = Z(acc);
@@ -487,17 +487,18 @@ int gs_assemble(const GsMap &gs, float *w, const CgState *st, hipStream_t s,
 }
 
 template <typename E, typename Z, typename W>
-static int gs_fin_t(const GsMap &gs, Z *z, const E *e, const W *w, hipStream_t s) {
+static int gs_fin_t(const GsMap &gs, Z *z, const E *e, const W *w, hipStream_t s,
+                    const double *lut = nullptr) {
   if (gs.ngroups == 0) return NKH_OK;
   if (gs.npk > 0)
     hipLaunchKernelGGL((k_gs_fin_pk<E, Z, W>), dim3(pk_blocks(gs.npk)), dim3(256), 0, s, z, e,
-                       w, gs.pk_m, gs.pk_head, gs.idx, gs.npk);
+                       w, gs.pk_m, gs.pk_head, gs.idx, gs.npk, lut);
   const int64_t nbig = gs.ngroups - gs.gpk;
   if (nbig > 0) {
     const int nb = gs_blocks(nbig);
     const int64_t chunk = (nbig + nb - 1) / nb;
     hipLaunchKernelGGL((k_gs_fin<E, Z, W>), dim3(nb), dim3(256), 0, s, z, e, w, gs.off,
-                       gs.idx, gs.gpk, gs.ngroups, chunk);
+                       gs.idx, gs.gpk, gs.ngroups, chunk, lut);
   }
   NK_HIP(hipGetLastError());
   return NKH_OK;
@@ -513,6 +514,11 @@ int gs_fin(const GsMap &gs, double *z, const float *e, const double *w, hipStrea
 
 int gs_fin(const GsMap &gs, float *z, const float *e, const float *w, hipStream_t s) {
   return gs_fin_t(gs, z, e, w, s);
+}
+
+int gs_fin(const GsMap &gs, double *z, const double *e, const uint8_t *w, const double *lut,
+           hipStream_t s) {
+  return gs_fin_t(gs, z, e, w, s, lut);
 }
 
 }  // namespace nekamd

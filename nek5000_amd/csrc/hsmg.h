@@ -45,6 +45,9 @@ struct MgLevel {
   double *e = nullptr, *r = nullptr;
   // single-precision copies for the mixed-precision smoother (fine level)
   float *fs32 = nullptr, *fl32 = nullptr, *wint32 = nullptr, *wgh32 = nullptr, *e32 = nullptr;
+  // fine level: mask*Schwarz weights as byte codes into Mg::wlut, when
+  // every value has one
+  uint8_t *mswt8 = nullptr;
   float *mswt32 = nullptr;  // mask * Schwarz weight, single precision
   float *rwt32 = nullptr;   // restriction weights, single precision
 };
@@ -79,6 +82,7 @@ struct Mg {
   int *fflag = nullptr;
   // the multiplicity weights as counts (wt = 1/count, count <= 255), plus a flag
   uint8_t *wt8 = nullptr;
+  double *wlut = nullptr;  // the 256 weights the fine level's byte codes index
   // GMRES work: V (kGmresM+1 fields), Z (kGmresM fields), x, r, w, partials
   double *gV = nullptr, *gZ = nullptr, *gx = nullptr, *gr = nullptr, *gw = nullptr;
   double *gpart = nullptr, *gsm = nullptr;

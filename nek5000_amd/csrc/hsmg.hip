@@ -728,22 +728,27 @@ __global__ void k_neq(const double *a, const double *b, int64_t n, int *flag) {
     if (a[i] != b[i]) atomicOr(flag, 1);
 }
 
-// The single-precision local solves with the six contractions on the matrix
-// cores: one wave per element; a contraction along one direction is the
-// product of the element's NL x NL matrix (zero-padded to 16 x 16) with the
-// NL x NL^2 array of pencils along that direction, in 16-column blocks.  BF:
-// bf16 operands (v_mfma_f32_16x16x16_bf16, one product per block), else FP32
-// operands (v_mfma_f32_16x16x4_f32, ceil(NL/4) products per block); FP32
-// accumulation either way.  Operand layouts (checked on gfx950): lane l
-// holds A[l%16][k], B[k][l%16] with k = l/16 (FP32, K = 4) or 4(l/16)+v
-// (bf16, K = 16), and D[4(l/16)+v][l%16].  Assembly of the extended array,
-// the diagonal and the write-back are those of k_sch_fdm.
-// Kernel of the single-precision local solves (process-wide): 0 the vector
-// kernel above, 1 FP32 matrix-core products, 2 bf16 matrix-core products.
+// The local solves with the six contractions on the matrix cores: one wave
+// per element; a contraction along one direction is the product of the
+// element's NL x NL matrix (zero-padded to 16 x 16) with the NL x NL^2 array
+// of pencils along that direction, in 16-column blocks.  Single precision
+// (R = float): BF, bf16 operands (v_mfma_f32_16x16x16_bf16, one product per
+// block), else FP32 operands (v_mfma_f32_16x16x4_f32, ceil(NL/4) products per
+// block); FP32 accumulation either way.  Double precision (R = double): FP64
+// operands and accumulation (v_mfma_f64_16x16x4_f64, ceil(NL/4) products per
+// block).  Operand layouts (checked on gfx950): lane l holds A[l%16][k],
+// B[k][l%16] with k = l/16 (FP32 and FP64, K = 4) or 4(l/16)+v (bf16,
+// K = 16); register v of the result holds D[4(l/16)+v][l%16] for the FP32
+// accumulators but D[4v+l/16][l%16] for the FP64 ones.  Assembly of the
+// extended array, the diagonal and the write-back are those of k_sch_fdm.
+// Kernel of the local solves (process-wide): 0 the vector kernels, 1 FP32
+// matrix-core products for the single-precision smoother, 2 bf16 ones, 3
+// FP64 matrix-core products for the double-precision smoother.
 int g_fdm_variant = 0;
 
 typedef short nk_v4s __attribute__((ext_vector_type(4)));
 typedef float nk_v4f __attribute__((ext_vector_type(4)));
+typedef double nk_v4d __attribute__((ext_vector_type(4)));
 
 __device__ __forceinline__ short nk_bf16(float x) {  // round to nearest even
   unsigned u = __float_as_uint(x);
@@ -751,17 +756,20 @@ __device__ __forceinline__ short nk_bf16(float x) {  // round to nearest even
   return (short)(u >> 16);
 }
 
-template <int NL, typename RI, bool BF>
+template <int NL, typename R, typename RI, bool BF>
 __global__ void __launch_bounds__(64)
-    k_sch_fdm_mx(float *__restrict__ wint, float *__restrict__ wgh, const RI *__restrict__ r,
+    k_sch_fdm_mx(R *__restrict__ wint, R *__restrict__ wgh, const RI *__restrict__ r,
                  const int32_t *__restrict__ gin, const int32_t *__restrict__ gout,
-                 const float *__restrict__ S, const float *__restrict__ lam, int64_t nel) {
+                 const R *__restrict__ S, const R *__restrict__ lam, int64_t nel) {
   static_assert(NL <= 16, "one 16 x 16 tile per contraction");
+  static_assert(!BF || std::is_same<R, float>::value, "bf16 operands accumulate in FP32");
+  constexpr bool F64 = std::is_same<R, double>::value;
+  using Acc = typename std::conditional<F64, nk_v4d, nk_v4f>::type;
   constexpr int N = NL - 2, N2 = N * N, N3 = N2 * N, M2 = NL * NL, NLAM = 3 * NL + 1;
   constexpr int P = NL | 1, PL = P * NL, T = 64, CB = (M2 + 15) / 16, KQ = (NL + 3) / 4;
-  __shared__ float u[PL * NL];
-  __shared__ float sS[3 * M2];
-  __shared__ float sl[NLAM];
+  __shared__ R u[PL * NL];
+  __shared__ R sS[3 * M2];
+  __shared__ R sl[NLAM];
   const int tid = threadIdx.x, lr = tid & 15, lg = tid >> 4;
   auto gpos = [](int q) {
     const int f = q / N2, a = q % N, b = (q / N) % N;
@@ -778,15 +786,15 @@ __global__ void __launch_bounds__(64)
   };
   // y_i = sum_m M(i,m) x_m for every pencil along direction D, M(i,m) =
   // Sm(m + NL i) when TR else Sm(i + NL m)
-  auto contract = [&](auto dir_tag, auto tr_tag, const float *Sm) {
+  auto contract = [&](auto dir_tag, auto tr_tag, const R *Sm) {
     constexpr int D = decltype(dir_tag)::value;
     constexpr bool TR = decltype(tr_tag)::value;
     constexpr int st = D == 0 ? 1 : (D == 1 ? P : PL);
-    auto Mv = [&](int i, int m) -> float {
-      return (i < NL && m < NL) ? (TR ? Sm[m + NL * i] : Sm[i + NL * m]) : 0.0f;
+    auto Mv = [&](int i, int m) -> R {
+      return (i < NL && m < NL) ? (TR ? Sm[m + NL * i] : Sm[i + NL * m]) : R(0);
     };
     nk_v4s ab;
-    float af[KQ];
+    R af[KQ];
     if constexpr (BF) {
 #pragma unroll
       for (int v = 0; v < 4; ++v) ab[v] = nk_bf16(Mv(lr, 4 * lg + v));
@@ -800,7 +808,7 @@ __global__ void __launch_bounds__(64)
       const bool cv = col < M2;
       const int pa = cv ? col % NL : 0, pb = cv ? col / NL : 0;
       const int base = D == 0 ? P * pa + PL * pb : (D == 1 ? pa + PL * pb : pa + P * pb);
-      nk_v4f acc = {0.0f, 0.0f, 0.0f, 0.0f};
+      Acc acc = {R(0), R(0), R(0), R(0)};
       if constexpr (BF) {
         nk_v4s bb;
 #pragma unroll
@@ -813,13 +821,16 @@ __global__ void __launch_bounds__(64)
 #pragma unroll
         for (int q = 0; q < KQ; ++q) {
           const int m = 4 * q + lg;
-          const float bv = (cv && m < NL) ? u[base + st * m] : 0.0f;
-          acc = __builtin_amdgcn_mfma_f32_16x16x4f32(af[q], bv, acc, 0, 0, 0);
+          const R bv = (cv && m < NL) ? u[base + st * m] : R(0);
+          if constexpr (F64)
+            acc = __builtin_amdgcn_mfma_f64_16x16x4f64(af[q], bv, acc, 0, 0, 0);
+          else
+            acc = __builtin_amdgcn_mfma_f32_16x16x4f32(af[q], bv, acc, 0, 0, 0);
         }
       }
 #pragma unroll
       for (int v = 0; v < 4; ++v) {
-        const int i = 4 * lg + v;
+        const int i = F64 ? 4 * v + lg : 4 * lg + v;
         if (cv && i < NL) u[base + st * i] = acc[v];
       }
     }
@@ -836,21 +847,21 @@ __global__ void __launch_bounds__(64)
     __syncthreads();
     const RI *re = r + e * N3;
     const int32_t *ge = gin + e * 6 * N2;
-    const float *Se = S + e * 3 * M2;
+    const R *Se = S + e * 3 * M2;
     for (int q = tid; q < 3 * M2; q += T) sS[q] = Se[q];
     if (tid < NLAM) sl[tid] = lam[e * NLAM + tid];
     for (int q = tid; q < N3; q += T) {
       const int i = q % N, j = (q / N) % N, k = q / N2;
-      u[(i + 1) + P * (j + 1) + PL * (k + 1)] = float(re[q]);
+      u[(i + 1) + P * (j + 1) + PL * (k + 1)] = R(re[q]);
     }
     for (int q = tid; q < 6 * N2; q += T) {
       const int32_t sidx = ge[q];
-      u[gpos(q)] = (sidx >= 0) ? float(r[sidx]) : 0.0f;
+      u[gpos(q)] = (sidx >= 0) ? R(r[sidx]) : R(0);
     }
     for (int q = tid; q < M2 * NL; q += T) {  // ghost edges and corners
       const int i = q % NL, j = (q / NL) % NL, k = q / M2;
       const int g = (i == 0 || i == NL - 1) + (j == 0 || j == NL - 1) + (k == 0 || k == NL - 1);
-      if (g >= 2) u[i + P * j + PL * k] = 0.0f;
+      if (g >= 2) u[i + P * j + PL * k] = R(0);
     }
     __syncthreads();
     contract(R_(), TR_(), sS);
@@ -861,10 +872,10 @@ __global__ void __launch_bounds__(64)
     __syncthreads();
     for (int q = tid; q < M2 * NL; q += T) {
       const int i = q % NL, j = (q / NL) % NL, k = q / M2;
-      const float lij = sl[i] + sl[NL + j], eps = sl[3 * NL];
-      const float diag = lij + sl[2 * NL + k];
+      const R lij = sl[i] + sl[NL + j], eps = sl[3 * NL];
+      const R diag = lij + sl[2 * NL + k];
       const int a = i + P * j + PL * k;
-      u[a] = (diag > eps) ? u[a] * (1.0f / diag) : 0.0f;
+      u[a] = (diag > eps) ? u[a] * (R(1) / diag) : R(0);
     }
     __syncthreads();
     contract(T_(), NT_(), sS + 2 * M2);
@@ -873,7 +884,7 @@ __global__ void __launch_bounds__(64)
     __syncthreads();
     contract(R_(), NT_(), sS);
     __syncthreads();
-    float *wi = wint + e * N3;
+    R *wi = wint + e * N3;
     for (int q = tid; q < N3; q += T) {
       const int i = q % N, j = (q / N) % N, k = q / N2;
       wi[q] = u[(i + 1) + P * (j + 1) + PL * (k + 1)];
@@ -886,11 +897,11 @@ __global__ void __launch_bounds__(64)
   }
 }
 
-template <int NL, typename RI, bool BF>
-void launch_sch_fdm_mx(float *wint, float *wgh, const float *S, const float *lam,
-                       const MgLevel &L, const RI *r, int64_t nel, hipStream_t s) {
+template <int NL, typename R, typename RI, bool BF>
+void launch_sch_fdm_mx(R *wint, R *wgh, const R *S, const R *lam, const MgLevel &L,
+                       const RI *r, int64_t nel, hipStream_t s) {
   const int grid = (int)std::min<int64_t>((nel + 7) / 8 * 8, 1 << 20);
-  hipLaunchKernelGGL((k_sch_fdm_mx<NL, RI, BF>), dim3(grid), dim3(64), 0, s, wint, wgh, r,
+  hipLaunchKernelGGL((k_sch_fdm_mx<NL, R, RI, BF>), dim3(grid), dim3(64), 0, s, wint, wgh, r,
                      L.gin, L.gout, S, lam, nel);
 }
 
@@ -942,22 +953,25 @@ static int schwarz_local(Mg &mg, int l, const RI *r, EO *e, hipStream_t s) {
     wint = L.wint32; wgh = L.wgh32; S = L.fs32; lam = L.fl32;
   }
   bool done = false;
-  if constexpr (std::is_same<R, float>::value) {
-    const int mx = g_fdm_variant;
-    if (mx > 0 && L.nh + 2 <= 16) {
-      switch (L.nh + 2) {
-#define NK_C(NL)                                                                     \
-  case NL:                                                                           \
-    if (mx == 2) launch_sch_fdm_mx<NL, RI, true>(wint, wgh, S, lam, L, r, mg.nel, s); \
-    else launch_sch_fdm_mx<NL, RI, false>(wint, wgh, S, lam, L, r, mg.nel, s);        \
+  const int mx = g_fdm_variant;
+  constexpr bool f32 = std::is_same<R, float>::value;
+  if ((f32 ? (mx == 1 || mx == 2) : mx == 3) && L.nh + 2 <= 16) {
+    switch (L.nh + 2) {
+#define NK_C(NL)                                                                \
+  case NL:                                                                      \
+    if constexpr (f32)                                                          \
+      if (mx == 2) {                                                            \
+        launch_sch_fdm_mx<NL, R, RI, true>(wint, wgh, S, lam, L, r, mg.nel, s);  \
+        break;                                                                  \
+      }                                                                         \
+    launch_sch_fdm_mx<NL, R, RI, false>(wint, wgh, S, lam, L, r, mg.nel, s);     \
     break;
-        NK_C(5) NK_C(6) NK_C(7) NK_C(8) NK_C(9) NK_C(10) NK_C(11) NK_C(12) NK_C(13) NK_C(14)
-        NK_C(15) NK_C(16)
+      NK_C(5) NK_C(6) NK_C(7) NK_C(8) NK_C(9) NK_C(10) NK_C(11) NK_C(12) NK_C(13) NK_C(14)
+      NK_C(15) NK_C(16)
 #undef NK_C
-        default: break;
-      }
-      done = true;
+      default: break;
     }
+    done = true;
   }
   if (!done) switch (L.nh + 2) {
 #define NK_C(NL) \
@@ -1037,7 +1051,7 @@ static int gs_fin_ov(const GsMap &gs, int nh, Z *z, const R *wint, const R *wgh,
 }
 
 int fdm_set_variant(int v) {
-  NK_CHECK(v >= 0 && v <= 2, NKH_ERR_ARG, "FDM kernel variant %d (0, 1 or 2)", v);
+  NK_CHECK(v >= 0 && v <= 3, NKH_ERR_ARG, "FDM kernel variant %d (0, 1, 2 or 3)", v);
   g_fdm_variant = v;
   return NKH_OK;
 }
@@ -1254,7 +1268,13 @@ static int mg_solve_t(Mesh &mesh, FT *z, const FT *r, hipStream_t s, int crs_mod
       NK_TRY((gs_fin_ov<double, double>(mesh.gs, T.nh, z, T.wint, T.wgh, T.mswt, s)));
     } else if (merged) {
       NK_TRY(run_tnsr<3>(z, T.nh, C.e, C.nh, T.mswt, C.J, nel, s, T.e));
-      NK_TRY(gs_fin(mesh.gs, z, T.e, T.mswt, s));
+      // the final sum reads the weights as byte codes where every one has a
+      // code (1 B instead of 8 per node; the prolongation above keeps the
+      // FP64 weights: a table lookup there cost more than it saved)
+      if (T.mswt8)
+        NK_TRY(gs_fin(mesh.gs, z, T.e, T.mswt8, mg.wlut, s));
+      else
+        NK_TRY(gs_fin(mesh.gs, z, T.e, T.mswt, s));
     } else {
       NK_TRY(run_tnsr<2>(z, T.nh, C.e, C.nh, T.mswt, C.J, nel, s));
       NK_TRY(gs_op(mesh.gs, z, 4, s));

@@ -266,6 +266,34 @@ __global__ void k_mask_schwarz_wt(double *mswt, const double *mask, const double
   }
 }
 
+// Byte codes of the fine level's mask * Schwarz weights: 0 where masked,
+// else 1 + 8 (count - 1) + visits, the table entry formed as
+// k_mask_schwarz_wt forms the weight.  Any node whose weight the table does
+// not hold exactly sets *bad.
+__global__ void k_weight_codes(uint8_t *mc, const double *mswt, const double *mask,
+                               const double *cnt, const double *lut, int n, int64_t nel,
+                               int *bad) {
+  const int64_t n3 = (int64_t)n * n * n;
+  int b = 0;
+  for (int64_t l = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; l < n3 * nel;
+       l += (int64_t)gridDim.x * blockDim.x) {
+    const int q = (int)(l % n3);
+    const int i = q % n, j = (q / n) % n, k = q / (n * n);
+    auto hits = [n](int a) { return (a == 0) + (a == 1) + (a == n - 2) + (a == n - 1); };
+    const bool inx = (i >= 2 && i <= n - 3), iny = (j >= 2 && j <= n - 3);
+    const int h = hits(i) + (inx ? hits(j) : 0) + ((inx && iny) ? hits(k) : 0);
+    const double c = cnt[l];
+    int code = 0;
+    if (mask[l] != 0.0) {
+      if (c >= 1.0 && c <= 31.0 && h <= 7) code = 1 + 8 * ((int)c - 1) + h;
+      else b = 1;
+    }
+    mc[l] = (uint8_t)code;
+    if (lut[code] != mswt[l]) b = 1;
+  }
+  if (b) atomicOr(bad, 1);
+}
+
 // 1 at ghost slots that have a neighbour (the overlap count of a field of
 // ones), 0 elsewhere
 __global__ void k_ghost_ones(double *w, const int32_t *gout, int64_t n) {
@@ -524,6 +552,7 @@ static void free_level(MgLevel &L) {
     if (p) (void)hipFree(p);
   for (float *p : {L.fs32, L.fl32, L.wint32, L.wgh32, L.e32, L.mswt32, L.rwt32})
     if (p) (void)hipFree(p);
+  if (L.mswt8) (void)hipFree(L.mswt8);
   if (L.own_gs) gs_free(L.gs);
 }
 
@@ -541,7 +570,7 @@ void mg_free(Mesh &mesh) {
                   (void *)mg->gr, (void *)mg->gw, (void *)mg->gpart, (void *)mg->gsm,
                   (void *)mg->xr, (void *)mg->xp, (void *)mg->xz, (void *)mg->xw,
                   (void *)mg->xx, (void *)mg->xwt, (void *)mg->xpart, mg->xst,
-                  (void *)mg->opart, (void *)mg->wt8})
+                  (void *)mg->opart, (void *)mg->wt8, (void *)mg->wlut})
     if (p) (void)hipFree(p);
   for (hipEvent_t e : mg->tev)
     if (e) (void)hipEventDestroy(e);
@@ -767,6 +796,31 @@ int mg_setup(Mesh &mesh, const int *fbc_h, const int64_t *vertex, const double *
     hipLaunchKernelGGL(k_mask_schwarz_wt, dim3(mg_grid(L.n)), dim3(256), 0, s, L.mswt, L.mask,
                        L.e, L.nh, nel);
     NK_HIP(hipGetLastError());
+    if (l == mg.lmax) {  // L.e holds the overlap counts here
+      std::vector<double> lut(256, 0.0);
+      for (int c = 1; c <= 31; ++c)
+        for (int h = 0; h <= 7; ++h) {
+          double w = 1.0;
+          const double r = 1.0 / c;
+          for (int t = 0; t < h; ++t) w *= r;
+          lut[(size_t)(1 + 8 * (c - 1) + h)] = 1.0 * w;
+        }
+      NK_TRY(upload(&mg.wlut, lut));
+      NK_HIP(hipMalloc(&L.mswt8, (size_t)L.n));
+      int *bad = nullptr, hbad = 1;
+      NK_HIP(hipMalloc(&bad, sizeof(int)));
+      NK_HIP(hipMemsetAsync(bad, 0, sizeof(int), s));
+      hipLaunchKernelGGL(k_weight_codes, dim3(mg_grid(L.n)), dim3(256), 0, s, L.mswt8, L.mswt,
+                         L.mask, L.e, mg.wlut, L.nh, nel, bad);
+      NK_HIP(hipGetLastError());
+      NK_HIP(hipMemcpyAsync(&hbad, bad, sizeof(int), hipMemcpyDeviceToHost, s));
+      NK_HIP(hipStreamSynchronize(s));
+      (void)hipFree(bad);
+      if (hbad) {
+        (void)hipFree(L.mswt8);
+        L.mswt8 = nullptr;
+      }
+    }
   }
   // coarse space: vertex ids, Dirichlet vertices dropped via the level-1 mask
   {

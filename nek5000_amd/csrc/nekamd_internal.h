@@ -174,6 +174,17 @@ __device__ inline double block_sum_partials(const double *__restrict__ part, int
   return s;
 }
 
+// A weight array read by the multigrid transfers and the final
+// gather-scatter: FP64 or FP32 values, or one byte per node indexing a
+// table of the level's distinct weights (lut).
+__device__ __forceinline__ double wdec(const double *w, int64_t i, const double *) { return w[i]; }
+__device__ __forceinline__ double wdec(const float *w, int64_t i, const double *) {
+  return double(w[i]);
+}
+__device__ __forceinline__ double wdec(const uint8_t *w, int64_t i, const double *lut) {
+  return lut[w[i]];
+}
+
 inline int axhelm_cg_grid(const Mesh &m) {
   const int64_t g = (m.nel + elems_per_block(m.np) - 1) / elems_per_block(m.np);
   return (int)(g < kAxBlocks ? g : kAxBlocks);
@@ -259,6 +270,9 @@ int gs_assemble(const GsMap &gs, float *w, const CgState *st, hipStream_t s,
 int gs_fin(const GsMap &gs, double *z, const double *e, const double *w, hipStream_t s);
 int gs_fin(const GsMap &gs, double *z, const float *e, const double *w, hipStream_t s);
 int gs_fin(const GsMap &gs, float *z, const float *e, const float *w, hipStream_t s);
+// the same with the weights as byte codes into lut
+int gs_fin(const GsMap &gs, double *z, const double *e, const uint8_t *w, const double *lut,
+           hipStream_t s);
 // Flags per group from mask/mult; *ok when every group has a 0/1 mask equal
 // at all members and mult summing to 1 (red: kRedBlocks scratch).
 int gs_prepare(GsMap &gs, const double *mask, const double *mult, double *red, bool *ok,

@@ -93,8 +93,9 @@ def set_axhelm_tuning(lx1, variant, grid=0):
 
 
 def fdm_variant():
-    """Kernel of the single-precision local FDM solves: 0 vector FMA, 1 FP32
-    matrix cores, 2 bf16 matrix cores (FP32 accumulation)."""
+    """Kernel of the local FDM solves: 0 vector FMA, 1 FP32 matrix cores and
+    2 bf16 matrix cores (FP32 accumulation) for the single-precision
+    smoother, 3 FP64 matrix cores for the double-precision one."""
     v = ctypes.c_int()
     check(init().nkh_fdm_variant(ctypes.byref(v)), "nkh_fdm_variant")
     return v.value

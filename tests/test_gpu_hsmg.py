@@ -380,7 +380,44 @@ def test_fp32_smoother_matrix_core_variants(gpu, case):
     finally:
         gpu.set_fdm_variant(0)
     with pytest.raises(gpu.NekAmdError):
+        gpu.set_fdm_variant(4)
+    F.destroy()
+
+
+@pytest.mark.parametrize("case", [MG_CASES[0], MG_CASES[1], MG_CASES[4]])
+def test_fp64_smoother_matrix_core_variant(gpu, case):
+    """The double-precision local solves with their six contractions as FP64
+    matrix-core products (variant 3): one V-cycle within 1e-12 of the
+    oracle's h1mg_solve, the FP64 flexible CG with the oracle's iteration
+    count (+-1) and solution (1e-9); the accumulation order inside a
+    product differs from the vector kernel's, so agreement is to rounding."""
+    P, mg, F = build(gpu, case)
+    rng = np.random.default_rng(13)
+    r = P.field.dssum(P.bm1 * rng.standard_normal(P.shape))
+    zo = mg.solve(r)
+    h1 = np.ones(P.shape)
+    h2 = np.full(P.shape, 10.0)
+    b = P.bm1 * P.uniform_field(73)
+    rhs = P.field.dssum(b) * P.mask
+    uo, ito, _, _ = mg.flex_cg(rhs, h1, h2, 300, -1e-10)
+    try:
         gpu.set_fdm_variant(3)
+        assert gpu.fdm_variant() == 3
+        say(f"[fdm variant 3] {case} V-cycle")
+        zd = F.zeros()
+        F.h1mg_solve(zd, F.to_dev(r))
+        e = l2rel(host(zd), zo)
+        say(f"[fdm variant 3] V-cycle rel L2 = {e:.3e}")
+        assert e < 1e-12
+        u = F.zeros()
+        out = F.hmholtz("PRES", u, F.to_dev(b), F.to_dev(h1), F.to_dev(h2), F.mask, F.mult,
+                        -1e-10, 300, params=gpu.default_params(param42=2, param21=-1e-10))
+        e = l2rel(host(u), uo)
+        say(f"[fdm variant 3] flex-CG {out['niter']} iterations (oracle {ito}), rel L2 {e:.3e}")
+        assert out["converged"] and abs(out["niter"] - ito) <= 1
+        assert e < 1e-9
+    finally:
+        gpu.set_fdm_variant(0)
     F.destroy()
 
 
