@@ -138,12 +138,13 @@ def cpu_baseline(nel1d, lx1, deform, target_seconds, solver="hsmg"):
                         f"h1=1 h2=10: {what} ({free:.0f} free DOF) in {el:.1f} s"))
 
 
-def order_sweep():
+def order_sweep(orders=range(3, 16)):
     """Polynomial-order sweep (jittered boxes, E^(1/3) ~ 368/N so every order
     carries about 5e7 unique DOF; h1 = h2 = 1; f ~ U(-1,1)): per order the
     autotuner's choice (made on the first apply), the CG-fused axhelm
     kernel's time per kernel variant (0/1 element/plane staged, 2/3 the
-    same with the wide slab packing; HIP events of the library's timers
+    same with the wide slab packing, 4 the r/s contractions on the FP64
+    matrix cores at N = 7..15; HIP events of the library's timers
     over 20 Jacobi-PCG iterations each), and 100 fixed Jacobi-PCG iterations
     of hmholtz with the FP64 operator (its axhelm kernel time and HBM
     fraction at 12 words per node) and with the FP32 one (mixed precision).
@@ -172,7 +173,7 @@ def order_sweep():
         lib.nkh_timers_read(b"axhm", ctypes.byref(sec), ctypes.byref(calls))
         return wall, sec.value / max(calls.value, 1) * 1e3, out
     rows = []
-    for N in range(3, 16):
+    for N in orders:
         e1 = {3: 110, 15: 24}.get(N, int(round(368.0 / N)))
         t0 = time.time()
         box = BoxMesh(e1, e1, e1, N + 1, jitter=0.05, seed=N)
@@ -192,7 +193,7 @@ def order_sweep():
 
         u = torch.zeros_like(rhs)
         per_variant = {}
-        for v in (0, 1, 2, 3):
+        for v in (0, 1, 2, 3, 4):
             solver.set_axhelm_tuning(N + 1, v, 0)
             timed_pcg(F, u, rhs, one, 5)
             per_variant[f"variant{v}_cg_ms"] = timed_pcg(F, u, rhs, one, 20)[1]
@@ -436,6 +437,8 @@ def main():
                     help="instead of the bench line: polynomial-order sweep N=3..15 of the "
                          "CG-fused axhelm (both kernel variants) and 100 Jacobi-PCG "
                          "iterations with the FP64 and the FP32 operator")
+    ap.add_argument("--sweep-orders", default="3-15",
+                    help="polynomial orders of --sweep, as first-last")
     ap.add_argument("--solver", choices=("hsmg", "jacobi"), default="hsmg",
                     help="hsmg: hmholtz('PRES') -> hmh_flex_cg with h1mg (param42=2); "
                          "jacobi: hmholtz -> cggo")
@@ -447,7 +450,8 @@ def main():
               "interface nodes).", file=sys.stderr)
         sys.exit(2)
     if args.sweep:
-        order_sweep()
+        lo, hi = (int(x) for x in args.sweep_orders.split("-"))
+        order_sweep(range(lo, hi + 1))
         return
     if args.adjoint:
         print(json.dumps(adjoint_run()), flush=True)

@@ -20,10 +20,11 @@ static int wide_mask(int np) {
 // Autotuner: on the first apply of an order on a mesh of at least 2^20
 // local nodes (and not inside a graph capture), time every kernel variant
 // the order has (element- or plane-staged slabs; the orders whose NP x NP
-// slabs leave lanes idle also pack epb_wide slabs per block) with two grid
-// sizes on the call's own data -- the plain apply is element-local, so
-// every candidate writes the same au -- and keep the fastest for that
-// order; then time the variants of the CG-fused and flexible-CG-fused
+// slabs leave lanes idle also pack epb_wide slabs per block; NP = 8..16 also
+// the matrix-core contractions) with two grid sizes on the call's own data
+// -- the plain apply is element-local, so every candidate writes au (the
+// same bits, or to rounding for the matrix cores) -- and keep the fastest
+// for that order; then time the variants of the CG-fused and flexible-CG-fused
 // applies (their grid is fixed) on scratch vectors and keep the fastest of
 // each.
 static int tune_axhelm(const Mesh &m, double *au, const double *u, Coef h1, Coef h2,
@@ -42,9 +43,10 @@ static int tune_axhelm(const Mesh &m, double *au, const double *u, Coef h1, Coef
     return NKH_OK;
   };
   const int wm = wide_mask(m.np);
-  int vars[4] = {0, 1, 0, 0}, nv = 2;
+  int vars[5] = {0, 1, 0, 0, 0}, nv = 2;
   if (wm & 4) vars[nv++] = 2;
   if (wm & 8) vars[nv++] = 3;
+  if (wm & 16) vars[nv++] = 4;
   float best = 1e30f;
   AxhelmChoice pick = ch;
   for (int c = 0; c < 2 * nv; ++c) {
@@ -68,7 +70,7 @@ static int tune_axhelm(const Mesh &m, double *au, const double *u, Coef h1, Coef
     double *p = buf, *r = buf + n, *d = buf + 2 * n, *w = buf + 3 * n, *rho = buf + 4 * n;
     CgState *st = reinterpret_cast<CgState *>(buf + 4 * n + part);
     NK_HIP(hipMemsetAsync(buf, 0, bytes, s));
-    float ms[4] = {0.f, 0.f, 0.f, 0.f};
+    float ms[5] = {0.f, 0.f, 0.f, 0.f, 0.f};
     for (int c = 0; c < nv; ++c) {
       ch.variant_cg = vars[c];
       NK_TRY(timed([&] { return launch_axhelm_cg(m, w, p, r, d, h1, h2, r, st, rho, s); },
@@ -127,7 +129,7 @@ int axhelm_tuning(int np, int *variant, int *grid, int *tuned) {
 }
 
 int axhelm_set_tuning(int np, int variant, int grid) {
-  NK_CHECK(np >= 2 && np <= kMaxNP && variant >= 0 && variant <= 3 && grid >= 0,
+  NK_CHECK(np >= 2 && np <= kMaxNP && variant >= 0 && variant <= 4 && grid >= 0,
            NKH_ERR_ARG, "axhelm tuning: lx1=%d variant=%d grid=%d", np, variant, grid);
   g_axhelm_choice[np] = AxhelmChoice{variant, grid, variant, variant, true};
   return NKH_OK;

@@ -344,6 +344,204 @@ __global__ void __launch_bounds__(NP * NP * EPB_)
   }
 }
 
+// ---------------------------------------------------------------------------
+// The same operator with the r- and s-direction contractions on the matrix
+// cores (v_mfma_f64_16x16x4_f64), for NP = 8..16 (the autotuner's variant 4).
+// One element per block of whole wavefronts; thread (i, j) < NP^2 owns the
+// t-column as in k_axhelm.  Per t-plane k the contractions are 16 x 16
+// products (zero-padded beyond NP, K in chunks of 4) that give [j][i] tiles:
+//   ur^T_k[j][i] = sum_m u(m,j,k) D(i,m)     us^T_k[j][i] = sum_m D(j,m) u(i,m,k)
+//   (ar + as)^T_k[j][i] = sum_m wr(m,j,k) D(m,i) + sum_m D(m,j) ws(i,m,k)
+// the 2 NP gradient products and the NP output products dealt over the
+// block's wavefronts.  Lane l of a product holds A[l%16][4c + l/16] and
+// B[4c + l/16][l%16] of chunk c and gets D[4v + l/16][l%16] in register v
+// (the FP64 layout, checked on gfx950).  The D fragments stay in registers;
+// u, ur/wr and us/ws are staged in LDS with an odd j-pitch, so the 16 rows of
+// a fragment fall in different banks.  The t-direction terms, the geometric
+// factors and the epilogues are those of k_axhelm, in the same order; the
+// matrix cores sum each product in their own order, so au agrees with the
+// other kernels to rounding, not bit for bit.
+// ---------------------------------------------------------------------------
+typedef double nk_d4 __attribute__((ext_vector_type(4)));
+
+template <int NP>
+constexpr bool has_mfma() {
+  return NP >= 8 && NP <= 16;
+}
+template <int NP>
+constexpr int mx_threads() {
+  return (NP * NP + 63) / 64 * 64;
+}
+
+template <int NP, int MODE, bool H1F, int H2M>
+__global__ void __launch_bounds__(mx_threads<NP>())
+    k_axhelm_mx(DMat<NP> Dm, const double *__restrict__ Dg, double *__restrict__ au,
+                const double *__restrict__ uin, const double *__restrict__ g, int64_t gstride,
+                const double *__restrict__ h1f, double h1s, const double *__restrict__ h2f,
+                double h2s, const double *__restrict__ bm1, int64_t nel,
+                double *__restrict__ pout, const double *__restrict__ r,
+                const double *__restrict__ dinv, const double *__restrict__ mult,
+                const CgState *__restrict__ st, double *__restrict__ rho_part,
+                const double *__restrict__ betap) {
+  static_assert(has_mfma<NP>(), "one 16 x 16 tile per plane");
+  constexpr int NP2 = NP * NP, NP3 = NP2 * NP, NW = mx_threads<NP>() / 64;
+  constexpr int PJ = NP | 1, PK = PJ * NP, KC = (NP + 3) / 4;
+  __shared__ double s_u[PK * NP];   // u, then ar + as
+  __shared__ double s_wr[PK * NP];  // ur, then wr
+  __shared__ double s_ws[PK * NP];  // us, then ws
+  double beta = 0.0;
+  if constexpr (MODE == 1) {
+    if (st->done) return;
+    beta = st->beta;
+  }
+  if constexpr (MODE == 2) beta = *betap;
+  if constexpr (MODE == 3) {
+    if (st->done) return;
+  }
+  const int tid = threadIdx.x;
+  const bool own = tid < NP2;
+  const int i = own ? tid % NP : 0, j = own ? tid / NP : 0;
+  const int lane = tid & 63, wv = tid >> 6, lr = lane & 15, lg = lane >> 4;
+  const bool lin = lr < NP;
+  // Dn[c] = D(lr, 4c + lg), Dt[c] = D(4c + lg, lr); 0 outside NP x NP
+  double Dn[KC], Dt[KC];
+#pragma unroll
+  for (int c = 0; c < KC; ++c) {
+    const int m = 4 * c + lg;
+    const bool in = lin && m < NP;
+    Dn[c] = in ? Dg[lr + NP * m] : 0.0;
+    Dt[c] = in ? Dg[m + NP * lr] : 0.0;
+  }
+  const int own_at = i + PJ * j;  // the owner's node in a plane of LDS
+  double rho = 0.0;
+  for (int64_t e = blockIdx.x; e < nel; e += gridDim.x) {
+    const int64_t base = e * NP3 + tid;
+    double r_u[NP], r_au[NP];
+    __syncthreads();  // the previous element's LDS reads are done
+#pragma unroll
+    for (int k = 0; k < NP; ++k) {
+      double v = 0.0;
+      if (own) {
+        const int64_t l = base + k * NP2;
+        if constexpr (MODE == 1) {
+          v = beta * ldnt(uin + l) + ldnt(dinv + l) * ldnt(r + l);
+          stnt(v, pout + l);
+        } else if constexpr (MODE == 2) {
+          v = beta * ldnt(uin + l) + ldnt(r + l);
+          stnt(v, pout + l);
+        } else {
+          v = ldnt(uin + l);
+        }
+        s_u[k * PK + own_at] = v;
+      }
+      r_u[k] = v;
+      r_au[k] = 0.0;
+    }
+    __syncthreads();
+    // ur and us of every plane: product p = 2k + (0: ur, 1: us)
+    for (int p = wv; p < 2 * NP; p += NW) {
+      const int k = p >> 1;
+      const double *uk = s_u + k * PK;
+      nk_d4 acc = {0.0, 0.0, 0.0, 0.0};
+      if ((p & 1) == 0) {
+#pragma unroll
+        for (int c = 0; c < KC; ++c) {
+          const int m = 4 * c + lg;
+          const double a = (lin && m < NP) ? uk[PJ * lr + m] : 0.0;
+          acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a, Dn[c], acc, 0, 0, 0);
+        }
+      } else {
+#pragma unroll
+        for (int c = 0; c < KC; ++c) {
+          const int m = 4 * c + lg;
+          const double b = (lin && m < NP) ? uk[PJ * m + lr] : 0.0;
+          acc = __builtin_amdgcn_mfma_f64_16x16x4f64(Dn[c], b, acc, 0, 0, 0);
+        }
+      }
+      double *dst = ((p & 1) == 0 ? s_wr : s_ws) + k * PK;
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const int jj = 4 * v + lg;
+        if (lin && jj < NP) dst[PJ * jj + lr] = acc[v];
+      }
+    }
+    __syncthreads();
+    if (own) {
+#pragma unroll
+      for (int k = 0; k < NP; ++k) {
+        const int at = k * PK + own_at;
+        const double ur = s_wr[at], us = s_ws[at];
+        double ut = 0.0;
+#pragma unroll
+        for (int m = 0; m < NP; ++m) ut += Dm.d[k + NP * m] * r_u[m];
+        const int64_t l = base + k * NP2;
+        const double G1 = ldnt(g + l), G2 = ldnt(g + gstride + l),
+                     G3 = ldnt(g + 2 * gstride + l);
+        const double G4 = ldnt(g + 3 * gstride + l), G5 = ldnt(g + 4 * gstride + l),
+                     G6 = ldnt(g + 5 * gstride + l);
+        double wr = ur * G1;
+        double ws = us * G2;
+        double wt = ut * G3;
+        wr = wr + us * G4;
+        wr = wr + ut * G5;
+        ws = ws + ur * G4;
+        ws = ws + ut * G6;
+        wt = wt + ur * G5;
+        wt = wt + us * G6;
+        const double h1 = H1F ? ldnt(h1f + l) : h1s;
+        wr *= h1;
+        ws *= h1;
+        wt *= h1;
+        s_wr[at] = wr;
+        s_ws[at] = ws;
+#pragma unroll
+        for (int kk = 0; kk < NP; ++kk) r_au[kk] += Dm.d[k + NP * kk] * wt;
+      }
+    }
+    __syncthreads();
+    // ar + as of every plane into s_u (u itself is in the owners' registers)
+    for (int k = wv; k < NP; k += NW) {
+      const double *wrk = s_wr + k * PK, *wsk = s_ws + k * PK;
+      nk_d4 acc = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+      for (int c = 0; c < KC; ++c) {
+        const int m = 4 * c + lg;
+        const double a = (lin && m < NP) ? wrk[PJ * lr + m] : 0.0;
+        acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a, Dt[c], acc, 0, 0, 0);
+      }
+#pragma unroll
+      for (int c = 0; c < KC; ++c) {
+        const int m = 4 * c + lg;
+        const double b = (lin && m < NP) ? wsk[PJ * m + lr] : 0.0;
+        acc = __builtin_amdgcn_mfma_f64_16x16x4f64(Dt[c], b, acc, 0, 0, 0);
+      }
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const int jj = 4 * v + lg;
+        if (lin && jj < NP) s_u[k * PK + PJ * jj + lr] = acc[v];
+      }
+    }
+    __syncthreads();
+    if (own) {
+#pragma unroll
+      for (int k = 0; k < NP; ++k) {
+        const int64_t l = base + k * NP2;
+        double a = s_u[k * PK + own_at] + r_au[k];
+        if constexpr (H2M == 1) a = a + (h2s * ldnt(bm1 + l)) * r_u[k];
+        if constexpr (H2M == 2) a = a + (ldnt(h2f + l) * ldnt(bm1 + l)) * r_u[k];
+        stnt(a, au + l);
+        if constexpr (MODE == 1 || MODE == 2) rho += a * r_u[k];
+      }
+    }
+  }
+  if constexpr (MODE == 1 || MODE == 2) {
+    if (rho_part != nullptr) {
+      const double t = block_sum(rho, s_u);  // after the last element's reads of s_u
+      if (threadIdx.x == 0) rho_part[blockIdx.x] = t;
+    }
+  }
+}
+
 // Per polynomial order: which of the two kernels (0: element staged in LDS,
 // 1: t-planes) for the plain apply (and its gated form), the CG-fused and
 // the flexible-CG-fused apply, and how many blocks for the plain apply;
@@ -358,8 +556,9 @@ struct AxhelmChoice {
 inline AxhelmChoice g_axhelm_choice[kMaxNP + 1];
 
 // One launch of the axhelm kernel (element- or plane-staged, EPB slabs per
-// block) for the apply MODE.
-template <int NP, int MODE, bool PLANE, int EPB>
+// block; MX: the matrix-core kernel, one element per block) for the apply
+// MODE.
+template <int NP, int MODE, bool PLANE, int EPB, bool MX = false>
 static int axhelm_go(const Mesh &m, const DMat<NP> &Dm, double *au, const double *u, Coef h1,
                      Coef h2, double *pout, const double *r, const double *dinv,
                      const double *mult, const CgState *st, double *rho_part, hipStream_t s,
@@ -380,12 +579,16 @@ static int axhelm_go(const Mesh &m, const DMat<NP> &Dm, double *au, const double
     grid = (int)(ngroups < cap ? ngroups : cap);
   }
   if (grid < 1) grid = 1;
-  const dim3 block(NP * NP * EPB);
+  const dim3 block(MX ? mx_threads<NP>() : NP * NP * EPB);
   // h2 mode: 0 none (reference ifh2 false), 1 uniform, 2 field
   const int h2m = h2.uniform() ? (h2.s != 0.0 ? 1 : 0) : 2;
   const bool h1f = !h1.uniform();
 #define NK_LAUNCH(H1F_, H2M_)                                                                \
-  if constexpr (PLANE)                                                                        \
+  if constexpr (MX)                                                                           \
+    hipLaunchKernelGGL((k_axhelm_mx<NP, MODE, H1F_, H2M_>), dim3(grid), block, 0, s, Dm,      \
+                       m.Dd, au, u, m.g, m.n, h1.f, h1.s, h2.f, h2.s, m.bm1, m.nel, pout, r,   \
+                       dinv, mult, st, rho_part, betap);                                      \
+  else if constexpr (PLANE)                                                                   \
     hipLaunchKernelGGL((k_axhelm_pl<NP, MODE, H1F_, H2M_, EPB>), dim3(grid), block, 0, s, Dm, \
                        m.Dd, au, u, m.g, m.n, h1.f, h1.s, h2.f, h2.s, m.bm1, m.nel, pout, r,   \
                        dinv, mult, st, rho_part, betap);                                      \
@@ -408,12 +611,14 @@ static int axhelm_go(const Mesh &m, const DMat<NP> &Dm, double *au, const double
 }
 
 // Variants: 0 element-staged, 1 plane-staged (epb slabs per block), 2
-// plane-staged and 3 element-staged with the wide packing (epb_wide); an
-// order without a distinct wide packing runs 1 for 2 and 0 for 3, as does
-// 3 where the wide element-staged kernel would not fit two blocks per CU.
-// The plain apply is element-local, so every variant gives the same au bit
-// for bit; the fused applies' partial sums of p.w group the elements by
-// block, so the wide packings round those sums differently.
+// plane-staged and 3 element-staged with the wide packing (epb_wide), 4 the
+// r/s contractions on the matrix cores (NP = 8..16); an order without a
+// distinct wide packing runs 1 for 2 and 0 for 3, as does 3 where the wide
+// element-staged kernel would not fit two blocks per CU, and an order
+// outside 8..16 runs 0 for 4.  The plain apply is element-local, so
+// variants 0-3 give the same au bit for bit; the fused applies' partial
+// sums of p.w group the elements by block, so the wide packings round those
+// sums differently; variant 4 agrees with the others to rounding.
 template <int NP, int MODE>
 static int axhelm_dispatch(const Mesh &m, double *au, const double *u, Coef h1, Coef h2,
                            double *pout, const double *r, const double *dinv,
@@ -435,15 +640,22 @@ static int axhelm_dispatch(const Mesh &m, double *au, const double *u, Coef h1, 
     if constexpr (has_wide_elem<NP>()) NK_GO(false, epb_wide<NP>());
     else NK_GO(false, epb<NP>());
   }
+  if (var == 4) {
+    if constexpr (has_mfma<NP>())
+      return axhelm_go<NP, MODE, false, 1, true>(m, Dm, au, u, h1, h2, pout, r, dinv, mult, st,
+                                                 rho_part, s, betap, gc);
+    else NK_GO(false, epb<NP>());
+  }
   if (var == 1) NK_GO(true, epb<NP>());
   NK_GO(false, epb<NP>());
 #undef NK_GO
 }
 
-// which variants an order has beyond 0 and 1 (bit 2: variant 2, bit 3: 3)
+// which variants an order has beyond 0 and 1 (bit 2: variant 2, bit 3: 3,
+// bit 4: 4)
 template <int NP>
 constexpr int axhelm_wide_mask() {
-  return (has_wide<NP>() ? 4 : 0) | (has_wide_elem<NP>() ? 8 : 0);
+  return (has_wide<NP>() ? 4 : 0) | (has_wide_elem<NP>() ? 8 : 0) | (has_mfma<NP>() ? 16 : 0);
 }
 
 #define NK_NP_SWITCH(np, CALL)                                      \

@@ -1,13 +1,15 @@
 """The axhelm kernels (element staged in LDS / one t-plane at a time, each
-with the default or the wide packing of thread slabs per block) and the
-per-order autotuner that picks between them, through the C ABI:
+with the default or the wide packing of thread slabs per block; the r/s
+contractions on the FP64 matrix cores) and the per-order autotuner that
+picks between them, through the C ABI:
 
-  * the plain apply of every variant is bit-identical and within 1e-13 of
-    the oracle, for every coefficient mode, at orders 3..16;
+  * the plain apply of variants 0-3 is bit-identical and within 1e-13 of
+    the oracle, for every coefficient mode, at orders 3..16; the matrix-core
+    variant 4 within 1e-13 of the oracle and of variant 0;
   * the Jacobi PCG (the rho-fused apply) and the multigrid flexible CG (the
     p-update-fused apply) give bit-identical iterates with variants 0 and 1,
-    and with the wide packings the same iteration counts and iterates up to
-    the rounding of the regrouped partial sums;
+    and with the wide packings and the matrix cores the same iteration
+    counts and iterates up to rounding;
   * the first apply of an order on a mesh of >= 2^20 nodes tunes it, after
     which the choice is reported and the results do not depend on it.
 """
@@ -50,15 +52,22 @@ def test_plain_apply_variants_identical(gpu, lx1):
                             (h1f, h2f, "field h1,h2")):
             ref = P.field.axhelm(u, h1, h2)
             outs = []
-            for variant, grid in ((0, 0), (1, 0), (1, 2), (2, 0), (2, 3), (3, 0)):
+            for variant, grid in ((0, 0), (1, 0), (1, 2), (2, 0), (2, 3), (3, 0), (4, 0),
+                                  (4, 5)):
                 say(f"[axhelm variants] lx1={lx1} {tag} variant={variant} grid={grid}")
                 gpu.set_axhelm_tuning(lx1, variant, grid)
                 au = F.zeros()
                 F.axhelm(au, F.to_dev(u), F.to_dev(h1), F.to_dev(h2))
                 outs.append(host(au))
             assert rel(outs[0], ref) < 1e-13, tag
-            for o in outs[1:]:
+            for o in outs[1:6]:
                 assert np.array_equal(o, outs[0]), tag
+            # the matrix cores sum each contraction in their own order
+            for o in outs[6:]:
+                assert rel(o, ref) < 1e-13 and rel(o, outs[0]) < 1e-13, tag
+                if lx1 < 8:  # no matrix-core kernel: variant 0 runs
+                    assert np.array_equal(o, outs[0]), tag
+            assert np.array_equal(outs[6], outs[7]), tag
     finally:
         gpu.set_axhelm_tuning(lx1, saved[0], saved[1])
     F.destroy()
@@ -90,41 +99,44 @@ def test_jacobi_pcg_variants_identical(gpu):
     F.destroy()
 
 
-def test_flex_cg_variants_identical(gpu):
-    b = box((3, 3, 3), 7, deform=0.1)
+@pytest.mark.parametrize("lx1", [7, 9])
+def test_flex_cg_variants_identical(gpu, lx1):
+    b = box((3, 3, 3), lx1, deform=0.1)
     P = Problem(b)
     F = gpu.box_field(b)
     x, y, z = b.xyz()
     F.h1mg_setup(b.fast_bc(), b.vertex, x, y, z)
     h1, h2 = F.zeros() + 1.0, F.zeros()
     rhs = F.to_dev(P.field.dssum(P.bm1 * P.uniform_field(3)) * P.mask)
-    saved = gpu.axhelm_tuning(7)
+    saved = gpu.axhelm_tuning(lx1)
     outs = []
     try:
-        for variant in (0, 1, 2, 3):
-            say(f"[flex-cg variants] variant={variant}")
-            gpu.set_axhelm_tuning(7, variant, 0)
+        for variant in (0, 1, 2, 3, 4):
+            say(f"[flex-cg variants] lx1={lx1} variant={variant}")
+            gpu.set_axhelm_tuning(lx1, variant, 0)
             res = rhs.clone()
             prm = gpu.default_params(istep=1, param42=2, param21=-1e-10)
             out = F.hmh_flex_cg(res, h1, h2, F.mult, 100, params=prm, history=True)
             assert out["converged"]
             outs.append((host(res), out))
     finally:
-        gpu.set_axhelm_tuning(7, saved[0], saved[1])
+        gpu.set_axhelm_tuning(lx1, saved[0], saved[1])
     assert outs[0][1]["niter"] == outs[1][1]["niter"]
     assert np.array_equal(outs[0][0], outs[1][0])
     # the wide packings (lx1 = 7: nine slabs per block) round the partial
-    # sums of p.w in another grouping
+    # sums of p.w in another grouping, the matrix cores (lx1 = 9) the
+    # contractions
     for o in outs[2:]:
         assert o[1]["niter"] == outs[0][1]["niter"]
         assert rel(o[0], outs[0][0]) < 1e-10
     F.destroy()
 
 
-@pytest.mark.parametrize("lx1", [5, 10, 13])
+@pytest.mark.parametrize("lx1", [5, 10, 13, 16])
 def test_jacobi_pcg_wide_variants(gpu, lx1):
-    """variants 2 and 3 in the CG-fused apply: the same iterates as 0 up to
-    the rounding of the partial sums"""
+    """variants 2, 3 and 4 in the CG-fused apply: the same iterates as 0 up
+    to the rounding of the partial sums (and of the matrix-core
+    contractions)"""
     b = box((3, 3, 2), lx1, deform=0.1)
     P = Problem(b)
     F = gpu.box_field(b)
@@ -134,7 +146,7 @@ def test_jacobi_pcg_wide_variants(gpu, lx1):
     saved = gpu.axhelm_tuning(lx1)
     outs = []
     try:
-        for variant in (0, 2, 3):
+        for variant in (0, 2, 3, 4):
             say(f"[pcg wide variants] lx1={lx1} variant={variant}")
             gpu.set_axhelm_tuning(lx1, variant, 0)
             u = F.zeros()
@@ -171,14 +183,17 @@ def test_autotuner_picks_and_keeps_results(gpu):
     variant, grid, tuned = gpu.axhelm_tuning(lx1)
     v_cg, v_flex = gpu.axhelm_tuning_fused(lx1)
     say(f"[autotune] chosen variant={variant} grid={grid} tuned={tuned}, fused {v_cg} {v_flex}")
-    assert tuned and variant in (0, 1, 2, 3) and grid >= 0 and v_cg in (0, 1, 2, 3)
-    assert v_flex in (0, 1, 2, 3)
+    assert tuned and variant in (0, 1, 2, 3, 4) and grid >= 0 and v_cg in (0, 1, 2, 3, 4)
+    assert v_flex in (0, 1, 2, 3, 4)
     a_tuned = host(au)
-    for v in (0, 1, 2, 3):
+    for v in (0, 1, 2, 3, 4):
         gpu.set_axhelm_tuning(lx1, v, 0)
         au2 = F.zeros()
         F.axhelm(au2, u, h1, h2)
-        assert np.array_equal(host(au2), a_tuned)
+        if (v == 4) == (variant == 4):
+            assert np.array_equal(host(au2), a_tuned)
+        else:  # matrix cores against vector FMA: equal to rounding
+            assert rel(host(au2), a_tuned) < 1e-13
     gpu.set_axhelm_tuning(lx1, variant, grid)
     assert gpu.axhelm_tuning_fused(lx1) == (variant, variant)
     F.destroy()
@@ -188,7 +203,7 @@ def test_tuning_arguments_checked(gpu):
     with pytest.raises(gpu.NekAmdError):
         gpu.set_axhelm_tuning(17, 0, 0)
     with pytest.raises(gpu.NekAmdError):
-        gpu.set_axhelm_tuning(8, 4, 0)
+        gpu.set_axhelm_tuning(8, 5, 0)
     with pytest.raises(gpu.NekAmdError):
         gpu.set_axhelm_tuning(8, 0, -1)
     with pytest.raises(gpu.NekAmdError):
