@@ -143,8 +143,9 @@ def order_sweep(orders=range(3, 16)):
     carries about 5e7 unique DOF; h1 = h2 = 1; f ~ U(-1,1)): per order the
     autotuner's choice (made on the first apply), the CG-fused axhelm
     kernel's time per kernel variant (0/1 element/plane staged, 2/3 the
-    same with the wide slab packing, 4 the r/s contractions on the FP64
-    matrix cores at N = 7..15; HIP events of the library's timers
+    same with the wide slab packing, 4/5 the r/s contractions on the FP64
+    matrix cores with the element / groups of t-planes staged, at N = 7..15;
+    HIP events of the library's timers
     over 20 Jacobi-PCG iterations each), and 100 fixed Jacobi-PCG iterations
     of hmholtz with the FP64 operator (its axhelm kernel time and HBM
     fraction at 12 words per node) and with the FP32 one (mixed precision).
@@ -193,7 +194,7 @@ def order_sweep(orders=range(3, 16)):
 
         u = torch.zeros_like(rhs)
         per_variant = {}
-        for v in (0, 1, 2, 3, 4):
+        for v in (0, 1, 2, 3, 4, 5):
             solver.set_axhelm_tuning(N + 1, v, 0)
             timed_pcg(F, u, rhs, one, 5)
             per_variant[f"variant{v}_cg_ms"] = timed_pcg(F, u, rhs, one, 20)[1]
@@ -543,7 +544,8 @@ def main():
     # the kernel the autotuner chose for this order's fused apply in the
     # warm-up solve
     v_cg, v_flex = solver.axhelm_tuning_fused(args.lx1)
-    kname = "k_axhelm_pl" if (v_flex if hsmg else v_cg) == 1 else "k_axhelm"
+    kname = {0: "k_axhelm", 1: "k_axhelm_pl", 2: "k_axhelm_pl", 3: "k_axhelm",
+             4: "k_axhelm_mx", 5: "k_axhelm_mxp"}[v_flex if hsmg else v_cg]
     if hsmg:
         # axhelm of ax (gmres.f:284) with the p update fused: reads p, z,
         # g1..g6, bm1; writes p, w

@@ -180,11 +180,12 @@ int nkd_axhelm(int mesh, double *au, const double *u, const double *h1, const do
    1 stages one t-plane at a time (3 lx1^2 instead of 3 lx1^3 doubles of LDS);
    2 (plane) and 3 (element) pack more lx1 x lx1 thread slabs per block where
    one leaves lanes of its last wavefront idle (lx1 = 5-7, 9, 10, 12-14;
-   elsewhere they run 1 and 0); 4 runs the r- and s-direction contractions
-   as FP64 matrix-core products (v_mfma_f64_16x16x4_f64, lx1 = 8-16; elsewhere
-   it runs 0).  Variants 0-3 give the same plain apply bit for bit, 4 the
-   same to rounding; the fused CG applies of 2 and 3 group their partial sums
-   differently (iterates equal to rounding).  grid = blocks of the plain
+   elsewhere they run 1 and 0); 4 (element in LDS) and 5 (groups of t-planes
+   in LDS) run the r- and s-direction contractions as FP64 matrix-core
+   products (v_mfma_f64_16x16x4_f64, lx1 = 8-16; elsewhere they run 0).
+   Variants 0-3 give the same plain apply bit for bit, as do 4 and 5, the two
+   sets agreeing to rounding; the fused CG applies of 2 and 3 group their
+   partial sums differently (iterates equal to rounding).  grid = blocks of the plain
    apply (0: one per element group).  Unless set, the first nkd_axhelm or
    solve of an order on a mesh of >= 2^20 nodes times the order's variants
    at two grids and keeps the fastest (*tuned = 1 afterwards). */

@@ -21,7 +21,8 @@ static int wide_mask(int np) {
 // local nodes (and not inside a graph capture), time every kernel variant
 // the order has (element- or plane-staged slabs; the orders whose NP x NP
 // slabs leave lanes idle also pack epb_wide slabs per block; NP = 8..16 also
-// the matrix-core contractions) with two grid sizes on the call's own data
+// the matrix-core contractions, element- or plane-group-staged) with two
+// grid sizes on the call's own data
 // -- the plain apply is element-local, so every candidate writes au (the
 // same bits, or to rounding for the matrix cores) -- and keep the fastest
 // for that order; then time the variants of the CG-fused and flexible-CG-fused
@@ -33,20 +34,25 @@ static int tune_axhelm(const Mesh &m, double *au, const double *u, Coef h1, Coef
   hipEvent_t e0, e1;
   NK_HIP(hipEventCreate(&e0));
   NK_HIP(hipEventCreate(&e1));
+  // one warm-up launch, then three timed back to back (their sum: a single
+  // launch's time varies by a few per cent, as much as the variants differ)
   auto timed = [&](auto &&launch, float *ms) -> int {
-    NK_TRY(launch());  // warm-up
-    NK_HIP(hipEventRecord(e0, s));
     NK_TRY(launch());
+    NK_HIP(hipEventRecord(e0, s));
+    for (int rep = 0; rep < 3; ++rep) NK_TRY(launch());
     NK_HIP(hipEventRecord(e1, s));
     NK_HIP(hipEventSynchronize(e1));
     NK_HIP(hipEventElapsedTime(ms, e0, e1));
     return NKH_OK;
   };
   const int wm = wide_mask(m.np);
-  int vars[5] = {0, 1, 0, 0, 0}, nv = 2;
+  int vars[6] = {0, 1, 0, 0, 0, 0}, nv = 2;
   if (wm & 4) vars[nv++] = 2;
   if (wm & 8) vars[nv++] = 3;
-  if (wm & 16) vars[nv++] = 4;
+  if (wm & 16) {
+    vars[nv++] = 4;
+    vars[nv++] = 5;
+  }
   float best = 1e30f;
   AxhelmChoice pick = ch;
   for (int c = 0; c < 2 * nv; ++c) {
@@ -70,7 +76,7 @@ static int tune_axhelm(const Mesh &m, double *au, const double *u, Coef h1, Coef
     double *p = buf, *r = buf + n, *d = buf + 2 * n, *w = buf + 3 * n, *rho = buf + 4 * n;
     CgState *st = reinterpret_cast<CgState *>(buf + 4 * n + part);
     NK_HIP(hipMemsetAsync(buf, 0, bytes, s));
-    float ms[5] = {0.f, 0.f, 0.f, 0.f, 0.f};
+    float ms[6] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
     for (int c = 0; c < nv; ++c) {
       ch.variant_cg = vars[c];
       NK_TRY(timed([&] { return launch_axhelm_cg(m, w, p, r, d, h1, h2, r, st, rho, s); },
@@ -129,7 +135,7 @@ int axhelm_tuning(int np, int *variant, int *grid, int *tuned) {
 }
 
 int axhelm_set_tuning(int np, int variant, int grid) {
-  NK_CHECK(np >= 2 && np <= kMaxNP && variant >= 0 && variant <= 4 && grid >= 0,
+  NK_CHECK(np >= 2 && np <= kMaxNP && variant >= 0 && variant <= 5 && grid >= 0,
            NKH_ERR_ARG, "axhelm tuning: lx1=%d variant=%d grid=%d", np, variant, grid);
   g_axhelm_choice[np] = AxhelmChoice{variant, grid, variant, variant, true};
   return NKH_OK;

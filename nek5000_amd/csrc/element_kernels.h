@@ -542,6 +542,197 @@ __global__ void __launch_bounds__(mx_threads<NP>())
   }
 }
 
+// The matrix-core operator staged G t-planes at a time (G = the block's
+// wavefronts; the autotuner's variant 5): 3 G NP^2 doubles of LDS instead of
+// 3 NP^3 (NP = 16: 26 KB, not 104), so several blocks share a CU.  Per group
+// of planes each wavefront forms ur and us of its plane, the owners do the
+// pointwise work of the group's planes (t-direction terms in registers as in
+// k_axhelm_pl), each wavefront forms ar + as of its plane, and the owners
+// keep those in registers until the t-direction sums are complete.  The
+// products, their layouts and the arithmetic order are those of
+// k_axhelm_mx, so the two give the same au bit for bit.
+template <int NP, int MODE, bool H1F, int H2M>
+__global__ void __launch_bounds__(mx_threads<NP>())
+    k_axhelm_mxp(DMat<NP> Dm, const double *__restrict__ Dg, double *__restrict__ au,
+                 const double *__restrict__ uin, const double *__restrict__ g, int64_t gstride,
+                 const double *__restrict__ h1f, double h1s, const double *__restrict__ h2f,
+                 double h2s, const double *__restrict__ bm1, int64_t nel,
+                 double *__restrict__ pout, const double *__restrict__ r,
+                 const double *__restrict__ dinv, const double *__restrict__ mult,
+                 const CgState *__restrict__ st, double *__restrict__ rho_part,
+                 const double *__restrict__ betap) {
+  static_assert(has_mfma<NP>(), "one 16 x 16 tile per plane");
+  constexpr int NP2 = NP * NP, NP3 = NP2 * NP, NW = mx_threads<NP>() / 64, G = NW;
+  constexpr int PJ = NP | 1, PP = PJ * NP, KC = (NP + 3) / 4, NG = (NP + G - 1) / G;
+  __shared__ double s_u[G * PP];   // the group's planes of u, then ar + as
+  __shared__ double s_wr[G * PP];  // ur, then wr
+  __shared__ double s_ws[G * PP];  // us, then ws
+  double beta = 0.0;
+  if constexpr (MODE == 1) {
+    if (st->done) return;
+    beta = st->beta;
+  }
+  if constexpr (MODE == 2) beta = *betap;
+  if constexpr (MODE == 3) {
+    if (st->done) return;
+  }
+  const int tid = threadIdx.x;
+  const bool own = tid < NP2;
+  const int i = own ? tid % NP : 0, j = own ? tid / NP : 0;
+  const int lane = tid & 63, wv = tid >> 6, lr = lane & 15, lg = lane >> 4;
+  const bool lin = lr < NP;
+  double Dn[KC], Dt[KC];
+#pragma unroll
+  for (int c = 0; c < KC; ++c) {
+    const int m = 4 * c + lg;
+    const bool in = lin && m < NP;
+    Dn[c] = in ? Dg[lr + NP * m] : 0.0;
+    Dt[c] = in ? Dg[m + NP * lr] : 0.0;
+  }
+  const int own_at = i + PJ * j;
+  double *const wu = s_u + wv * PP, *const wwr = s_wr + wv * PP, *const wws = s_ws + wv * PP;
+  double rho = 0.0;
+  for (int64_t e = blockIdx.x; e < nel; e += gridDim.x) {
+    const int64_t base = e * NP3 + tid;
+    double r_u[NP], r_au[NP], r_o[NP];
+#pragma unroll
+    for (int k = 0; k < NP; ++k) {
+      double v = 0.0;
+      if (own) {
+        const int64_t l = base + k * NP2;
+        if constexpr (MODE == 1) {
+          v = beta * ldnt(uin + l) + ldnt(dinv + l) * ldnt(r + l);
+          stnt(v, pout + l);
+        } else if constexpr (MODE == 2) {
+          v = beta * ldnt(uin + l) + ldnt(r + l);
+          stnt(v, pout + l);
+        } else {
+          v = ldnt(uin + l);
+        }
+      }
+      r_u[k] = v;
+      r_au[k] = 0.0;
+      r_o[k] = 0.0;
+    }
+#pragma unroll
+    for (int gq = 0; gq < NG; ++gq) {
+      __builtin_amdgcn_sched_barrier(0);
+      __syncthreads();  // the previous group's (or element's) LDS reads are done
+      if (own) {
+#pragma unroll
+        for (int q = 0; q < G; ++q)
+          if (gq * G + q < NP) s_u[q * PP + own_at] = r_u[gq * G + q];
+      }
+      __syncthreads();
+      const bool act = gq * G + wv < NP;  // wave-uniform: this wave's plane exists
+      if (act) {
+        nk_d4 acc = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+        for (int c = 0; c < KC; ++c) {
+          const int m = 4 * c + lg;
+          const double a = (lin && m < NP) ? wu[PJ * lr + m] : 0.0;
+          acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a, Dn[c], acc, 0, 0, 0);
+        }
+        nk_d4 acs = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+        for (int c = 0; c < KC; ++c) {
+          const int m = 4 * c + lg;
+          const double b = (lin && m < NP) ? wu[PJ * m + lr] : 0.0;
+          acs = __builtin_amdgcn_mfma_f64_16x16x4f64(Dn[c], b, acs, 0, 0, 0);
+        }
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+          const int jj = 4 * v + lg;
+          if (lin && jj < NP) {
+            wwr[PJ * jj + lr] = acc[v];
+            wws[PJ * jj + lr] = acs[v];
+          }
+        }
+      }
+      __syncthreads();
+      if (own) {
+#pragma unroll
+        for (int q = 0; q < G; ++q) {
+          const int k = gq * G + q;
+          if (k < NP) {
+            const int at = q * PP + own_at;
+            const double ur = s_wr[at], us = s_ws[at];
+            double ut = 0.0;
+#pragma unroll
+            for (int m = 0; m < NP; ++m) ut += Dm.d[k + NP * m] * r_u[m];
+            const int64_t l = base + k * NP2;
+            const double G1 = ldnt(g + l), G2 = ldnt(g + gstride + l),
+                         G3 = ldnt(g + 2 * gstride + l);
+            const double G4 = ldnt(g + 3 * gstride + l), G5 = ldnt(g + 4 * gstride + l),
+                         G6 = ldnt(g + 5 * gstride + l);
+            double wr = ur * G1;
+            double ws = us * G2;
+            double wt = ut * G3;
+            wr = wr + us * G4;
+            wr = wr + ut * G5;
+            ws = ws + ur * G4;
+            ws = ws + ut * G6;
+            wt = wt + ur * G5;
+            wt = wt + us * G6;
+            const double h1 = H1F ? ldnt(h1f + l) : h1s;
+            wr *= h1;
+            ws *= h1;
+            wt *= h1;
+            s_wr[at] = wr;
+            s_ws[at] = ws;
+#pragma unroll
+            for (int kk = 0; kk < NP; ++kk) r_au[kk] += Dm.d[k + NP * kk] * wt;
+          }
+        }
+      }
+      __syncthreads();
+      if (act) {
+        nk_d4 acc = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+        for (int c = 0; c < KC; ++c) {
+          const int m = 4 * c + lg;
+          const double a = (lin && m < NP) ? wwr[PJ * lr + m] : 0.0;
+          acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a, Dt[c], acc, 0, 0, 0);
+        }
+#pragma unroll
+        for (int c = 0; c < KC; ++c) {
+          const int m = 4 * c + lg;
+          const double b = (lin && m < NP) ? wws[PJ * m + lr] : 0.0;
+          acc = __builtin_amdgcn_mfma_f64_16x16x4f64(Dt[c], b, acc, 0, 0, 0);
+        }
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+          const int jj = 4 * v + lg;
+          if (lin && jj < NP) wu[PJ * jj + lr] = acc[v];
+        }
+      }
+      __syncthreads();
+      if (own) {
+#pragma unroll
+        for (int q = 0; q < G; ++q)
+          if (gq * G + q < NP) r_o[gq * G + q] = s_u[q * PP + own_at];
+      }
+    }
+    if (own) {
+#pragma unroll
+      for (int k = 0; k < NP; ++k) {
+        const int64_t l = base + k * NP2;
+        double a = r_o[k] + r_au[k];
+        if constexpr (H2M == 1) a = a + (h2s * ldnt(bm1 + l)) * r_u[k];
+        if constexpr (H2M == 2) a = a + (ldnt(h2f + l) * ldnt(bm1 + l)) * r_u[k];
+        stnt(a, au + l);
+        if constexpr (MODE == 1 || MODE == 2) rho += a * r_u[k];
+      }
+    }
+  }
+  if constexpr (MODE == 1 || MODE == 2) {
+    if (rho_part != nullptr) {
+      const double t = block_sum(rho, s_u);  // after the last group's reads of s_u
+      if (threadIdx.x == 0) rho_part[blockIdx.x] = t;
+    }
+  }
+}
+
 // Per polynomial order: which of the two kernels (0: element staged in LDS,
 // 1: t-planes) for the plain apply (and its gated form), the CG-fused and
 // the flexible-CG-fused apply, and how many blocks for the plain apply;
@@ -556,9 +747,9 @@ struct AxhelmChoice {
 inline AxhelmChoice g_axhelm_choice[kMaxNP + 1];
 
 // One launch of the axhelm kernel (element- or plane-staged, EPB slabs per
-// block; MX: the matrix-core kernel, one element per block) for the apply
-// MODE.
-template <int NP, int MODE, bool PLANE, int EPB, bool MX = false>
+// block; MX 1 or 2: the matrix-core kernel staging the element or groups of
+// planes, one element per block) for the apply MODE.
+template <int NP, int MODE, bool PLANE, int EPB, int MX = 0>
 static int axhelm_go(const Mesh &m, const DMat<NP> &Dm, double *au, const double *u, Coef h1,
                      Coef h2, double *pout, const double *r, const double *dinv,
                      const double *mult, const CgState *st, double *rho_part, hipStream_t s,
@@ -579,13 +770,17 @@ static int axhelm_go(const Mesh &m, const DMat<NP> &Dm, double *au, const double
     grid = (int)(ngroups < cap ? ngroups : cap);
   }
   if (grid < 1) grid = 1;
-  const dim3 block(MX ? mx_threads<NP>() : NP * NP * EPB);
+  const dim3 block(MX != 0 ? mx_threads<NP>() : NP * NP * EPB);
   // h2 mode: 0 none (reference ifh2 false), 1 uniform, 2 field
   const int h2m = h2.uniform() ? (h2.s != 0.0 ? 1 : 0) : 2;
   const bool h1f = !h1.uniform();
 #define NK_LAUNCH(H1F_, H2M_)                                                                \
-  if constexpr (MX)                                                                           \
+  if constexpr (MX == 1)                                                                      \
     hipLaunchKernelGGL((k_axhelm_mx<NP, MODE, H1F_, H2M_>), dim3(grid), block, 0, s, Dm,      \
+                       m.Dd, au, u, m.g, m.n, h1.f, h1.s, h2.f, h2.s, m.bm1, m.nel, pout, r,   \
+                       dinv, mult, st, rho_part, betap);                                      \
+  else if constexpr (MX == 2)                                                                 \
+    hipLaunchKernelGGL((k_axhelm_mxp<NP, MODE, H1F_, H2M_>), dim3(grid), block, 0, s, Dm,     \
                        m.Dd, au, u, m.g, m.n, h1.f, h1.s, h2.f, h2.s, m.bm1, m.nel, pout, r,   \
                        dinv, mult, st, rho_part, betap);                                      \
   else if constexpr (PLANE)                                                                   \
@@ -611,14 +806,15 @@ static int axhelm_go(const Mesh &m, const DMat<NP> &Dm, double *au, const double
 }
 
 // Variants: 0 element-staged, 1 plane-staged (epb slabs per block), 2
-// plane-staged and 3 element-staged with the wide packing (epb_wide), 4 the
-// r/s contractions on the matrix cores (NP = 8..16); an order without a
-// distinct wide packing runs 1 for 2 and 0 for 3, as does 3 where the wide
-// element-staged kernel would not fit two blocks per CU, and an order
-// outside 8..16 runs 0 for 4.  The plain apply is element-local, so
-// variants 0-3 give the same au bit for bit; the fused applies' partial
-// sums of p.w group the elements by block, so the wide packings round those
-// sums differently; variant 4 agrees with the others to rounding.
+// plane-staged and 3 element-staged with the wide packing (epb_wide), 4 and
+// 5 the r/s contractions on the matrix cores (NP = 8..16) with the element
+// or groups of planes staged; an order without a distinct wide packing runs
+// 1 for 2 and 0 for 3, as does 3 where the wide element-staged kernel would
+// not fit two blocks per CU, and an order outside 8..16 runs 0 for 4 and 5.
+// The plain apply is element-local, so variants 0-3 give the same au bit
+// for bit, as do 4 and 5; the fused applies' partial sums of p.w group the
+// elements by block, so the wide packings round those sums differently;
+// the matrix cores agree with the vector kernels to rounding.
 template <int NP, int MODE>
 static int axhelm_dispatch(const Mesh &m, double *au, const double *u, Coef h1, Coef h2,
                            double *pout, const double *r, const double *dinv,
@@ -640,11 +836,16 @@ static int axhelm_dispatch(const Mesh &m, double *au, const double *u, Coef h1, 
     if constexpr (has_wide_elem<NP>()) NK_GO(false, epb_wide<NP>());
     else NK_GO(false, epb<NP>());
   }
-  if (var == 4) {
-    if constexpr (has_mfma<NP>())
-      return axhelm_go<NP, MODE, false, 1, true>(m, Dm, au, u, h1, h2, pout, r, dinv, mult, st,
-                                                 rho_part, s, betap, gc);
-    else NK_GO(false, epb<NP>());
+  if (var == 4 || var == 5) {
+    if constexpr (has_mfma<NP>()) {
+      if (var == 4)
+        return axhelm_go<NP, MODE, false, 1, 1>(m, Dm, au, u, h1, h2, pout, r, dinv, mult, st,
+                                                rho_part, s, betap, gc);
+      return axhelm_go<NP, MODE, false, 1, 2>(m, Dm, au, u, h1, h2, pout, r, dinv, mult, st,
+                                              rho_part, s, betap, gc);
+    } else {
+      NK_GO(false, epb<NP>());
+    }
   }
   if (var == 1) NK_GO(true, epb<NP>());
   NK_GO(false, epb<NP>());
@@ -652,7 +853,7 @@ static int axhelm_dispatch(const Mesh &m, double *au, const double *u, Coef h1, 
 }
 
 // which variants an order has beyond 0 and 1 (bit 2: variant 2, bit 3: 3,
-// bit 4: 4)
+// bit 4: 4 and 5)
 template <int NP>
 constexpr int axhelm_wide_mask() {
   return (has_wide<NP>() ? 4 : 0) | (has_wide_elem<NP>() ? 8 : 0) | (has_mfma<NP>() ? 16 : 0);

@@ -69,9 +69,10 @@ def _ptr(t):
 def axhelm_tuning(lx1):
     """(variant, grid, tuned) of the axhelm kernels for order lx1: variant 0
     stages the element in LDS, 1 one t-plane at a time, 2 and 3 the same with
-    more thread slabs per block (identical plain applies), 4 the r/s
-    contractions on the FP64 matrix cores (equal to rounding); grid = blocks
-    of the plain apply (0: one per element group)."""
+    more thread slabs per block (identical plain applies), 4 and 5 the r/s
+    contractions on the FP64 matrix cores with the element or groups of
+    t-planes staged (equal to rounding); grid = blocks of the plain apply
+    (0: one per element group)."""
     lib = init()
     v, g, t = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
     check(lib.nkh_axhelm_tuning(int(lx1), ctypes.byref(v), ctypes.byref(g), ctypes.byref(t)),
