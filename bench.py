@@ -148,7 +148,9 @@ def order_sweep(orders=range(3, 16)):
     HIP events of the library's timers
     over 20 Jacobi-PCG iterations each), and 100 fixed Jacobi-PCG iterations
     of hmholtz with the FP64 operator (its axhelm kernel time and HBM
-    fraction at 12 words per node) and with the FP32 one (mixed precision).
+    fraction at 12 words per node) and with the FP32 one (mixed precision),
+    the latter also with each kernel of the FP32 operator (vector FMA,
+    matrix cores).
     Prints one JSON line with the table."""
     import ctypes
 
@@ -208,6 +210,15 @@ def order_sweep(orders=range(3, 16)):
         p32 = solver.default_params(precision=2)
         timed_pcg(F, u, rhs, one, 100, p32)
         pcg32_s, _, out32 = timed_pcg(F, u, rhs, one, 100, p32)
+        # and with each kernel of the FP32 operator (vector FMA / matrix cores)
+        v32 = solver.axhelm32_variant(N + 1)
+        per32 = {}
+        for v in ((0, 1) if N + 1 >= 8 else (0,)):
+            solver.set_axhelm32_variant(N + 1, v)
+            timed_pcg(F, u, rhs, one, 20, p32)
+            t32, _, o32 = timed_pcg(F, u, rhs, one, 100, p32)
+            per32[f"pcg32_variant{v}_GDOF_iter_s"] = free * o32["niter"] / t32 / 1e9
+        solver.set_axhelm32_variant(N + 1, v32)
         row = dict(N=N, E=box.nel, local_nodes=box.n, free_dof=free, setup_s=round(setup, 1),
                    axhelm_variant=chosen[0], axhelm_grid=chosen[1],
                    axhelm_cg_variant=chosen_fused[0], **per_variant,
@@ -216,7 +227,8 @@ def order_sweep(orders=range(3, 16)):
                    pcg_iterations=out["niter"], pcg_ms=pcg_s * 1e3,
                    pcg_GDOF_iter_s=free * out["niter"] / pcg_s / 1e9,
                    pcg32_ms=pcg32_s * 1e3,
-                   pcg32_GDOF_iter_s=free * out32["niter"] / pcg32_s / 1e9)
+                   pcg32_GDOF_iter_s=free * out32["niter"] / pcg32_s / 1e9,
+                   axhelm32_variant=v32, **per32)
         log(f"[sweep] {row}")
         rows.append(row)
         F.destroy()

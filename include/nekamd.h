@@ -195,6 +195,14 @@ int nkh_axhelm_set_tuning(int lx1, int variant, int grid);
 /* the kernels of the CG-fused (cggo) and flexible-CG-fused applies, which
    the autotuner times separately on scratch vectors */
 int nkh_axhelm_tuning_fused(int lx1, int *variant_cg, int *variant_flex);
+/* Kernel of the single-precision operator of the mixed solvers
+   (precision = 2) for order lx1 (no counterpart in the reference): 0 vector
+   FMA, 1 the r- and s-direction contractions as FP32 matrix-core products
+   (v_mfma_f32_16x16x4_f32, lx1 = 8-16, NKH_ERR_UNSUPPORTED elsewhere); the
+   two agree to FP32 rounding.  The defaults are the faster of the two per
+   order as measured on MI355X.  Process-wide. */
+int nkh_axhelm32_set_variant(int lx1, int variant);
+int nkh_axhelm32_variant(int lx1, int *variant);
 /* Kernel of the local FDM solves (no counterpart in the reference, whose
    hsmg_do_fast has one form), for Schwarz blocks of up to 16 points: 0 vector
    FMA (default); for the single-precision smoother (precision 1 and 2) 1 the

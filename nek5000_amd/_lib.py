@@ -86,6 +86,8 @@ SIGNATURES = {
     "nkh_axhelm_tuning": (c_int, [c_int, P(c_int), P(c_int), P(c_int)]),
     "nkh_axhelm_set_tuning": (c_int, [c_int, c_int, c_int]),
     "nkh_axhelm_tuning_fused": (c_int, [c_int, P(c_int), P(c_int)]),
+    "nkh_axhelm32_set_variant": (c_int, [c_int, c_int]),
+    "nkh_axhelm32_variant": (c_int, [c_int, P(c_int)]),
     "nkh_fdm_set_variant": (c_int, [c_int]),
     "nkh_fdm_variant": (c_int, [P(c_int)]),
     "nkd_dssum": (c_int, [c_int, c_void_p]),

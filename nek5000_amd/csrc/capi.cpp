@@ -770,6 +770,13 @@ int nkh_axhelm_tuning_fused(int lx1, int *variant_cg, int *variant_flex) {
   return axhelm_tuning_fused(lx1, variant_cg, variant_flex);
 }
 
+int nkh_axhelm32_set_variant(int lx1, int variant) { return axhelm32_set_variant(lx1, variant); }
+
+int nkh_axhelm32_variant(int lx1, int *variant) {
+  NK_CHECK(variant, NKH_ERR_ARG, "nkh_axhelm32_variant: null pointer");
+  return axhelm32_variant(lx1, variant);
+}
+
 int nkh_fdm_set_variant(int variant) { return fdm_set_variant(variant); }
 
 int nkh_fdm_variant(int *variant) {

@@ -303,6 +303,180 @@ __global__ void __launch_bounds__(NP * NP * epb<NP>())
   if (threadIdx.x == 0) den_part[blockIdx.x] = t;
 }
 
+// The same single-precision operator with the r- and s-direction
+// contractions on the matrix cores (v_mfma_f32_16x16x4_f32, NP = 8..16), G
+// t-planes staged at a time (G = the block's wavefronts), as k_axhelm_mxp
+// does in FP64: per plane the gradients ur, us and the transposed sums
+// ar + as are 16 x 16 products (zero-padded beyond NP) whose [j][i] tiles
+// go one plane per wavefront; the owner thread of each t-column does the
+// t-direction terms, the factors and the epilogue as in k_axhelm32.  Lane l
+// of a product holds A[l%16][4c + l/16] and B[4c + l/16][l%16] of chunk c and
+// gets D[4(l/16)+v][l%16] in register v (the FP32 layout, checked on
+// gfx950).  Results agree with k_axhelm32 to FP32 rounding.
+typedef float nk_f4 __attribute__((ext_vector_type(4)));
+
+template <int NP, bool H2, bool JAC>
+__global__ void __launch_bounds__(mx_threads<NP>())
+    k_axhelm32_mxp(DMatF<NP> Dk, const double *__restrict__ Dg, float *__restrict__ w,
+                   float *__restrict__ p, const float *__restrict__ z,
+                   const float *__restrict__ dd, const float *__restrict__ g, int64_t gstride,
+                   float h1, float h2, const float *__restrict__ bm1, int64_t nel,
+                   const double *__restrict__ betap, const int *__restrict__ donep,
+                   double *__restrict__ den_part) {
+  static_assert(has_mfma<NP>(), "one 16 x 16 tile per plane");
+  constexpr int NP2 = NP * NP, NP3 = NP2 * NP, T = mx_threads<NP>(), NW = T / 64, G = NW;
+  constexpr int PJ = NP | 1, PP = PJ * NP, KC = (NP + 3) / 4, NG = (NP + G - 1) / G;
+  __shared__ float s_u[G * PP];   // the group's planes of p, then ar + as
+  __shared__ float s_wr[G * PP];  // ur, then wr
+  __shared__ float s_ws[G * PP];  // us, then ws
+  __shared__ double s_red[T];
+  if constexpr (JAC) {
+    if (*donep) return;
+  }
+  const float beta = (float)*betap;
+  const int tid = threadIdx.x;
+  const bool own = tid < NP2;
+  const int i = own ? tid % NP : 0, j = own ? tid / NP : 0;
+  const int lane = tid & 63, wv = tid >> 6, lr = lane & 15, lg = lane >> 4;
+  const bool lin = lr < NP;
+  float Dn[KC], Dt[KC];  // D(lr, 4c + lg) and D(4c + lg, lr), 0 outside NP x NP
+#pragma unroll
+  for (int c = 0; c < KC; ++c) {
+    const int m = 4 * c + lg;
+    const bool in = lin && m < NP;
+    Dn[c] = in ? (float)Dg[lr + NP * m] : 0.0f;
+    Dt[c] = in ? (float)Dg[m + NP * lr] : 0.0f;
+  }
+  const int own_at = i + PJ * j;
+  float *const wu = s_u + wv * PP, *const wwr = s_wr + wv * PP, *const wws = s_ws + wv * PP;
+  double den = 0.0;
+  for (int64_t e = blockIdx.x; e < nel; e += gridDim.x) {
+    const int64_t base = e * NP3 + tid;
+    float r_u[NP], r_au[NP], r_o[NP];
+#pragma unroll
+    for (int k = 0; k < NP; ++k) {
+      float v = 0.0f;
+      if (own) {
+        const int64_t l = base + k * NP2;
+        if constexpr (JAC) v = ldnt32(dd + l) * ldnt32(z + l) + beta * ldnt32(p + l);
+        else v = beta * ldnt32(p + l) + ldnt32(z + l);
+        __builtin_nontemporal_store(v, p + l);
+      }
+      r_u[k] = v;
+      r_au[k] = 0.0f;
+      r_o[k] = 0.0f;
+    }
+#pragma unroll
+    for (int gq = 0; gq < NG; ++gq) {
+      __builtin_amdgcn_sched_barrier(0);
+      __syncthreads();  // the previous group's (or element's) LDS reads are done
+      if (own) {
+#pragma unroll
+        for (int q = 0; q < G; ++q)
+          if (gq * G + q < NP) s_u[q * PP + own_at] = r_u[gq * G + q];
+      }
+      __syncthreads();
+      const bool act = gq * G + wv < NP;  // wave-uniform
+      if (act) {
+        nk_f4 acc = {0.0f, 0.0f, 0.0f, 0.0f}, acs = {0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+        for (int c = 0; c < KC; ++c) {
+          const int m = 4 * c + lg;
+          const float a = (lin && m < NP) ? wu[PJ * lr + m] : 0.0f;
+          acc = __builtin_amdgcn_mfma_f32_16x16x4f32(a, Dn[c], acc, 0, 0, 0);
+        }
+#pragma unroll
+        for (int c = 0; c < KC; ++c) {
+          const int m = 4 * c + lg;
+          const float b = (lin && m < NP) ? wu[PJ * m + lr] : 0.0f;
+          acs = __builtin_amdgcn_mfma_f32_16x16x4f32(Dn[c], b, acs, 0, 0, 0);
+        }
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+          const int jj = 4 * lg + v;
+          if (lin && jj < NP) {
+            wwr[PJ * jj + lr] = acc[v];
+            wws[PJ * jj + lr] = acs[v];
+          }
+        }
+      }
+      __syncthreads();
+      if (own) {
+#pragma unroll
+        for (int q = 0; q < G; ++q) {
+          const int k = gq * G + q;
+          if (k < NP) {
+            const int at = q * PP + own_at;
+            const float ur = s_wr[at], us = s_ws[at];
+            float ut = 0.0f;
+#pragma unroll
+            for (int m = 0; m < NP; ++m) ut += Dk.d[k + NP * m] * r_u[m];
+            const int64_t l = base + k * NP2;
+            const float G1 = ldnt32(g + l), G2 = ldnt32(g + gstride + l),
+                        G3 = ldnt32(g + 2 * gstride + l);
+            const float G4 = ldnt32(g + 3 * gstride + l), G5 = ldnt32(g + 4 * gstride + l),
+                        G6 = ldnt32(g + 5 * gstride + l);
+            const float wr = h1 * (ur * G1 + us * G4 + ut * G5);
+            const float ws = h1 * (ur * G4 + us * G2 + ut * G6);
+            const float wt = h1 * (ur * G5 + us * G6 + ut * G3);
+            s_wr[at] = wr;
+            s_ws[at] = ws;
+#pragma unroll
+            for (int kk = 0; kk < NP; ++kk) r_au[kk] += Dk.d[k + NP * kk] * wt;
+          }
+        }
+      }
+      __syncthreads();
+      if (act) {
+        nk_f4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+        for (int c = 0; c < KC; ++c) {
+          const int m = 4 * c + lg;
+          const float a = (lin && m < NP) ? wwr[PJ * lr + m] : 0.0f;
+          acc = __builtin_amdgcn_mfma_f32_16x16x4f32(a, Dt[c], acc, 0, 0, 0);
+        }
+#pragma unroll
+        for (int c = 0; c < KC; ++c) {
+          const int m = 4 * c + lg;
+          const float b = (lin && m < NP) ? wws[PJ * m + lr] : 0.0f;
+          acc = __builtin_amdgcn_mfma_f32_16x16x4f32(Dt[c], b, acc, 0, 0, 0);
+        }
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+          const int jj = 4 * lg + v;
+          if (lin && jj < NP) wu[PJ * jj + lr] = acc[v];
+        }
+      }
+      __syncthreads();
+      if (own) {
+#pragma unroll
+        for (int q = 0; q < G; ++q)
+          if (gq * G + q < NP) r_o[gq * G + q] = s_u[q * PP + own_at];
+      }
+    }
+    if (own) {
+#pragma unroll
+      for (int k = 0; k < NP; ++k) {
+        const int64_t l = base + k * NP2;
+        float a = r_o[k] + r_au[k];
+        if constexpr (H2) a = a + (h2 * ldnt32(bm1 + l)) * r_u[k];
+        __builtin_nontemporal_store(a, w + l);
+        den += (double)a * (double)r_u[k];
+      }
+    }
+  }
+  const double t = block_sum(den, s_red);
+  if (threadIdx.x == 0) den_part[blockIdx.x] = t;
+}
+
+// Kernel of the FP32 operator per order: 0 vector FMA (k_axhelm32), 1 the
+// matrix cores (k_axhelm32_mxp, lx1 = 8..16); nkh_axhelm32_set_variant fixes
+// it.  Defaults from 100 mixed Jacobi-PCG iterations per kernel on boxes of
+// about 5e7 unique DOF (MI355X, GDOF*iter/s vector -> matrix cores): lx1 = 11
+// 30.3 -> 32.0, 14 26.2 -> 27.3, 15 25.4 -> 28.1; the other orders are
+// within 2 % or faster with vector FMA (lx1 = 12: 32.3 -> 29.0).
+int g_ax32_variant[kMaxNP + 1] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 1, 0, 0, 1, 1, 0};
+
 template <int NP, bool JAC>
 static int axhelm32_dispatch(const Mesh &m, float *w, float *p, const float *z, const float *dd,
                              float h1, float h2, const double *betap, const int *donep,
@@ -310,6 +484,20 @@ static int axhelm32_dispatch(const Mesh &m, float *w, float *p, const float *z, 
   DMatF<NP> Dm;
   for (int t = 0; t < NP * NP; ++t) Dm.d[t] = (float)m.D[t];
   const int grid = axhelm_cg_grid(m);
+  if constexpr (has_mfma<NP>()) {
+    if (g_ax32_variant[NP] == 1) {
+      const dim3 bm(mx_threads<NP>());
+      if (h2 != 0.0f)
+        hipLaunchKernelGGL((k_axhelm32_mxp<NP, true, JAC>), dim3(grid), bm, 0, s, Dm, m.Dd, w, p,
+                           z, dd, m.g32, m.n, h1, h2, m.bm1_32, m.nel, betap, donep, den_part);
+      else
+        hipLaunchKernelGGL((k_axhelm32_mxp<NP, false, JAC>), dim3(grid), bm, 0, s, Dm, m.Dd, w,
+                           p, z, dd, m.g32, m.n, h1, h2, m.bm1_32, m.nel, betap, donep,
+                           den_part);
+      NK_HIP(hipGetLastError());
+      return NKH_OK;
+    }
+  }
   const dim3 block(NP * NP * epb<NP>());
   if (h2 != 0.0f)
     hipLaunchKernelGGL((k_axhelm32<NP, true, JAC>), dim3(grid), block, 0, s, Dm, m.Dd, w, p, z,
@@ -334,6 +522,21 @@ __global__ void k_to_f32(float *__restrict__ d, const double *__restrict__ s, in
   for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
        i += (int64_t)gridDim.x * blockDim.x)
     d[i] = (float)s[i];
+}
+
+int axhelm32_set_variant(int np, int v) {
+  NK_CHECK(np >= 2 && np <= kMaxNP && (v == 0 || v == 1), NKH_ERR_ARG,
+           "FP32 axhelm variant: lx1=%d variant=%d (0 or 1)", np, v);
+  NK_CHECK(v == 0 || (np >= 8 && np <= 16), NKH_ERR_UNSUPPORTED,
+           "FP32 axhelm on the matrix cores: lx1=%d outside 8..16", np);
+  g_ax32_variant[np] = v;
+  return NKH_OK;
+}
+
+int axhelm32_variant(int np, int *v) {
+  NK_CHECK(np >= 2 && np <= kMaxNP, NKH_ERR_ARG, "FP32 axhelm variant: lx1=%d", np);
+  *v = g_ax32_variant[np];
+  return NKH_OK;
 }
 
 int ensure_geom32(Mesh &m, hipStream_t s) {

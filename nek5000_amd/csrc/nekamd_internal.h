@@ -321,6 +321,8 @@ int mg_solve32(Mesh &m, float *z, const float *r, hipStream_t s, int crs_mode);
 // products, 2 bf16 matrix-core products with FP32 accumulation.
 int fdm_set_variant(int v);
 int fdm_variant();
+int axhelm32_set_variant(int np, int v);
+int axhelm32_variant(int np, int *v);
 int mg_mask_fine(Mesh &m, double *r, hipStream_t s);
 int mg_info(const Mesh &m, int *lmax, int *nh, int *ncrs, double *crs_avg_iters);
 int run_flex_cg(Mesh &m, double *res, Coef h1, Coef h2, const double *wt, const double *mask,

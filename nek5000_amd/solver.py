@@ -104,6 +104,18 @@ def fdm_variant():
     return v.value
 
 
+def axhelm32_variant(lx1):
+    """Kernel of the FP32 operator of the mixed solvers for order lx1: 0
+    vector FMA, 1 the r/s contractions on the FP32 matrix cores."""
+    v = ctypes.c_int()
+    check(init().nkh_axhelm32_variant(int(lx1), ctypes.byref(v)), "nkh_axhelm32_variant")
+    return v.value
+
+
+def set_axhelm32_variant(lx1, variant):
+    check(init().nkh_axhelm32_set_variant(int(lx1), int(variant)), "nkh_axhelm32_set_variant")
+
+
 def set_fdm_variant(variant):
     check(init().nkh_fdm_set_variant(int(variant)), "nkh_fdm_set_variant")
 
