@@ -559,10 +559,12 @@ def main():
     kname = {0: "k_axhelm", 1: "k_axhelm_pl", 2: "k_axhelm_pl", 3: "k_axhelm",
              4: "k_axhelm_mx", 5: "k_axhelm_mxp"}[v_flex if hsmg else v_cg]
     if hsmg:
-        # axhelm of ax (gmres.f:284) with the p update fused: reads p, z,
-        # g1..g6, bm1; writes p, w
-        bpn = 8.0 * 11
-        ax_kernel = f"{kname}<{args.lx1},2,false,1> (flex-CG axhelm, p = beta p + z fused)"
+        # axhelm of ax (gmres.f:284) with the p update and the previous
+        # iteration's x += alpha p fused: reads p, z, x, g1..g6, bm1; writes
+        # p, x, w
+        bpn = 8.0 * 13
+        ax_kernel = (f"{kname}<{args.lx1},2,false,1> (flex-CG axhelm, p = beta p + z and "
+                     f"x += alpha p fused)")
     else:
         bpn = fused_axhelm_bytes_per_node(args.lx1)
         ax_kernel = f"{kname}<{args.lx1},1,false,1> (CG-fused axhelm)"

@@ -86,7 +86,10 @@ __device__ __forceinline__ void stnt(double v, double *p) { __builtin_nontempora
 // rho = (Q Q^T w, p)_mult for a continuous, masked p -- is written to one
 // slot per block when rho_part is given.  MODE 2 (flexible CG): the same
 // with p = beta*p + z (add2s1, hmholtz.f:2243), z passed as r, beta read
-// from *betap.  MODE 3: MODE 0 gated by a state (returns while st->done),
+// from *betap; with xacc, also the previous iteration's solution update
+// x += alpha p (add2s2, hmholtz.f:2262) on the p it reads, alpha from
+// *alphap -- the same arithmetic one iteration later, so that the update
+// kernel need not read p and x again.  MODE 3: MODE 0 gated by a state (returns while st->done),
 // for applies that a captured iteration runs only on some passes.
 // ---------------------------------------------------------------------------
 template <int NP, int MODE, bool H1F, int H2M, int EPB_ = epb<NP>()>
@@ -98,7 +101,8 @@ __global__ void __launch_bounds__(NP * NP * EPB_)
              double *__restrict__ pout, const double *__restrict__ r,
              const double *__restrict__ dinv, const double *__restrict__ mult,
              const CgState *__restrict__ st, double *__restrict__ rho_part,
-             const double *__restrict__ betap) {
+             const double *__restrict__ betap, double *__restrict__ xacc,
+             const double *__restrict__ alphap) {
   constexpr int NP2 = NP * NP, NP3 = NP2 * NP, EPB = EPB_;
   __shared__ double s_D[NP][NP + 1];
   __shared__ double s_u[EPB][NP3];
@@ -111,7 +115,11 @@ __global__ void __launch_bounds__(NP * NP * EPB_)
     if (st->done) return;
     beta = st->beta;
   }
-  if constexpr (MODE == 2) beta = *betap;
+  double alpha = 0.0;  // MODE 2 with xacc: x += alpha p (the previous p)
+  if constexpr (MODE == 2) {
+    beta = *betap;
+    if (xacc) alpha = *alphap;
+  }
   if constexpr (MODE == 3) {
     if (st->done) return;
   }
@@ -138,7 +146,9 @@ __global__ void __launch_bounds__(NP * NP * EPB_)
           v = beta * ldnt(uin + l) + ldnt(dinv + l) * ldnt(r + l);
           stnt(v, pout + l);
         } else if constexpr (MODE == 2) {
-          v = beta * ldnt(uin + l) + ldnt(r + l);
+          const double po = ldnt(uin + l);
+          v = beta * po + ldnt(r + l);
+          if (xacc) stnt(ldnt(xacc + l) + alpha * po, xacc + l);
           stnt(v, pout + l);
         } else {
           v = ldnt(uin + l);
@@ -225,7 +235,8 @@ __global__ void __launch_bounds__(NP * NP * EPB_)
                 double *__restrict__ pout, const double *__restrict__ r,
                 const double *__restrict__ dinv, const double *__restrict__ mult,
                 const CgState *__restrict__ st, double *__restrict__ rho_part,
-                const double *__restrict__ betap) {
+                const double *__restrict__ betap, double *__restrict__ xacc,
+             const double *__restrict__ alphap) {
   constexpr int NP2 = NP * NP, EPB = EPB_;
   __shared__ double s_D[NP][NP + 1];
   __shared__ double s_u[EPB][NP2];
@@ -237,7 +248,11 @@ __global__ void __launch_bounds__(NP * NP * EPB_)
     if (st->done) return;
     beta = st->beta;
   }
-  if constexpr (MODE == 2) beta = *betap;
+  double alpha = 0.0;  // MODE 2 with xacc: x += alpha p (the previous p)
+  if constexpr (MODE == 2) {
+    beta = *betap;
+    if (xacc) alpha = *alphap;
+  }
   if constexpr (MODE == 3) {
     if (st->done) return;
   }
@@ -261,7 +276,9 @@ __global__ void __launch_bounds__(NP * NP * EPB_)
           v = beta * ldnt(uin + l) + ldnt(dinv + l) * ldnt(r + l);
           stnt(v, pout + l);
         } else if constexpr (MODE == 2) {
-          v = beta * ldnt(uin + l) + ldnt(r + l);
+          const double po = ldnt(uin + l);
+          v = beta * po + ldnt(r + l);
+          if (xacc) stnt(ldnt(xacc + l) + alpha * po, xacc + l);
           stnt(v, pout + l);
         } else {
           v = ldnt(uin + l);
@@ -382,7 +399,8 @@ __global__ void __launch_bounds__(mx_threads<NP>())
                 double *__restrict__ pout, const double *__restrict__ r,
                 const double *__restrict__ dinv, const double *__restrict__ mult,
                 const CgState *__restrict__ st, double *__restrict__ rho_part,
-                const double *__restrict__ betap) {
+                const double *__restrict__ betap, double *__restrict__ xacc,
+             const double *__restrict__ alphap) {
   static_assert(has_mfma<NP>(), "one 16 x 16 tile per plane");
   constexpr int NP2 = NP * NP, NP3 = NP2 * NP, NW = mx_threads<NP>() / 64;
   constexpr int PJ = NP | 1, PK = PJ * NP, KC = (NP + 3) / 4;
@@ -394,7 +412,11 @@ __global__ void __launch_bounds__(mx_threads<NP>())
     if (st->done) return;
     beta = st->beta;
   }
-  if constexpr (MODE == 2) beta = *betap;
+  double alpha = 0.0;  // MODE 2 with xacc: x += alpha p (the previous p)
+  if constexpr (MODE == 2) {
+    beta = *betap;
+    if (xacc) alpha = *alphap;
+  }
   if constexpr (MODE == 3) {
     if (st->done) return;
   }
@@ -427,7 +449,9 @@ __global__ void __launch_bounds__(mx_threads<NP>())
           v = beta * ldnt(uin + l) + ldnt(dinv + l) * ldnt(r + l);
           stnt(v, pout + l);
         } else if constexpr (MODE == 2) {
-          v = beta * ldnt(uin + l) + ldnt(r + l);
+          const double po = ldnt(uin + l);
+          v = beta * po + ldnt(r + l);
+          if (xacc) stnt(ldnt(xacc + l) + alpha * po, xacc + l);
           stnt(v, pout + l);
         } else {
           v = ldnt(uin + l);
@@ -560,7 +584,8 @@ __global__ void __launch_bounds__(mx_threads<NP>())
                  double *__restrict__ pout, const double *__restrict__ r,
                  const double *__restrict__ dinv, const double *__restrict__ mult,
                  const CgState *__restrict__ st, double *__restrict__ rho_part,
-                 const double *__restrict__ betap) {
+                 const double *__restrict__ betap, double *__restrict__ xacc,
+             const double *__restrict__ alphap) {
   static_assert(has_mfma<NP>(), "one 16 x 16 tile per plane");
   constexpr int NP2 = NP * NP, NP3 = NP2 * NP, NW = mx_threads<NP>() / 64, G = NW;
   constexpr int PJ = NP | 1, PP = PJ * NP, KC = (NP + 3) / 4, NG = (NP + G - 1) / G;
@@ -572,7 +597,11 @@ __global__ void __launch_bounds__(mx_threads<NP>())
     if (st->done) return;
     beta = st->beta;
   }
-  if constexpr (MODE == 2) beta = *betap;
+  double alpha = 0.0;  // MODE 2 with xacc: x += alpha p (the previous p)
+  if constexpr (MODE == 2) {
+    beta = *betap;
+    if (xacc) alpha = *alphap;
+  }
   if constexpr (MODE == 3) {
     if (st->done) return;
   }
@@ -604,7 +633,9 @@ __global__ void __launch_bounds__(mx_threads<NP>())
           v = beta * ldnt(uin + l) + ldnt(dinv + l) * ldnt(r + l);
           stnt(v, pout + l);
         } else if constexpr (MODE == 2) {
-          v = beta * ldnt(uin + l) + ldnt(r + l);
+          const double po = ldnt(uin + l);
+          v = beta * po + ldnt(r + l);
+          if (xacc) stnt(ldnt(xacc + l) + alpha * po, xacc + l);
           stnt(v, pout + l);
         } else {
           v = ldnt(uin + l);
@@ -753,7 +784,8 @@ template <int NP, int MODE, bool PLANE, int EPB, int MX = 0>
 static int axhelm_go(const Mesh &m, const DMat<NP> &Dm, double *au, const double *u, Coef h1,
                      Coef h2, double *pout, const double *r, const double *dinv,
                      const double *mult, const CgState *st, double *rho_part, hipStream_t s,
-                     const double *betap, int gridcap) {
+                     const double *betap, int gridcap, double *xacc = nullptr,
+                     const double *alphap = nullptr) {
   const int64_t ngroups = (m.nel + EPB - 1) / EPB;
   int grid;
   if (MODE == 3) {
@@ -778,19 +810,19 @@ static int axhelm_go(const Mesh &m, const DMat<NP> &Dm, double *au, const double
   if constexpr (MX == 1)                                                                      \
     hipLaunchKernelGGL((k_axhelm_mx<NP, MODE, H1F_, H2M_>), dim3(grid), block, 0, s, Dm,      \
                        m.Dd, au, u, m.g, m.n, h1.f, h1.s, h2.f, h2.s, m.bm1, m.nel, pout, r,   \
-                       dinv, mult, st, rho_part, betap);                                      \
+                       dinv, mult, st, rho_part, betap, xacc, alphap);                        \
   else if constexpr (MX == 2)                                                                 \
     hipLaunchKernelGGL((k_axhelm_mxp<NP, MODE, H1F_, H2M_>), dim3(grid), block, 0, s, Dm,     \
                        m.Dd, au, u, m.g, m.n, h1.f, h1.s, h2.f, h2.s, m.bm1, m.nel, pout, r,   \
-                       dinv, mult, st, rho_part, betap);                                      \
+                       dinv, mult, st, rho_part, betap, xacc, alphap);                        \
   else if constexpr (PLANE)                                                                   \
     hipLaunchKernelGGL((k_axhelm_pl<NP, MODE, H1F_, H2M_, EPB>), dim3(grid), block, 0, s, Dm, \
                        m.Dd, au, u, m.g, m.n, h1.f, h1.s, h2.f, h2.s, m.bm1, m.nel, pout, r,   \
-                       dinv, mult, st, rho_part, betap);                                      \
+                       dinv, mult, st, rho_part, betap, xacc, alphap);                        \
   else                                                                                        \
     hipLaunchKernelGGL((k_axhelm<NP, MODE, H1F_, H2M_, EPB>), dim3(grid), block, 0, s, Dm,    \
                        m.Dd, au, u, m.g, m.n, h1.f, h1.s, h2.f, h2.s, m.bm1, m.nel, pout, r,   \
-                       dinv, mult, st, rho_part, betap)
+                       dinv, mult, st, rho_part, betap, xacc, alphap)
   if (h1f) {
     if (h2m == 0) NK_LAUNCH(true, 0);
     else if (h2m == 1) NK_LAUNCH(true, 1);
@@ -819,7 +851,8 @@ template <int NP, int MODE>
 static int axhelm_dispatch(const Mesh &m, double *au, const double *u, Coef h1, Coef h2,
                            double *pout, const double *r, const double *dinv,
                            const double *mult, const CgState *st, double *rho_part,
-                           hipStream_t s, const double *betap = nullptr) {
+                           hipStream_t s, const double *betap = nullptr,
+                           double *xacc = nullptr, const double *alphap = nullptr) {
   DMat<NP> Dm;
   for (int t = 0; t < NP * NP; ++t) Dm.d[t] = m.D[t];
   const AxhelmChoice ch = g_axhelm_choice[NP];
@@ -827,7 +860,7 @@ static int axhelm_dispatch(const Mesh &m, double *au, const double *u, Coef h1, 
   const int gc = ch.grid;
 #define NK_GO(PLANE_, EPB_)                                                                  \
   return axhelm_go<NP, MODE, PLANE_, EPB_>(m, Dm, au, u, h1, h2, pout, r, dinv, mult, st,   \
-                                          rho_part, s, betap, gc)
+                                          rho_part, s, betap, gc, xacc, alphap)
   if (var == 2) {
     if constexpr (has_wide<NP>()) NK_GO(true, epb_wide<NP>());
     else NK_GO(true, epb<NP>());
@@ -840,9 +873,9 @@ static int axhelm_dispatch(const Mesh &m, double *au, const double *u, Coef h1, 
     if constexpr (has_mfma<NP>()) {
       if (var == 4)
         return axhelm_go<NP, MODE, false, 1, 1>(m, Dm, au, u, h1, h2, pout, r, dinv, mult, st,
-                                                rho_part, s, betap, gc);
+                                                rho_part, s, betap, gc, xacc, alphap);
       return axhelm_go<NP, MODE, false, 1, 2>(m, Dm, au, u, h1, h2, pout, r, dinv, mult, st,
-                                              rho_part, s, betap, gc);
+                                              rho_part, s, betap, gc, xacc, alphap);
     } else {
       NK_GO(false, epb<NP>());
     }

@@ -1383,6 +1383,7 @@ __global__ void __launch_bounds__(256)
     st->div0 = div0;
     st->tolpss = (param21 < 0) ? fabs(param21) * div0 : tolps;
     st->rho1 = 1.0;
+    st->alpha = 0.0;  // the first deferred x update adds 0 * p (p = 0)
     st->iter = 0;
     st->done = 0;
     st->rnorm = div0;
@@ -1504,6 +1505,34 @@ __global__ void __launch_bounds__(256)
   block_part(a, part);
 }
 
+// r -= alpha w; partial sum r*r*wt (the fused path: x += alpha p is done by
+// the next iteration's axhelm, or by k_fcg_xfin after the last one)
+template <typename WT>
+__global__ void __launch_bounds__(256)
+    k_fcg_update_r(double *__restrict__ r, const double *__restrict__ w,
+                   const WT *__restrict__ wt, const FcgState *st, int64_t n, double *part) {
+  const double alpha = st->alpha;
+  double a = 0.0;
+  NK_GRID_LOOP(i, n) {
+    const double ri =
+        __builtin_nontemporal_load(r + i) - alpha * __builtin_nontemporal_load(w + i);
+    __builtin_nontemporal_store(ri, r + i);
+    a = a + ri * ri * wt_at(wt, i);
+  }
+  block_part(a, part);
+}
+
+// the last iteration's x += alpha p
+__global__ void k_fcg_xfin(double *__restrict__ x, const double *__restrict__ p,
+                           const FcgState *st, int64_t n) {
+  const double alpha = st->alpha;
+  NK_GRID_LOOP(i, n) {
+    __builtin_nontemporal_store(__builtin_nontemporal_load(x + i) +
+                                    alpha * __builtin_nontemporal_load(p + i),
+                                x + i);
+  }
+}
+
 __global__ void __launch_bounds__(256) k_fcg_rnorm(FcgState *st, const double *part, double vol) {
   const double t = final_sum(part);
   if (threadIdx.x == 0) {
@@ -1585,7 +1614,8 @@ int run_flex_cg(Mesh &m, double *res, Coef h1, Coef h2, const double *wt, const 
     hipLaunchKernelGGL(k_fcg_beta, dim3(1), b, 0, s, mg.fst, pa, pb);
     if (fused) {
       if (tm.on) (void)hipEventRecord(ev[0], s);
-      if ((rc = launch_axhelm_flex(m, w, p, z, h1, h2, &mg.fst->beta, rho_part, s)) != NKH_OK)
+      if ((rc = launch_axhelm_flex(m, w, p, z, h1, h2, &mg.fst->beta, rho_part, s, res,
+                                   &mg.fst->alpha)) != NKH_OK)
         break;
       if (tm.on) (void)hipEventRecord(ev[1], s);
       if ((rc = gs_assemble(m.gs, w, nullptr, s)) != NKH_OK) break;
@@ -1601,7 +1631,11 @@ int run_flex_cg(Mesh &m, double *res, Coef h1, Coef h2, const double *wt, const 
       hipLaunchKernelGGL(k_fcg_den, g, b, 0, s, w, mask, wt, p, n, pa);
       hipLaunchKernelGGL(k_fcg_alpha, dim3(1), b, 0, s, mg.fst, pa);
     }
-    if (wtc)
+    if (fused && wtc)
+      hipLaunchKernelGGL(k_fcg_update_r<uint8_t>, g, b, 0, s, r, w, wtc, mg.fst, n, pa);
+    else if (fused)
+      hipLaunchKernelGGL(k_fcg_update_r<double>, g, b, 0, s, r, w, wt, mg.fst, n, pa);
+    else if (wtc)
       hipLaunchKernelGGL(k_fcg_update<uint8_t>, g, b, 0, s, res, r, p, w, wtc, mg.fst, n, pa);
     else
       hipLaunchKernelGGL(k_fcg_update<double>, g, b, 0, s, res, r, p, w, wt, mg.fst, n, pa);
@@ -1642,6 +1676,12 @@ int run_flex_cg(Mesh &m, double *res, Coef h1, Coef h2, const double *wt, const 
     for (auto &e : ev)
       if (e) (void)hipEventDestroy(e);
   NK_TRY(rc);
+  // the fused path defers each x += alpha p to the next iteration's axhelm:
+  // the last one is applied here
+  if (fused && iter > 0) {
+    hipLaunchKernelGGL(k_fcg_xfin, g, b, 0, s, res, p, mg.fst, n);
+    NK_HIP(hipGetLastError());
+  }
   // hmholtz.f:2272: orthogonalise the solution against the null space
   if (mg_null_space_on(mg, h2)) NK_TRY(mg_ortho(mg, res, n, s));
   if (iter == 0) {

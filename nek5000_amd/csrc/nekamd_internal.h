@@ -243,9 +243,11 @@ int launch_axhelm_cg(const Mesh &m, double *w, double *p, const double *r, const
                      double *rho_part, hipStream_t s);
 
 // axhelm fused into the flexible-CG iteration: p = beta*p + z (beta read
-// from *betap), w = A p, rho partials of p . w over all local nodes.
+// from *betap), w = A p, rho partials of p . w over all local nodes; with x,
+// also x += alpha p on the p read (alpha read from *alphap).
 int launch_axhelm_flex(const Mesh &m, double *w, double *p, const double *z, Coef h1, Coef h2,
-                       const double *betap, double *rho_part, hipStream_t s);
+                       const double *betap, double *rho_part, hipStream_t s,
+                       double *x = nullptr, const double *alphap = nullptr);
 
 // ---- adjoint.hip ----
 // g1 = -(D lam)^T G (D u), g2 = -lam*bm1*u per local node (g2 may be null)
