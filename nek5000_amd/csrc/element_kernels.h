@@ -87,7 +87,7 @@ __device__ __forceinline__ void stnt(double v, double *p) { __builtin_nontempora
 // slot per block when rho_part is given.  MODE 2 (flexible CG): the same
 // with p = beta*p + z (add2s1, hmholtz.f:2243), z passed as r, beta read
 // from *betap; with xacc, also the previous iteration's solution update
-// x += alpha p (add2s2, hmholtz.f:2262) on the p it reads, alpha from
+// res += alpha p (hmholtz.f:2250-2251) on the p it reads, alpha from
 // *alphap -- the same arithmetic one iteration later, so that the update
 // kernel need not read p and x again.  MODE 3: MODE 0 gated by a state (returns while st->done),
 // for applies that a captured iteration runs only on some passes.
