@@ -521,4 +521,9 @@ int gs_fin(const GsMap &gs, double *z, const double *e, const uint8_t *w, const 
   return gs_fin_t(gs, z, e, w, s, lut);
 }
 
+int gs_fin(const GsMap &gs, float *z, const float *e, const uint8_t *w, const double *lut,
+           hipStream_t s) {
+  return gs_fin_t(gs, z, e, w, s, lut);
+}
+
 }  // namespace nekamd

@@ -82,7 +82,9 @@ struct Mg {
   int *fflag = nullptr;
   // the multiplicity weights as counts (wt = 1/count, count <= 255), plus a flag
   uint8_t *wt8 = nullptr;
-  double *wlut = nullptr;  // the 256 weights the fine level's byte codes index
+  // the 256 weights the fine level's byte codes index, then the same 256
+  // rounded to FP32
+  double *wlut = nullptr;
   // GMRES work: V (kGmresM+1 fields), Z (kGmresM fields), x, r, w, partials
   double *gV = nullptr, *gZ = nullptr, *gx = nullptr, *gr = nullptr, *gw = nullptr;
   double *gpart = nullptr, *gsm = nullptr;
@@ -101,6 +103,9 @@ int mg_grid(int64_t n);
 int mg_prepare_fp32(Mg &mg, hipStream_t s, bool inner);
 // navier1.f:223-255 ORTHO: v <- v - sum(v)/n over the n local nodes (device)
 int mg_ortho(Mg &mg, double *v, int64_t n, hipStream_t s);
+// the multiplicity weights wt as count bytes (wt = 1/count), into mg.wt8;
+// *codes = nullptr when some weight is not 1/count for a count in 1..255
+int mg_wt_codes(Mg &mg, const double *wt, int64_t n, hipStream_t s, const uint8_t **codes);
 // the constant null space is present for this solve (no Dirichlet face, h2 = 0)
 inline bool mg_null_space_on(const Mg &mg, Coef h2) {
   return mg.null_space && h2.uniform() && h2.s == 0.0;

@@ -1257,7 +1257,11 @@ static int mg_solve_t(Mesh &mesh, FT *z, const FT *r, hipStream_t s, int crs_mod
     if constexpr (f32io) {
       NK_TRY((run_tnsr<3, float, double, FT, float>(z, T.nh, C.e, C.nh, T.mswt32, C.J, nel, s,
                                                     T.e32)));
-      NK_TRY(gs_fin(mesh.gs, z, T.e32, T.mswt32, s));
+      // byte codes into the FP32-rounded half of the table: the same weights
+      if (T.mswt8)
+        NK_TRY(gs_fin(mesh.gs, z, T.e32, T.mswt8, mg.wlut + 256, s));
+      else
+        NK_TRY(gs_fin(mesh.gs, z, T.e32, T.mswt32, s));
     } else {
       NK_TRY((run_tnsr<3, float, double, FT>(z, T.nh, C.e, C.nh, T.mswt, C.J, nel, s, T.e32)));
       NK_TRY(gs_fin(mesh.gs, z, T.e32, T.mswt, s));
@@ -1545,6 +1549,21 @@ __global__ void __launch_bounds__(256) k_fcg_rnorm(FcgState *st, const double *p
 
 }  // namespace
 
+int mg_wt_codes(Mg &mg, const double *wt, int64_t n, hipStream_t s, const uint8_t **codes) {
+  // the flag after the n code bytes
+  const int64_t wt8_flag = (n + 7) / 8 * 8;
+  if (!mg.wt8) NK_HIP(hipMalloc(&mg.wt8, (size_t)wt8_flag + sizeof(int)));
+  int *wt8_bad = reinterpret_cast<int *>(mg.wt8 + wt8_flag);
+  hipLaunchKernelGGL(k_zero_int, dim3(1), dim3(1), 0, s, wt8_bad);
+  hipLaunchKernelGGL(k_wt_code, dim3(mg_grid(n)), dim3(256), 0, s, wt, mg.wt8, n, wt8_bad);
+  NK_HIP(hipGetLastError());
+  int wbad = 1;
+  NK_HIP(hipMemcpyAsync(&wbad, wt8_bad, sizeof(int), hipMemcpyDeviceToHost, s));
+  NK_HIP(hipStreamSynchronize(s));
+  *codes = wbad ? nullptr : mg.wt8;
+  return NKH_OK;
+}
+
 int run_flex_cg(Mesh &m, double *res, Coef h1, Coef h2, const double *wt, const double *mask,
                 int maxit, const nkh_params &prm, nkh_result *out, double *rnorm_history) {
   NK_CHECK(m.mg, NKH_ERR_STATE, "hmh_flex_cg needs the multigrid hierarchy (nkh_mg_setup)");
@@ -1578,17 +1597,9 @@ int run_flex_cg(Mesh &m, double *res, Coef h1, Coef h2, const double *wt, const 
   hipLaunchKernelGGL(k_zero_int, dim3(1), dim3(1), 0, s, mg.fflag);
   hipLaunchKernelGGL(k_neq, dim3(mg_grid(n)), b, 0, s, mask, mgmask, n, mg.fflag);
   NK_HIP(hipMemcpyAsync(&differ, mg.fflag, sizeof(int), hipMemcpyDeviceToHost, s));
-  // the weights as counts, when every one is 1/count (the flag after the
-  // n code bytes)
-  const int64_t wt8_flag = (n + 7) / 8 * 8;
-  if (!mg.wt8) NK_HIP(hipMalloc(&mg.wt8, (size_t)wt8_flag + sizeof(int)));
-  int *wt8_bad = reinterpret_cast<int *>(mg.wt8 + wt8_flag);
-  hipLaunchKernelGGL(k_zero_int, dim3(1), dim3(1), 0, s, wt8_bad);
-  hipLaunchKernelGGL(k_wt_code, dim3(mg_grid(n)), b, 0, s, wt, mg.wt8, n, wt8_bad);
-  int wbad = 1;
-  NK_HIP(hipMemcpyAsync(&wbad, wt8_bad, sizeof(int), hipMemcpyDeviceToHost, s));
-  NK_HIP(hipStreamSynchronize(s));
-  const uint8_t *wtc = wbad ? nullptr : mg.wt8;
+  // the weights as counts, when every one is 1/count
+  const uint8_t *wtc = nullptr;
+  NK_TRY(mg_wt_codes(mg, wt, n, s, &wtc));
   // den inside the fused axhelm (p . A_L p) and the mask inside the
   // assembly, when the conditions for that identity hold (p = beta p + z
   // is continuous and zero where pmask is when pmask is the fine mask)

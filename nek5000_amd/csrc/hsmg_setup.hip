@@ -797,7 +797,7 @@ int mg_setup(Mesh &mesh, const int *fbc_h, const int64_t *vertex, const double *
                        L.e, L.nh, nel);
     NK_HIP(hipGetLastError());
     if (l == mg.lmax) {  // L.e holds the overlap counts here
-      std::vector<double> lut(256, 0.0);
+      std::vector<double> lut(512, 0.0);
       for (int c = 1; c <= 31; ++c)
         for (int h = 0; h <= 7; ++h) {
           double w = 1.0;
@@ -805,6 +805,9 @@ int mg_setup(Mesh &mesh, const int *fbc_h, const int64_t *vertex, const double *
           for (int t = 0; t < h; ++t) w *= r;
           lut[(size_t)(1 + 8 * (c - 1) + h)] = 1.0 * w;
         }
+      // entries 256-511: the same weights rounded to FP32, as the
+      // single-precision smoother's copies hold them
+      for (int c = 0; c < 256; ++c) lut[(size_t)(256 + c)] = (double)(float)lut[(size_t)c];
       NK_TRY(upload(&mg.wlut, lut));
       NK_HIP(hipMalloc(&L.mswt8, (size_t)L.n));
       int *bad = nullptr, hbad = 1;

@@ -88,15 +88,23 @@ __global__ void k_mx_state(MxState *st, double tol) {
   st->tol = tol;
 }
 
+// the FP32 multiplicity weight of node i: the FP32 copy, or the count byte
+// c decoded as the FP32 rounding of 1/c (what the copy holds when wt = 1/c)
+__device__ __forceinline__ float wt32(const float *__restrict__ w, int64_t i) { return w[i]; }
+__device__ __forceinline__ float wt32(const uint8_t *__restrict__ c, int64_t i) {
+  return (float)(1.0 / (double)c[i]);
+}
+
 // partials of sum z*wt*r and sum z*wt*w in FP64 (rho1 and the rho2 term,
 // as the FP64 solver: rho2 = -alpha sum z*wt*w_prev)
+template <typename WT>
 __global__ void __launch_bounds__(256)
     k_mx_dots(const float *__restrict__ z, const float *__restrict__ r,
-              const float *__restrict__ w, const float *__restrict__ wt, int64_t n, double *pa,
+              const float *__restrict__ w, const WT *__restrict__ wt, int64_t n, double *pa,
               double *pb) {
   double a = 0.0, b = 0.0;
   NK_LOOP(i, n) {
-    const double zw = (double)z[i] * (double)wt[i];
+    const double zw = (double)z[i] * (double)wt32(wt, i);
     a += zw * (double)r[i];
     b += zw * (double)w[i];
   }
@@ -125,9 +133,10 @@ __global__ void __launch_bounds__(1024) k_mx_alpha(MxState *st, const double *pa
 }
 
 // x += alpha p; r -= alpha w; partial sum r*r*wt (FP64)
+template <typename WT>
 __global__ void __launch_bounds__(256)
     k_mx_update(float *__restrict__ x, float *__restrict__ r, const float *__restrict__ p,
-                const float *__restrict__ w, const float *__restrict__ wt, const MxState *st,
+                const float *__restrict__ w, const WT *__restrict__ wt, const MxState *st,
                 int64_t n, double *part) {
   // 4 nodes per step with nontemporal hints (the vectors are separate
   // 256-byte aligned allocations), the tail node by node
@@ -139,14 +148,23 @@ __global__ void __launch_bounds__(256)
   v4 *r4 = reinterpret_cast<v4 *>(r);
   const v4 *p4 = reinterpret_cast<const v4 *>(p);
   const v4 *w4 = reinterpret_cast<const v4 *>(w);
-  const v4 *t4 = reinterpret_cast<const v4 *>(wt);
   NK_LOOP(i, n4) {
     __builtin_nontemporal_store(__builtin_nontemporal_load(x4 + i) +
                                     alpha * __builtin_nontemporal_load(p4 + i),
                                 x4 + i);
     const v4 ri = __builtin_nontemporal_load(r4 + i) - alpha * __builtin_nontemporal_load(w4 + i);
     __builtin_nontemporal_store(ri, r4 + i);
-    const v4 t = __builtin_nontemporal_load(t4 + i);
+    v4 t;
+    if constexpr (std::is_same<WT, float>::value) {
+      t = __builtin_nontemporal_load(reinterpret_cast<const v4 *>(wt) + i);
+    } else {  // four count bytes
+      typedef unsigned char u4 __attribute__((ext_vector_type(4)));
+      const u4 c = __builtin_nontemporal_load(reinterpret_cast<const u4 *>(wt) + i);
+      t.x = (float)(1.0 / (double)c.x);
+      t.y = (float)(1.0 / (double)c.y);
+      t.z = (float)(1.0 / (double)c.z);
+      t.w = (float)(1.0 / (double)c.w);
+    }
     a += (double)ri.x * (double)ri.x * (double)t.x + (double)ri.y * (double)ri.y * (double)t.y +
          (double)ri.z * (double)ri.z * (double)t.z + (double)ri.w * (double)ri.w * (double)t.w;
   }
@@ -155,7 +173,7 @@ __global__ void __launch_bounds__(256)
     x[i] = x[i] + alpha * p[i];
     const float ri = r[i] - alpha * w[i];
     r[i] = ri;
-    a += (double)ri * (double)ri * (double)wt[i];
+    a += (double)ri * (double)ri * (double)wt32(wt, i);
   }
   part1(a, part);
 }
@@ -625,6 +643,10 @@ int run_flex_cg_mixed(Mesh &m, double *res, Coef h1, Coef h2, const double *wt,
   const dim3 G(kMxBlocks), B(256);
   NK_TRY(ensure_geom32(m, s));
   hipLaunchKernelGGL(k_mx_to_f32, dim3(mg_grid(n)), B, 0, s, mg.xwt, wt, n);
+  // the weights as count bytes where every one is 1/count (1 B instead of 4
+  // per node in the inner dots and update)
+  const uint8_t *wtc = nullptr;
+  NK_TRY(mg_wt_codes(mg, wt, n, s, &wtc));
   MxState *st = (MxState *)mg.xst;
   double *pa = mg.xpart, *pb = mg.xpart + kMxBlocks, *pc = mg.xpart + 2 * kMxBlocks;
   double *den_part = mg.xpart + 3 * kMxBlocks;
@@ -663,12 +685,20 @@ int run_flex_cg_mixed(Mesh &m, double *res, Coef h1, Coef h2, const double *wt,
       // the inner V-cycles' coarse problem: one AMG V-cycle for param40 = 1,
       // else the coarse AMG-PCG to 0.1 relative (crs_mode 2)
       NK_TRY(mg_solve32(m, mg.xz, mg.xr, s, prm.param40 == 1 ? 1 : 2));
-      hipLaunchKernelGGL(k_mx_dots, G, B, 0, s, mg.xz, mg.xr, mg.xw, mg.xwt, n, pa, pb);
+      if (wtc)
+        hipLaunchKernelGGL(k_mx_dots<uint8_t>, G, B, 0, s, mg.xz, mg.xr, mg.xw, wtc, n, pa, pb);
+      else
+        hipLaunchKernelGGL(k_mx_dots<float>, G, B, 0, s, mg.xz, mg.xr, mg.xw, mg.xwt, n, pa, pb);
       hipLaunchKernelGGL(k_mx_beta, dim3(1), B, 0, s, st, pa, pb);
       NK_TRY(launch_axhelm32_flex(m, mg.xw, mg.xp, mg.xz, h1f, h2f, st, den_part, s));
       NK_TRY(gs_assemble(m.gs, mg.xw, nullptr, s));
       hipLaunchKernelGGL(k_mx_alpha, dim3(1), dim3(1024), 0, s, st, den_part, nden);
-      hipLaunchKernelGGL(k_mx_update, G, B, 0, s, mg.xx, mg.xr, mg.xp, mg.xw, mg.xwt, st, n, pa);
+      if (wtc)
+        hipLaunchKernelGGL(k_mx_update<uint8_t>, G, B, 0, s, mg.xx, mg.xr, mg.xp, mg.xw, wtc, st,
+                           n, pa);
+      else
+        hipLaunchKernelGGL(k_mx_update<float>, G, B, 0, s, mg.xx, mg.xr, mg.xp, mg.xw, mg.xwt,
+                           st, n, pa);
       hipLaunchKernelGGL(k_mx_rnorm, dim3(1), B, 0, s, st, pa, m.vol);
       NK_HIP(hipGetLastError());
       NK_HIP(hipMemcpyAsync(&hs, st, sizeof(MxState), hipMemcpyDeviceToHost, s));

@@ -275,6 +275,8 @@ int gs_fin(const GsMap &gs, float *z, const float *e, const float *w, hipStream_
 // the same with the weights as byte codes into lut
 int gs_fin(const GsMap &gs, double *z, const double *e, const uint8_t *w, const double *lut,
            hipStream_t s);
+int gs_fin(const GsMap &gs, float *z, const float *e, const uint8_t *w, const double *lut,
+           hipStream_t s);
 // Flags per group from mask/mult; *ok when every group has a 0/1 mask equal
 // at all members and mult summing to 1 (red: kRedBlocks scratch).
 int gs_prepare(GsMap &gs, const double *mask, const double *mult, double *red, bool *ok,
