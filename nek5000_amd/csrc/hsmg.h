@@ -45,9 +45,10 @@ struct MgLevel {
   double *e = nullptr, *r = nullptr;
   // single-precision copies for the mixed-precision smoother (fine level)
   float *fs32 = nullptr, *fl32 = nullptr, *wint32 = nullptr, *wgh32 = nullptr, *e32 = nullptr;
-  // fine level: mask*Schwarz weights as byte codes into Mg::wlut, when
-  // every value has one
-  uint8_t *mswt8 = nullptr;
+  // fine level: mask*Schwarz weights as byte codes into Mg::wlut, and
+  // restriction weights as counts (CountCode: weight 1/count), when every
+  // value has one
+  uint8_t *mswt8 = nullptr, *rwt8 = nullptr;
   float *mswt32 = nullptr;  // mask * Schwarz weight, single precision
   float *rwt32 = nullptr;   // restriction weights, single precision
 };

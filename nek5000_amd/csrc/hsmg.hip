@@ -608,7 +608,7 @@ __global__ void k_tnsr(VT *__restrict__ v, const UT *__restrict__ u,
     __syncthreads();
     for (int t = threadIdx.x; t < NU3; t += blockDim.x) {
       double x = double(u[(int64_t)e * NU3 + t]);
-      if constexpr (MODE == 1) x = x * double(w[(int64_t)e * NU3 + t]);
+      if constexpr (MODE == 1) x = x * wdec(w, (int64_t)e * NU3 + t, nullptr);
       su[t] = x;
     }
     __syncthreads();
@@ -632,16 +632,17 @@ __global__ void k_tnsr(VT *__restrict__ v, const UT *__restrict__ u,
       for (int m = 0; m < NU; ++m) s += s2[ij + NV * NV * m] * sA[k + NV * m];
       const int64_t o = (int64_t)e * NV3 + t;
       if constexpr (MODE == 2) {
-        v[o] = VT(double(v[o]) * double(w[o]) + s);
+        v[o] = VT(double(v[o]) * wdec(w, o, nullptr) + s);
       } else if constexpr (MODE == 3) {
         const int i = t % NV, j = (t / NV) % NV, k = t / (NV * NV);
         const bool bnd = (i == 0 || i == NV - 1 || j == 0 || j == NV - 1 || k == 0 || k == NV - 1);
-        v[o] = VT(bnd ? s : double(eb[o]) * double(w[o]) + s);
+        v[o] = VT(bnd ? s : double(eb[o]) * wdec(w, o, nullptr) + s);
       } else if constexpr (MODE == 4) {
         const int i = t % NV, j = (t / NV) % NV, k = t / (NV * NV);
         const bool bnd = (i == 0 || i == NV - 1 || j == 0 || j == NV - 1 || k == 0 || k == NV - 1);
         v[o] = VT(bnd ? s
-                      : double(EO(overlap_value<NV, EB>(eb, eg, e, i, j, k))) * double(w[o]) +
+                      : double(EO(overlap_value<NV, EB>(eb, eg, e, i, j, k))) *
+                                wdec(w, o, nullptr) +
                             s);
       } else {
         v[o] = VT(om ? s * om[o] : s);
@@ -1167,6 +1168,10 @@ static int mg_solve_t(Mesh &mesh, FT *z, const FT *r, hipStream_t s, int crs_mod
     if constexpr (f32io)  // single-precision input: single-precision weights
       NK_TRY((run_tnsr<1, double, FT, double, float>(C.r, C.nh, r, T.nh, T.rwt32, C.Jt, nel,
                                                      s, (const double *)nullptr, C.mask)));
+    else if (T.rwt8)  // the weights 1/count as count bytes
+      NK_TRY((run_tnsr<1, double, FT, double, CountCode>(
+          C.r, C.nh, r, T.nh, reinterpret_cast<const CountCode *>(T.rwt8), C.Jt, nel, s,
+          (const double *)nullptr, C.mask)));
     else
       NK_TRY((run_tnsr<1, double, FT, double>(C.r, C.nh, r, T.nh, T.rwt, C.Jt, nel, s,
                                               (const double *)nullptr, C.mask)));
@@ -1273,8 +1278,9 @@ static int mg_solve_t(Mesh &mesh, FT *z, const FT *r, hipStream_t s, int crs_mod
     } else if (merged) {
       NK_TRY(run_tnsr<3>(z, T.nh, C.e, C.nh, T.mswt, C.J, nel, s, T.e));
       // the final sum reads the weights as byte codes where every one has a
-      // code (1 B instead of 8 per node; the prolongation above keeps the
-      // FP64 weights: a table lookup there cost more than it saved)
+      // code (1 B instead of 8 per node).  In the prolongation above the
+      // codes saved nothing (571 us with the weight decoded by arithmetic
+      // from the code, slower through the table), so it keeps FP64 weights
       if (T.mswt8)
         NK_TRY(gs_fin(mesh.gs, z, T.e, T.mswt8, mg.wlut, s));
       else

@@ -268,11 +268,12 @@ __global__ void k_mask_schwarz_wt(double *mswt, const double *mask, const double
 
 // Byte codes of the fine level's mask * Schwarz weights: 0 where masked,
 // else 1 + 8 (count - 1) + visits, the table entry formed as
-// k_mask_schwarz_wt forms the weight.  Any node whose weight the table does
-// not hold exactly sets *bad.
-__global__ void k_weight_codes(uint8_t *mc, const double *mswt, const double *mask,
-                               const double *cnt, const double *lut, int n, int64_t nel,
-                               int *bad) {
+// k_mask_schwarz_wt forms the weight; and of the restriction weights, the
+// count c with 1/c the weight.  A node whose weight has no exact code sets
+// bit 1 (mask * Schwarz) or bit 2 (restriction) of *bad.
+__global__ void k_weight_codes(uint8_t *mc, uint8_t *rc, const double *mswt, const double *rwt,
+                               const double *mask, const double *cnt, const double *lut, int n,
+                               int64_t nel, int *bad) {
   const int64_t n3 = (int64_t)n * n * n;
   int b = 0;
   for (int64_t l = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; l < n3 * nel;
@@ -289,9 +290,15 @@ __global__ void k_weight_codes(uint8_t *mc, const double *mswt, const double *ma
       else b = 1;
     }
     mc[l] = (uint8_t)code;
-    if (lut[code] != mswt[l]) b = 1;
+    if (lut[code] != mswt[l]) b |= 1;
+    // restriction weight: 1/count
+    const double rw = rwt[l];
+    const double rr = rw > 0.0 ? rint(1.0 / rw) : 0.0;
+    const int rcode = (rr >= 1.0 && rr <= 255.0) ? (int)rr : 0;
+    rc[l] = (uint8_t)rcode;
+    if (rcode == 0 || 1.0 / (double)rcode != rw) b |= 2;
   }
-  if (b) atomicOr(bad, 1);
+  if (b) atomicOr(bad, b);
 }
 
 // 1 at ghost slots that have a neighbour (the overlap count of a field of
@@ -552,7 +559,8 @@ static void free_level(MgLevel &L) {
     if (p) (void)hipFree(p);
   for (float *p : {L.fs32, L.fl32, L.wint32, L.wgh32, L.e32, L.mswt32, L.rwt32})
     if (p) (void)hipFree(p);
-  if (L.mswt8) (void)hipFree(L.mswt8);
+  for (uint8_t *p : {L.mswt8, L.rwt8})
+    if (p) (void)hipFree(p);
   if (L.own_gs) gs_free(L.gs);
 }
 
@@ -813,13 +821,18 @@ int mg_setup(Mesh &mesh, const int *fbc_h, const int64_t *vertex, const double *
       int *bad = nullptr, hbad = 1;
       NK_HIP(hipMalloc(&bad, sizeof(int)));
       NK_HIP(hipMemsetAsync(bad, 0, sizeof(int), s));
-      hipLaunchKernelGGL(k_weight_codes, dim3(mg_grid(L.n)), dim3(256), 0, s, L.mswt8, L.mswt,
-                         L.mask, L.e, mg.wlut, L.nh, nel, bad);
+      NK_HIP(hipMalloc(&L.rwt8, (size_t)L.n));
+      hipLaunchKernelGGL(k_weight_codes, dim3(mg_grid(L.n)), dim3(256), 0, s, L.mswt8, L.rwt8,
+                         L.mswt, L.rwt, L.mask, L.e, mg.wlut, L.nh, nel, bad);
       NK_HIP(hipGetLastError());
       NK_HIP(hipMemcpyAsync(&hbad, bad, sizeof(int), hipMemcpyDeviceToHost, s));
       NK_HIP(hipStreamSynchronize(s));
       (void)hipFree(bad);
-      if (hbad) {
+      if (hbad & 2) {
+        (void)hipFree(L.rwt8);
+        L.rwt8 = nullptr;
+      }
+      if (hbad & 1) {
         (void)hipFree(L.mswt8);
         L.mswt8 = nullptr;
       }

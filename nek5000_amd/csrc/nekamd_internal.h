@@ -184,6 +184,13 @@ __device__ __forceinline__ double wdec(const float *w, int64_t i, const double *
 __device__ __forceinline__ double wdec(const uint8_t *w, int64_t i, const double *lut) {
   return lut[w[i]];
 }
+// a byte code decoded by arithmetic, without a table load: the weight 1/c
+struct CountCode {
+  uint8_t c;
+};
+__device__ __forceinline__ double wdec(const CountCode *w, int64_t i, const double *) {
+  return 1.0 / (double)w[i].c;
+}
 
 inline int axhelm_cg_grid(const Mesh &m) {
   const int64_t g = (m.nel + elems_per_block(m.np) - 1) / elems_per_block(m.np);
