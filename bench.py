@@ -16,8 +16,8 @@ fine-level local FDM solves, the flexible-CG axhelm or the assembly, with
 its algorithmic bytes per launch; "kernels" lists all three.
 cpu_baseline: the oracle (C, OpenMP over all host threads) solving the same
 operator with the same solver on a bounded sample (E = 16^3).
-north_star: the mixed-precision solve's time to solution against FP64's,
-the five-solve batch (mixed, batched) against five FP64 solves one after
+targets: the mixed-precision solve's time to solution against FP64's, the
+five-solve batch (mixed, batched) against five FP64 solves one after
 another, and the adjoint gradient's finite-difference check.
 """
 import argparse
@@ -678,19 +678,19 @@ def main():
         traffic = measured_traffic("sch_fdm_E262144_N7_pmc.md" if "sch_fdm" in dom["kernel"]
                                    else "axhelm_flexcg_E262144_N7_pmc.md")
 
-    north = None
+    targets = None
     if not args.no_variants and hsmg and args.precision == 0:
-        # the comparisons the north star states, recorded with the bench line
+        # the stated targets of this workload, recorded with the bench line
         mixed = [v for v in variants if v["variant"].startswith("mixed: FP32 inner flexible CG")]
         F.destroy()
         del rhs0, rhss, h1, h2, bm1
         torch.cuda.empty_cache()
-        log("[bench] north star: batch ...")
+        log("[bench] targets: batch ...")
         bl = batch_run()
         bv = bl["variants"]
-        log("[bench] north star: adjoint ...")
+        log("[bench] targets: adjoint ...")
         al = adjoint_run()
-        north = {
+        targets = {
             "mixed_time_to_solution_vs_fp64": mixed[0]["time_to_solution_vs_fp64"] if mixed else None,
             "mixed_value": mixed[0]["value"] if mixed else None,
             "mixed_final_rel_residual": mixed[0]["final_rel_residual"] if mixed else None,
@@ -710,7 +710,7 @@ def main():
                                            "fd_max_rel_diff", "forward_ms", "adjoint_ms",
                                            "gradient_kernel_ms", "gradient_kernel_GBs")},
         }
-        log(f"[bench] north star {north}")
+        log(f"[bench] targets {targets}")
 
     cpu = None
     if not args.no_cpu:
@@ -763,7 +763,7 @@ def main():
             "bytes_per_launch": dom["bytes_per_launch"],
         },
         "cpu_baseline": cpu,
-        "north_star": north,
+        "targets": targets,
     }
     print(json.dumps(line), flush=True)
 
