@@ -211,3 +211,40 @@ def test_tuning_arguments_checked(gpu):
         gpu.set_axhelm_tuning(8, 0, -1)
     with pytest.raises(gpu.NekAmdError):
         gpu.axhelm_tuning(1)
+
+
+@pytest.mark.parametrize("variant", [4, 5])
+def test_gated_apply_matrix_core_variants(gpu, variant):
+    """The gated apply (the mixed Jacobi PCG's FP64 residual refreshes) on the
+    matrix cores: the mixed solve's true FP64 residual, recomputed with the
+    oracle's operator, meets the tolerance, in about the FP64 solve's
+    iteration count."""
+    lx1 = 10
+    b = box((3, 3, 2), lx1, deform=0.1)
+    P = Problem(b)
+    F = gpu.box_field(b)
+    h1v, h2v, tli = 0.7, 20.0, -1e-9
+    rhs = P.bm1 * P.uniform_field(31)
+    h1, h2 = F.zeros() + h1v, F.zeros() + h2v
+    saved = gpu.axhelm_tuning(lx1)
+    outs = {}
+    try:
+        for tag, v, prec in (("fp64", 0, 0), ("mixed", variant, 2)):
+            gpu.set_axhelm_tuning(lx1, v, 0)
+            say(f"[gated variants] lx1={lx1} {tag} variant={v}")
+            u = F.zeros()
+            r = F.to_dev(rhs)
+            prm = gpu.default_params(istep=20, param22=0.0, restol=0.0, precision=prec)
+            out = F.hmholtz("VELX", u, r, h1, h2, F.mask, F.mult, tli, 900, params=prm)
+            outs[tag] = (host(u), out, host(r))
+    finally:
+        gpu.set_axhelm_tuning(lx1, saved[0], saved[1])
+    u64, o64, f = outs["fp64"]
+    u32, o32, _ = outs["mixed"]
+    assert o64["converged"] and o32["converged"]
+    assert o32["niter"] <= int(1.1 * o64["niter"]) + 3
+    au = P.field.axhelm(u32, np.full(P.shape, h1v), np.full(P.shape, h2v))
+    res = f - P.mask * P.field.dssum(au)
+    rt = float(np.sqrt(np.sum(P.field.binv * P.mult * res * res) / P.field.vol))
+    assert rt <= o32["tol"] * (1 + 1e-6)
+    F.destroy()
