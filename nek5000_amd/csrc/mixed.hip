@@ -490,10 +490,12 @@ __global__ void __launch_bounds__(mx_threads<NP>())
 // Kernel of the FP32 operator per order: 0 vector FMA (k_axhelm32), 1 the
 // matrix cores (k_axhelm32_mxp, lx1 = 8..16); nkh_axhelm32_set_variant fixes
 // it.  Defaults from 100 mixed Jacobi-PCG iterations per kernel on boxes of
-// about 5e7 unique DOF (MI355X, GDOF*iter/s vector -> matrix cores): lx1 = 11
-// 30.3 -> 32.0, 14 26.2 -> 27.3, 15 25.4 -> 28.1; the other orders are
-// within 2 % or faster with vector FMA (lx1 = 12: 32.3 -> 29.0).
-int g_ax32_variant[kMaxNP + 1] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 1, 0, 0, 1, 1, 0};
+// about 5e7 unique DOF (MI355X, GDOF*iter/s vector -> matrix cores): lx1 = 8
+// 30.9 -> 31.6 (and the mixed flexible CG at E = 64^3 122.9-123.3 -> 122.1-
+// 122.8 ms per solve), 11 30.3 -> 32.0, 14 26.2 -> 27.3, 15 25.4 -> 28.1;
+// the other orders are faster with vector FMA or within 1 % (lx1 = 12:
+// 32.3 -> 29.0).
+int g_ax32_variant[kMaxNP + 1] = {0, 0, 0, 0, 0, 0, 0, 0, 1, 0, 0, 1, 0, 0, 1, 1, 0};
 
 template <int NP, bool JAC>
 static int axhelm32_dispatch(const Mesh &m, float *w, float *p, const float *z, const float *dd,
