@@ -185,10 +185,11 @@ int nkd_axhelm(int mesh, double *au, const double *u, const double *h1, const do
    products (v_mfma_f64_16x16x4_f64, lx1 = 8-16; elsewhere they run 0).
    Variants 0-3 give the same plain apply bit for bit, as do 4 and 5, the two
    sets agreeing to rounding; the fused CG applies of 2 and 3 group their
-   partial sums differently (iterates equal to rounding).  grid = blocks of the plain
-   apply (0: one per element group).  Unless set, the first nkd_axhelm or
-   solve of an order on a mesh of >= 2^20 nodes times the order's variants
-   at two grids and keeps the fastest (*tuned = 1 afterwards). */
+   partial sums differently (iterates equal to rounding).  grid = blocks of
+   the plain apply (0: one per element group).  Unless set, the first
+   nkd_axhelm or solve of an order on a mesh of >= 2^20 nodes times the
+   order's variants at two grids and keeps the fastest (*tuned = 1
+   afterwards). */
 int nkh_axhelm_tuning(int lx1, int *variant, int *grid, int *tuned);
 /* sets the plain, CG-fused and flexible-CG-fused applies to variant */
 int nkh_axhelm_set_tuning(int lx1, int variant, int grid);
