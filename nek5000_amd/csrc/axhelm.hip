@@ -88,8 +88,12 @@ static int tune_axhelm(const Mesh &m, double *au, const double *u, Coef h1, Coef
     pick.variant_cg = vars[bc];
     for (int c = 0; c < nv; ++c) {
       ch.variant_flex = vars[c];
-      NK_TRY(timed([&] { return launch_axhelm_flex(m, w, p, r, h1, h2, &st->beta, rho, s); },
-                   &ms[c]));
+      // with the solution update the flexible CG fuses into this apply (d
+      // stands in for x; alpha is 0 in the zeroed state)
+      auto flex = [&] {
+        return launch_axhelm_flex(m, w, p, r, h1, h2, &st->beta, rho, s, d, &st->alpha);
+      };
+      NK_TRY(timed(flex, &ms[c]));
     }
     bc = 0;
     for (int c = 1; c < nv; ++c)
